@@ -1,0 +1,101 @@
+/*
+ * gcoracle — CPU FP64 restatement of the galaxy-collision force path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.  The product path (libgcsim) never links or calls it.
+ *
+ * Every function restates the reference's arithmetic in the same FP64
+ * operation order (compiled with -ffp-contract=off), so that results are
+ * bit-identical to the reference on the same inputs.  Arrays are
+ * particle-major: pos/vel/acc are n x 3 (x,y,z interleaved), mass is n.
+ *
+ * Parity pins (all exercised by tests/test_oracle_*.py):
+ *   direct (symmetric + targets)  — reference golden snapshots/diagnostics
+ *                                   (direct_{250,500,1000}), closed-form
+ *                                   two-body, momentum conservation,
+ *                                   oracle/_ref on random inputs.
+ *   kick/drift/leapfrog           — golden snapshot_000020 (20 KDK steps),
+ *                                   time reversibility, circular orbit.
+ *   diagnostics                   — golden diagnostics.csv rows.
+ *   P2M/M2M                       — M2M(P2M(children)) == P2M(all),
+ *                                   brute-force monomial sums.
+ *   M2P                           — p=0 limit == point-mass force,
+ *                                   error falls with p, _ref per call.
+ *   FMM                           — golden readme snapshot_000000 FMM
+ *                                   accelerations (bit-level), smoke-test
+ *                                   80-particle threshold, _ref.
+ */
+#ifndef GCORACLE_H
+#define GCORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- direct all-pairs (cpp/direct/direct_solver.cpp) ---------------- */
+void orc_softened_acceleration(const double target[3], const double source[3],
+                               double source_mass, double G, double eps,
+                               double out[3]);
+void orc_direct_symmetric(size_t n, const double* pos, const double* mass,
+                          double G, double eps, double* acc);
+void orc_direct_targets(size_t n, const double* pos, const double* mass,
+                        double G, double eps, size_t begin, size_t end,
+                        double* acc);
+/* Same per-target arithmetic as orc_direct_targets, targets split over
+ * OpenMP threads (bit-identical result).  Returns threads used. */
+int orc_direct_targets_omp(size_t n, const double* pos, const double* mass,
+                           double G, double eps, size_t begin, size_t end,
+                           double* acc, int threads);
+
+/* ---- integrator (cpp/core/integrator.cpp) --------------------------- */
+void orc_kick(size_t n, double* vel, const double* acc, double dt);
+void orc_drift(size_t n, double* pos, const double* vel, double dt);
+
+/* solver: 0 = direct symmetric, 1 = direct targets, 2 = fmm */
+typedef struct {
+    int solver;
+    double G, eps;
+    double theta;
+    size_t leaf_capacity;
+    int max_depth;
+    int expansion_order;
+} orc_solver;
+
+int orc_compute_accelerations(const orc_solver* s, size_t n, const double* pos,
+                              const double* mass, double* acc);
+/* `steps` kick-drift-kick steps; acc must hold a(pos) on entry. */
+int orc_leapfrog(const orc_solver* s, size_t n, double* pos, double* vel,
+                 double* acc, const double* mass, double dt, int steps);
+
+/* ---- diagnostics (cpp/core/diagnostics.cpp) ------------------------- */
+/* out[16]: total_mass, KE, PE, E, P[3], COM[3], L[3], (pad) */
+void orc_diagnostics(size_t n, const double* pos, const double* vel,
+                     const double* mass, double G, double eps, double* out);
+
+/* ---- Cartesian multipoles (cpp/fmm/multipole.cpp) ------------------- */
+void orc_multipole_exponents(int* out /* 35*3 */);
+void orc_p2m(double* moments /*35*/, const double offset[3], double mass);
+void orc_m2m(double* parent, const double* child, const double offset[3],
+             double child_mass);
+void orc_m2p(const double target[3], const double com[3], double mass,
+             const double* moments, double G, double eps, int order,
+             double out[3]);
+
+/* ---- FMM (cpp/fmm/fmm_solver.cpp) ----------------------------------- */
+typedef struct {
+    size_t node_count, leaf_count, far_interactions, near_interactions;
+    int max_depth_reached;
+} orc_fmm_stats;
+
+int orc_fmm(size_t n, const double* pos, const double* mass, double G,
+            double eps, double theta, size_t leaf_capacity, int max_depth,
+            int expansion_order, size_t begin, size_t end, double* acc,
+            orc_fmm_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
