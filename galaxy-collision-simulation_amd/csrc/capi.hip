@@ -1,0 +1,485 @@
+// capi.hip — the gcsim C ABI (include/gcsim.h): contexts owning
+// device-resident SoA particle state, the hip-direct force paths (fp64 and
+// mixed), KDK stepping and diagnostics.  No exception crosses the ABI; every
+// HIP error becomes GC_ERR_HIP with a message from gc_last_error.
+#include "gc_internal.hpp"
+#include "gc_fmm.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+using namespace gc;
+
+struct gc_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    gc_params params{};
+    DeviceState s;
+    MixedLayout mixed;
+    LaunchShape shape;
+    double* stage = nullptr;  // 3 * cap doubles: interleaved host transfers
+    double* diag_scratch = nullptr;
+    size_t diag_scratch_doubles = 0;
+    double* diag_out = nullptr;  // 13 doubles
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    gc::FmmEngine* fmm = nullptr;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_thread_err;
+
+gc_status fail(gc_ctx* c, gc_status code, const std::string& msg) {
+    if (c) c->err = msg;
+    g_thread_err = msg;
+    return code;
+}
+
+#define GC_HIP(ctx, call, what)                                                                  \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,      \
+                        std::string(what) + ": " + hipGetErrorString(e_));                      \
+        }                                                                                        \
+    } while (0)
+
+bool is_gfx950(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return false;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+gc_status check_params(gc_ctx* c, const gc_params* p) {
+    if (!p) return fail(c, GC_ERR_INVALID_ARGUMENT, "params: null");
+    if (!(p->softening >= 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "softening must be non-negative");
+    if (p->solver != GC_SOLVER_DIRECT && p->solver != GC_SOLVER_FMM)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "unknown solver kind");
+    if (p->precision != GC_PRECISION_FP64 && p->precision != GC_PRECISION_MIXED)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "unknown precision");
+    return GC_OK;
+}
+
+void free_state(gc_ctx* c) {
+    (void)hipFree(c->s.base);
+    (void)hipFree(c->mixed.jp);
+    (void)hipFree(c->mixed.centre);
+    (void)hipFree(c->mixed.part);
+    (void)hipFree(c->stage);
+    (void)hipFree(c->diag_scratch);
+    c->s = DeviceState{};
+    c->mixed = MixedLayout{};
+    c->stage = nullptr;
+    c->diag_scratch = nullptr;
+    c->diag_scratch_doubles = 0;
+}
+
+gc_status ensure_capacity(gc_ctx* c, size_t n) {
+    const size_t cap = std::max<size_t>(round_up(std::max<size_t>(n, 1), kTile), kTile);
+    if (cap <= c->s.cap) {
+        c->s.n = n;
+        c->mixed.ntiles = (n + kTile - 1) / kTile;
+        return GC_OK;
+    }
+    GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize before reallocation");
+    free_state(c);
+    GC_HIP(c, hipMalloc(&c->s.base, sizeof(double) * 10 * cap), "allocate particle state");
+    double* b = c->s.base;
+    c->s.x = b;
+    c->s.y = b + cap;
+    c->s.z = b + 2 * cap;
+    c->s.vx = b + 3 * cap;
+    c->s.vy = b + 4 * cap;
+    c->s.vz = b + 5 * cap;
+    c->s.ax = b + 6 * cap;
+    c->s.ay = b + 7 * cap;
+    c->s.az = b + 8 * cap;
+    c->s.m = b + 9 * cap;
+    c->s.cap = cap;
+    c->s.n = n;
+    const size_t ntiles_cap = cap / kTile;
+    GC_HIP(c, hipMalloc(&c->mixed.jp, sizeof(float4) * cap), "allocate mixed source tiles");
+    GC_HIP(c, hipMalloc(&c->mixed.centre, sizeof(double) * 3 * ntiles_cap), "allocate tile centres");
+    c->mixed.ntiles = (n + kTile - 1) / kTile;
+    GC_HIP(c, hipMalloc(&c->stage, sizeof(double) * 3 * cap), "allocate staging buffer");
+    c->diag_scratch_doubles = diagnostics_scratch_doubles(cap);
+    GC_HIP(c, hipMalloc(&c->diag_scratch, sizeof(double) * c->diag_scratch_doubles),
+           "allocate diagnostics scratch");
+    GC_HIP(c, hipMemsetAsync(c->s.base, 0, sizeof(double) * 10 * cap, c->stream), "clear state");
+    return GC_OK;
+}
+
+// Mixed-kernel launch shape: enough blocks for >= 8 waves per SIMD on 256
+// CUs (2048 blocks of 4 waves), split over source tile ranges.
+void mixed_shape(const gc_ctx* c, size_t ntargets, int* ipt_out, int* jsplit_out) {
+    int ipt = c->shape.ipt;
+    if (ipt != 1 && ipt != 2 && ipt != 4) ipt = 1;
+    const size_t span = static_cast<size_t>(kTile) * ipt;
+    const size_t iblocks = std::max<size_t>(1, (ntargets + span - 1) / span);
+    const int ntiles = static_cast<int>(std::max<size_t>(1, c->mixed.ntiles));
+    int js = c->shape.jsplit;
+    if (js <= 0) js = static_cast<int>(std::min<size_t>(64, (2048 + iblocks - 1) / iblocks));
+    js = std::max(1, std::min(js, ntiles));
+    const int tps = (ntiles + js - 1) / js;
+    *ipt_out = ipt;
+    *jsplit_out = (ntiles + tps - 1) / tps;
+}
+
+gc_status ensure_partials(gc_ctx* c, int jsplit) {
+    const size_t need = static_cast<size_t>(jsplit) * 6 * c->s.cap;
+    if (need <= c->mixed.part_floats) return GC_OK;
+    GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize before partials reallocation");
+    (void)hipFree(c->mixed.part);
+    c->mixed.part = nullptr;
+    c->mixed.part_floats = 0;
+    GC_HIP(c, hipMalloc(&c->mixed.part, sizeof(float) * need), "allocate mixed partial sums");
+    c->mixed.part_floats = need;
+    return GC_OK;
+}
+
+gc_status launch_status(gc_ctx* c, const char* what) {
+    GC_HIP(c, hipGetLastError(), what);
+    return GC_OK;
+}
+
+// a(x) for targets [begin, end); optionally the closing half kick.
+gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double half_dt) {
+    const gc_params& p = c->params;
+    if (p.solver == GC_SOLVER_FMM) {
+        if (!c->fmm) c->fmm = new (std::nothrow) gc::FmmEngine();
+        if (!c->fmm) return fail(c, GC_ERR_OUT_OF_MEMORY, "allocate FMM engine");
+        std::string msg;
+        const int rc = c->fmm->compute(c->s, p, begin, end, close_kick, half_dt, c->stream, &msg);
+        if (rc != GC_OK) return fail(c, static_cast<gc_status>(rc), "hip-fmm: " + msg);
+        return GC_OK;
+    }
+    if (p.precision == GC_PRECISION_FP64) {
+        launch_direct_fp64(c->s, p.gravitational_constant, p.softening, begin, end, close_kick,
+                           half_dt, c->stream);
+        return launch_status(c, "launch FP64 direct kernel");
+    }
+    int ipt = 1, js = 1;
+    mixed_shape(c, end - begin, &ipt, &js);
+    gc_status st = ensure_partials(c, js);
+    if (st != GC_OK) return st;
+    launch_pack_mixed(c->s, c->mixed, p.gravitational_constant, c->stream);
+    launch_direct_mixed(c->s, c->mixed, p.softening, ipt, js, begin, end, c->stream);
+    launch_merge_mixed(c->s, c->mixed, js, begin, end, close_kick, half_dt, c->stream);
+    return launch_status(c, "launch mixed direct kernels");
+}
+
+gc_status need_particles(gc_ctx* c) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    GC_HIP(c, hipSetDevice(c->device), "select device");
+    return GC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void gc_default_params(gc_params* p) {
+    if (!p) return;
+    p->gravitational_constant = 1.0;
+    p->softening = 0.02;
+    p->solver = GC_SOLVER_DIRECT;
+    p->precision = GC_PRECISION_FP64;
+    p->theta = 0.6;
+    p->leaf_capacity = 16;
+    p->max_depth = 32;
+    p->expansion_order = 4;
+}
+
+int gc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int gc_solver_available(void) {
+    const int n = gc_device_count();
+    for (int d = 0; d < n; ++d)
+        if (is_gfx950(d)) return 1;
+    return 0;
+}
+
+const char* gc_status_string(gc_status s) {
+    switch (s) {
+    case GC_OK: return "ok";
+    case GC_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case GC_ERR_NO_DEVICE: return "no gfx950 HIP device";
+    case GC_ERR_HIP: return "HIP runtime error";
+    case GC_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case GC_ERR_UNSUPPORTED: return "unsupported";
+    case GC_ERR_STATE: return "invalid state";
+    }
+    return "unknown status";
+}
+
+const char* gc_last_error(const gc_ctx* c) { return c ? c->err.c_str() : g_thread_err.c_str(); }
+
+gc_status gc_create(gc_ctx** out, int device, const gc_params* p) {
+    if (!out) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "gc_create: null output");
+    *out = nullptr;
+    gc_status st = check_params(nullptr, p);
+    if (st != GC_OK) return st;
+    const int count = gc_device_count();
+    if (count <= 0) return fail(nullptr, GC_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count)
+        return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    if (!is_gfx950(device))
+        return fail(nullptr, GC_ERR_NO_DEVICE, "device is not gfx950 (MI355X); kernels are built for gfx950 only");
+    gc_ctx* c = new (std::nothrow) gc_ctx();
+    if (!c) return fail(nullptr, GC_ERR_OUT_OF_MEMORY, "allocate context");
+    c->device = device;
+    c->params = *p;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e == hipSuccess) e = hipMalloc(&c->diag_out, sizeof(double) * 16);
+    if (e != hipSuccess) {
+        std::string msg = std::string("create context: ") + hipGetErrorString(e);
+        gc_destroy(c);
+        return fail(nullptr, GC_ERR_HIP, msg);
+    }
+    c->stream = c->own_stream;
+    *out = c;
+    return GC_OK;
+}
+
+void gc_destroy(gc_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c->fmm;
+    free_state(c);
+    (void)hipFree(c->diag_out);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+gc_status gc_set_params(gc_ctx* c, const gc_params* p) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    gc_status st = check_params(c, p);
+    if (st != GC_OK) return st;
+    c->params = *p;
+    return GC_OK;
+}
+
+gc_status gc_set_stream(gc_ctx* c, void* stream) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    return GC_OK;
+}
+
+void* gc_get_stream(gc_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
+
+gc_status gc_upload(gc_ctx* c, size_t n, const double* pos, const double* vel,
+                    const double* mass) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (n > 0 && (!pos || !mass)) return fail(c, GC_ERR_INVALID_ARGUMENT, "upload: null positions or masses");
+    if (n > static_cast<size_t>(1) << 30) return fail(c, GC_ERR_INVALID_ARGUMENT, "upload: particle count exceeds 2^30");
+    st = ensure_capacity(c, n);
+    if (st != GC_OK) return st;
+    if (n == 0) return GC_OK;
+    const size_t b3 = sizeof(double) * 3 * n;
+    GC_HIP(c, hipMemcpyAsync(c->stage, pos, b3, hipMemcpyHostToDevice, c->stream), "upload positions");
+    launch_scatter_xyz(c->stage, c->s.x, c->s.y, c->s.z, n, c->stream);
+    // the staging buffer is reused: order the copies on the stream
+    GC_HIP(c, hipStreamSynchronize(c->stream), "upload positions");
+    if (vel) {
+        GC_HIP(c, hipMemcpyAsync(c->stage, vel, b3, hipMemcpyHostToDevice, c->stream), "upload velocities");
+        launch_scatter_xyz(c->stage, c->s.vx, c->s.vy, c->s.vz, n, c->stream);
+    } else {
+        GC_HIP(c, hipMemsetAsync(c->s.vx, 0, sizeof(double) * 3 * c->s.cap, c->stream), "clear velocities");
+    }
+    GC_HIP(c, hipMemcpyAsync(c->s.m, mass, sizeof(double) * n, hipMemcpyHostToDevice, c->stream), "upload masses");
+    GC_HIP(c, hipMemsetAsync(c->s.ax, 0, sizeof(double) * 3 * c->s.cap, c->stream), "clear accelerations");
+    GC_HIP(c, hipStreamSynchronize(c->stream), "upload");
+    return launch_status(c, "upload kernels");
+}
+
+gc_status gc_set_accelerations(gc_ctx* c, const double* acc) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (!acc) return fail(c, GC_ERR_INVALID_ARGUMENT, "set_accelerations: null array");
+    const size_t n = c->s.n;
+    if (n == 0) return GC_OK;
+    GC_HIP(c, hipMemcpyAsync(c->stage, acc, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream),
+           "upload accelerations");
+    launch_scatter_xyz(c->stage, c->s.ax, c->s.ay, c->s.az, n, c->stream);
+    GC_HIP(c, hipStreamSynchronize(c->stream), "upload accelerations");
+    return launch_status(c, "upload accelerations");
+}
+
+gc_status gc_download(gc_ctx* c, double* pos, double* vel, double* acc) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    const size_t n = c->s.n;
+    if (n == 0) return GC_OK;
+    const size_t b3 = sizeof(double) * 3 * n;
+    struct {
+        double* host;
+        double *x, *y, *z;
+    } items[3] = {{pos, c->s.x, c->s.y, c->s.z}, {vel, c->s.vx, c->s.vy, c->s.vz},
+                  {acc, c->s.ax, c->s.ay, c->s.az}};
+    for (auto& it : items) {
+        if (!it.host) continue;
+        launch_gather_xyz(it.x, it.y, it.z, c->stage, n, c->stream);
+        GC_HIP(c, hipMemcpyAsync(it.host, c->stage, b3, hipMemcpyDeviceToHost, c->stream), "download");
+        GC_HIP(c, hipStreamSynchronize(c->stream), "download");
+    }
+    return launch_status(c, "download kernels");
+}
+
+size_t gc_particle_count(const gc_ctx* c) { return c ? c->s.n : 0; }
+
+gc_status gc_device_state(gc_ctx* c, gc_device_view* v) {
+    if (!c || !v) return fail(c, GC_ERR_INVALID_ARGUMENT, "device_state: null argument");
+    v->n = c->s.n;
+    v->x = c->s.x;
+    v->y = c->s.y;
+    v->z = c->s.z;
+    v->vx = c->s.vx;
+    v->vy = c->s.vy;
+    v->vz = c->s.vz;
+    v->ax = c->s.ax;
+    v->ay = c->s.ay;
+    v->az = c->s.az;
+    v->mass = c->s.m;
+    v->stream = c->stream;
+    return GC_OK;
+}
+
+gc_status gc_compute_accelerations(gc_ctx* c) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (c->s.n == 0) return GC_OK;
+    return force(c, 0, c->s.n, false, 0.0);
+}
+
+gc_status gc_compute_accelerations_for_targets(gc_ctx* c, size_t begin, size_t end) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    end = std::min(end, c->s.n);
+    if (begin >= end) return GC_OK;
+    return force(c, begin, end, false, 0.0);
+}
+
+gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (!(dt > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "dt must be positive");
+    if (nsteps < 0) return fail(c, GC_ERR_INVALID_ARGUMENT, "steps must be non-negative");
+    if (c->s.n == 0) return GC_OK;
+    const double half = 0.5 * dt;
+    for (int k = 0; k < nsteps; ++k) {
+        launch_open_step(c->s, half, dt, c->stream);
+        st = force(c, 0, c->s.n, true, half);
+        if (st != GC_OK) return st;
+    }
+    return launch_status(c, "launch step kernels");
+}
+
+gc_status gc_synchronize(gc_ctx* c) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize");
+    return GC_OK;
+}
+
+gc_status gc_diagnostics_compute(gc_ctx* c, gc_diagnostics* out) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (!out) return fail(c, GC_ERR_INVALID_ARGUMENT, "diagnostics: null output");
+    std::memset(out, 0, sizeof(*out));
+    if (c->s.n == 0) return GC_OK;
+    launch_diagnostics(c->s, c->params.gravitational_constant, c->params.softening,
+                       c->diag_scratch, c->diag_scratch_doubles, c->diag_out, c->stream);
+    st = launch_status(c, "launch diagnostics kernels");
+    if (st != GC_OK) return st;
+    double host[13];
+    GC_HIP(c, hipMemcpyAsync(host, c->diag_out, sizeof(host), hipMemcpyDeviceToHost, c->stream), "read diagnostics");
+    GC_HIP(c, hipStreamSynchronize(c->stream), "diagnostics");
+    out->kinetic_energy = host[0];
+    out->potential_energy = host[1];
+    out->total_energy = host[2];
+    out->total_mass = host[3];
+    for (int k = 0; k < 3; ++k) {
+        out->momentum[k] = host[4 + k];
+        out->center_of_mass[k] = host[7 + k];
+        out->angular_momentum[k] = host[10 + k];
+    }
+    return GC_OK;
+}
+
+gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (reps <= 0 || !ms) return fail(c, GC_ERR_INVALID_ARGUMENT, "time_force_kernel: bad arguments");
+    const size_t n = c->s.n;
+    if (n == 0) return fail(c, GC_ERR_STATE, "time_force_kernel: no particles");
+    if (c->params.solver == GC_SOLVER_FMM) {
+        if (!c->fmm) c->fmm = new (std::nothrow) gc::FmmEngine();
+        if (!c->fmm) return fail(c, GC_ERR_OUT_OF_MEMORY, "allocate FMM engine");
+        std::string msg;
+        const int rc = c->fmm->time_kernels(c->s, c->params, reps, c->stream, ms, inter, &msg);
+        if (rc != GC_OK) return fail(c, static_cast<gc_status>(rc), "hip-fmm timing: " + msg);
+        return GC_OK;
+    }
+    int ipt = 1, js = 1;
+    if (c->params.precision == GC_PRECISION_MIXED) {
+        mixed_shape(c, n, &ipt, &js);
+        st = ensure_partials(c, js);
+        if (st != GC_OK) return st;
+        launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
+    }
+    GC_HIP(c, hipEventRecord(c->ev0, c->stream), "record start event");
+    for (int r = 0; r < reps; ++r) {
+        if (c->params.precision == GC_PRECISION_MIXED) {
+            launch_direct_mixed(c->s, c->mixed, c->params.softening, ipt, js, 0, n, c->stream);
+        } else {
+            launch_direct_fp64(c->s, c->params.gravitational_constant, c->params.softening, 0, n,
+                               false, 0.0, c->stream);
+        }
+    }
+    GC_HIP(c, hipEventRecord(c->ev1, c->stream), "record stop event");
+    GC_HIP(c, hipEventSynchronize(c->ev1), "wait stop event");
+    st = launch_status(c, "timed force kernels");
+    if (st != GC_OK) return st;
+    float total = 0.f;
+    GC_HIP(c, hipEventElapsedTime(&total, c->ev0, c->ev1), "elapsed time");
+    *ms = static_cast<double>(total) / reps;
+    if (inter) *inter = static_cast<double>(n) * static_cast<double>(n - 1);
+    if (c->params.precision == GC_PRECISION_MIXED) {
+        // restore accelerations consistent with the state
+        launch_merge_mixed(c->s, c->mixed, js, 0, n, false, 0.0, c->stream);
+    }
+    return launch_status(c, "timed force kernels");
+}
+
+gc_status gc_set_launch_shape(gc_ctx* c, int ipt, int jsplit) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (ipt != 0 && ipt != 1 && ipt != 2 && ipt != 4)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2 or 4");
+    if (jsplit < 0 || jsplit > 1024) return fail(c, GC_ERR_INVALID_ARGUMENT, "jsplit out of range");
+    c->shape.ipt = ipt;
+    c->shape.jsplit = jsplit;
+    return GC_OK;
+}
+
+gc_status gc_get_launch_shape(gc_ctx* c, int* ipt, int* jsplit) {
+    if (!c || !ipt || !jsplit) return fail(c, GC_ERR_INVALID_ARGUMENT, "get_launch_shape: null argument");
+    mixed_shape(c, c->s.n, ipt, jsplit);
+    return GC_OK;
+}
+
+}  // extern "C"

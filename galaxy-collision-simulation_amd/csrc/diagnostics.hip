@@ -1,0 +1,185 @@
+// diagnostics.hip — FP64 conservation diagnostics of the resident state
+// (cpp/core/diagnostics.cpp:7-37) on the device.
+//
+//   linear pass  : total mass, KE = sum m|v|^2/2, P = sum m v, sum m x,
+//                  L = sum m (x cross v); grid-stride lanes, wavefront
+//                  (DPP/shuffle) reductions, one partial row per block.
+//   potential    : PE = -sum_{i<j} G m_i m_j / sqrt(|x_j - x_i|^2 + eps^2),
+//                  one target per lane over sources j > i staged in LDS,
+//                  Neumaier-compensated per lane, one partial per block.
+//   final        : one block folds the partials in fixed order with
+//                  compensation and writes the gc_diagnostics record.
+// Deterministic (no atomics).  Per-pair arithmetic follows the reference
+// ((G*m_i)*m_j / d, d = sqrt(((dx*dx + dy*dy) + dz*dz) + eps^2)); only the
+// summation order differs.
+#include "gc_internal.hpp"
+
+#include <algorithm>
+
+namespace gc {
+namespace {
+
+constexpr int kLin = 11;  // M, KE, Px, Py, Pz, Cx, Cy, Cz, Lx, Ly, Lz
+constexpr int kLinBlocksMax = 1024;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Neumaier compensated add of v into (s, c).
+__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) {
+        c += (s - t) + v;
+    } else {
+        c += (v - t) + s;
+    }
+    s = t;
+}
+
+__global__ __launch_bounds__(256) void diag_linear_kernel(
+    int n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ vx, const double* __restrict__ vy,
+    const double* __restrict__ vz, const double* __restrict__ m, double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double red[kLin][4];
+    double acc[kLin];
+#pragma unroll
+    for (int k = 0; k < kLin; ++k) acc[k] = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double mi = m[i], px = x[i], py = y[i], pz = z[i];
+        const double ux = vx[i], uy = vy[i], uz = vz[i];
+        acc[0] += mi;
+        acc[1] += 0.5 * mi * (ux * ux + uy * uy + uz * uz);
+        acc[2] += ux * mi;
+        acc[3] += uy * mi;
+        acc[4] += uz * mi;
+        acc[5] += px * mi;
+        acc[6] += py * mi;
+        acc[7] += pz * mi;
+        acc[8] += (py * uz - pz * uy) * mi;
+        acc[9] += (pz * ux - px * uz) * mi;
+        acc[10] += (px * uy - py * ux) * mi;
+    }
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kLin; ++k) {
+        const double v = wave_sum(acc[k]);
+        if ((threadIdx.x & 63) == 0) red[k][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kLin) {
+        const int k = threadIdx.x;
+        partial[blockIdx.x * kLin + k] = ((red[k][0] + red[k][1]) + red[k][2]) + red[k][3];
+    }
+}
+
+struct alignas(32) Src64 {
+    double x, y, z, m;
+};
+
+__global__ __launch_bounds__(256) void diag_potential_kernel(
+    int n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, double G, double eps2,
+    double* __restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ Src64 tile[256];
+    __shared__ double red[4];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * 256 + t;
+    const bool active = i < n;
+    const double xi = active ? x[i] : 0.0, yi = active ? y[i] : 0.0, zi = active ? z[i] : 0.0;
+    const double gmi = active ? G * m[i] : 0.0;
+    double s = 0.0, c = 0.0;
+    for (int j0 = blockIdx.x * 256; j0 < n; j0 += 256) {
+        __syncthreads();
+        if (j0 + t < n) {
+            Src64 q;
+            q.x = x[j0 + t];
+            q.y = y[j0 + t];
+            q.z = z[j0 + t];
+            q.m = m[j0 + t];
+            tile[t] = q;
+        }
+        __syncthreads();
+        const int cnt = min(256, n - j0);
+        for (int jj = 0; jj < cnt; ++jj) {
+            const Src64 q = tile[jj];
+            if (j0 + jj <= i) continue;
+            const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+            const double d = __builtin_sqrt((dx * dx + dy * dy + dz * dz) + eps2);
+            if (d == 0.0) continue;
+            neumaier(s, c, gmi * q.m / d);
+        }
+    }
+    double v = active ? (s + c) : 0.0;
+    v = wave_sum(v);
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// Writes the gc_diagnostics field order: KE, PE, E, M, P[3], COM[3], L[3].
+__global__ __launch_bounds__(64) void diag_final_kernel(int nlin, const double* __restrict__ lin,
+                                                         int npot, const double* __restrict__ pot,
+                                                         double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double tot[kLin + 1];
+    const int k = threadIdx.x;
+    if (k < kLin) {
+        double s = 0.0, c = 0.0;
+        for (int b = 0; b < nlin; ++b) neumaier(s, c, lin[b * kLin + k]);
+        tot[k] = s + c;
+    } else if (k == kLin) {
+        double s = 0.0, c = 0.0;
+        for (int b = 0; b < npot; ++b) neumaier(s, c, pot[b]);
+        tot[kLin] = -(s + c);
+    }
+    __syncthreads();
+    if (k == 0) {
+        const double M = tot[0];
+        out[0] = tot[1];
+        out[1] = tot[kLin];
+        out[2] = tot[1] + tot[kLin];
+        out[3] = M;
+        out[4] = tot[2];
+        out[5] = tot[3];
+        out[6] = tot[4];
+        out[7] = M > 0.0 ? tot[5] / M : tot[5];
+        out[8] = M > 0.0 ? tot[6] / M : tot[6];
+        out[9] = M > 0.0 ? tot[7] / M : tot[7];
+        out[10] = tot[8];
+        out[11] = tot[9];
+        out[12] = tot[10];
+    }
+}
+
+int linear_blocks(size_t n) {
+    return static_cast<int>(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, kLinBlocksMax)));
+}
+
+int potential_blocks(size_t n) { return static_cast<int>(std::max<size_t>(1, (n + 255) / 256)); }
+
+}  // namespace
+
+size_t diagnostics_scratch_doubles(size_t n) {
+    return static_cast<size_t>(linear_blocks(n)) * kLin + potential_blocks(n);
+}
+
+void launch_diagnostics(const DeviceState& s, double G, double eps, double* scratch,
+                        size_t scratch_doubles, double* out_dev, hipStream_t st) {
+    const int nl = linear_blocks(s.n), np = potential_blocks(s.n);
+    if (scratch_doubles < static_cast<size_t>(nl) * kLin + np) return;
+    double* lin = scratch;
+    double* pot = scratch + static_cast<size_t>(nl) * kLin;
+    const int n = static_cast<int>(s.n);
+    hipLaunchKernelGGL(diag_linear_kernel, dim3(nl), dim3(256), 0, st, n, s.x, s.y, s.z, s.vx,
+                       s.vy, s.vz, s.m, lin);
+    hipLaunchKernelGGL(diag_potential_kernel, dim3(np), dim3(256), 0, st, n, s.x, s.y, s.z, s.m, G,
+                       eps * eps, pot);
+    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(64), 0, st, nl, lin, np, pot, out_dev);
+}
+
+}  // namespace gc

@@ -1,0 +1,30 @@
+// gc_fmm.hpp — hip-fmm engine: octree, P2M/M2M, far/near lists, M2P + P2P
+// on the resident particle state.
+#pragma once
+
+#include "gc_internal.hpp"
+
+#include <string>
+
+namespace gc {
+
+class FmmEngine {
+public:
+    FmmEngine();
+    ~FmmEngine();
+    FmmEngine(const FmmEngine&) = delete;
+    FmmEngine& operator=(const FmmEngine&) = delete;
+
+    // Accelerations of targets [begin, end) into s.ax/ay/az (and the closing
+    // half kick when close_kick).  Returns a gc_status.
+    int compute(const DeviceState& s, const gc_params& p, size_t begin, size_t end,
+                bool close_kick, double half_dt, hipStream_t st, std::string* err);
+    int time_kernels(const DeviceState& s, const gc_params& p, int reps, hipStream_t st,
+                     double* ms, double* interactions, std::string* err);
+
+private:
+    struct Impl;
+    Impl* impl_;
+};
+
+}  // namespace gc

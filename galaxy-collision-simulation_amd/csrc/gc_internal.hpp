@@ -1,0 +1,67 @@
+// gc_internal.hpp — device-side layout and kernel entry points shared by the
+// gcsim translation units.  Host code only; kernels live in *.hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+
+#include "gcsim.h"
+
+namespace gc {
+
+// Source tile: the LDS j-tile of the all-pairs kernels, and the group of
+// particles that shares one FP64 centre in the mixed-precision layout.
+constexpr int kTile = 256;
+
+inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+
+// Resident SoA particle state in HBM (FP64), capacity padded to kTile.
+struct DeviceState {
+    size_t n = 0, cap = 0;
+    double* base = nullptr;  // one allocation: 10 * cap doubles
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    double* m = nullptr;
+};
+
+// Mixed-precision source layout: per particle (x - c, y - c, z - c, G*m) in
+// FP32 relative to its tile's FP64 centre c; per tile the FP64 centre.
+struct MixedLayout {
+    size_t ntiles = 0;
+    float4* jp = nullptr;      // ntiles * kTile entries, zero-mass padding
+    double* centre = nullptr;  // 3 * ntiles (SoA: cx | cy | cz)
+    float* part = nullptr;     // jsplit * 6 * cap (hi x,y,z | lo x,y,z)
+    size_t part_floats = 0;
+};
+
+struct LaunchShape {
+    int ipt = 0;     // targets per thread (1, 2, 4); 0 = automatic
+    int jsplit = 0;  // source-range splits; 0 = automatic
+};
+
+// ---- kernel launchers (defined in the .hip files) ------------------------
+// direct_fp64.hip
+void launch_direct_fp64(const DeviceState& s, double G, double eps, size_t begin, size_t end,
+                        bool close_kick, double half_dt, hipStream_t st);
+// direct_mixed.hip
+void launch_pack_mixed(const DeviceState& s, MixedLayout& L, double G, hipStream_t st);
+void launch_direct_mixed(const DeviceState& s, MixedLayout& L, double eps, int ipt, int jsplit,
+                         size_t begin, size_t end, hipStream_t st);
+void launch_merge_mixed(const DeviceState& s, const MixedLayout& L, int jsplit, size_t begin,
+                        size_t end, bool close_kick, double half_dt, hipStream_t st);
+// integrate.hip
+void launch_open_step(const DeviceState& s, double half_dt, double dt, hipStream_t st);
+void launch_kick(const DeviceState& s, double dt, hipStream_t st);
+void launch_scatter_xyz(const double* interleaved, double* x, double* y, double* z, size_t n,
+                        hipStream_t st);
+void launch_gather_xyz(const double* x, const double* y, const double* z, double* interleaved,
+                       size_t n, hipStream_t st);
+// diagnostics.hip: out[13] as gc_diagnostics order (M, KE, P3, C3, L3, PE)
+void launch_diagnostics(const DeviceState& s, double G, double eps, double* scratch,
+                        size_t scratch_doubles, double* out_dev, hipStream_t st);
+size_t diagnostics_scratch_doubles(size_t n);
+
+}  // namespace gc

@@ -1,0 +1,130 @@
+// hip_solver.hpp — C++ force-solver interface of the gfx950 engine, shaped
+// like the reference's solver free functions so that `hip-direct` and
+// `hip-fmm` slot into its `solver = ...` dispatch
+// (cpp/core/main.cpp:83-139) next to `direct` and `fmm`.
+//
+// Same conventions as the reference (cpp/direct/direct_solver.hpp:22-28,
+// cpp/fmm/fmm_solver.hpp:71-83, cpp/cuda/cuda_solver.hpp:10-12):
+//   * std::vector<Particle>& is mutated in place (acceleration for force
+//     calls; position, velocity and acceleration for step calls);
+//   * errors throw std::runtime_error("context: message");
+//   * the caller owns the host vector.
+// Differences: there is no silent CPU fallback (hip_solver_available()
+// reports whether a gfx950 device is visible and every call without one
+// throws), and HipForceSolver keeps the particles resident in HBM across
+// steps instead of re-uploading per call.
+//
+// Built inside the reference, define GCSIM_REFERENCE_TYPES so these
+// declarations use the reference's own Particle / PhysicsParams / FmmOptions;
+// otherwise layout-identical local definitions are used.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gcsim.h"
+
+#if defined(GCSIM_REFERENCE_TYPES)
+#include "core/diagnostics.hpp"
+#include "core/particle.hpp"
+#include "core/simulation_params.hpp"
+#include "fmm/fmm_solver.hpp"
+#else
+namespace fmmgalaxy {
+
+struct Vec3 {
+    double x{0.0};
+    double y{0.0};
+    double z{0.0};
+};
+using Vec2 = Vec3;
+
+struct Particle {
+    Vec2 position{};
+    Vec2 velocity{};
+    Vec2 acceleration{};
+    double mass{1.0};
+    std::uint32_t group_id{0};
+};
+
+struct PhysicsParams {
+    double gravitational_constant{1.0};
+    double softening{0.02};
+};
+
+struct FmmOptions {
+    double theta{0.6};
+    std::size_t leaf_capacity{16};
+    int max_depth{32};
+    int expansion_order{4};
+};
+
+struct Diagnostics {
+    double kinetic_energy{0.0};
+    double potential_energy{0.0};
+    double total_energy{0.0};
+    double total_mass{0.0};
+    Vec2 momentum{};
+    Vec2 center_of_mass{};
+    Vec2 angular_momentum{};
+};
+
+}  // namespace fmmgalaxy
+#endif
+
+namespace fmmgalaxy {
+
+enum class HipPrecision { fp64, mixed };
+
+// Solver-name predicates in the style of cpp/core/main.cpp:62-73.
+bool uses_hip_direct_solver(const std::string& solver);
+bool uses_hip_fmm_solver(const std::string& solver);
+// "fp64" | "double" -> fp64, "mixed" | "fp32" -> mixed; throws otherwise.
+HipPrecision parse_hip_precision(const std::string& value);
+
+bool hip_solver_available();
+
+void compute_hip_direct_accelerations(std::vector<Particle>& particles, const PhysicsParams& params,
+                                      HipPrecision precision = HipPrecision::fp64);
+void compute_hip_direct_accelerations_for_targets(std::vector<Particle>& particles,
+                                                  const PhysicsParams& params, std::size_t begin,
+                                                  std::size_t end,
+                                                  HipPrecision precision = HipPrecision::fp64);
+void hip_direct_leapfrog_step(std::vector<Particle>& particles, double dt,
+                              const PhysicsParams& params,
+                              HipPrecision precision = HipPrecision::fp64);
+void compute_hip_fmm_accelerations(std::vector<Particle>& particles, const PhysicsParams& params,
+                                   FmmOptions options = {},
+                                   HipPrecision precision = HipPrecision::mixed);
+void compute_hip_fmm_accelerations_for_targets(std::vector<Particle>& particles,
+                                               const PhysicsParams& params, std::size_t begin,
+                                               std::size_t end, FmmOptions options = {},
+                                               HipPrecision precision = HipPrecision::mixed);
+
+// Resident-state solver: upload once, step many times, download on demand.
+class HipForceSolver {
+public:
+    enum class Kind { direct, fmm };
+    HipForceSolver(const PhysicsParams& params, Kind kind, HipPrecision precision,
+                   FmmOptions options = {}, int device = 0);
+    ~HipForceSolver();
+    HipForceSolver(const HipForceSolver&) = delete;
+    HipForceSolver& operator=(const HipForceSolver&) = delete;
+
+    // with_accelerations: also take each particle's acceleration as a(x)
+    void upload(const std::vector<Particle>& particles, bool with_accelerations = false);
+    void compute_accelerations();
+    void compute_accelerations_for_targets(std::size_t begin, std::size_t end);
+    void step(double dt, int steps = 1);
+    void download(std::vector<Particle>& particles);
+    Diagnostics diagnostics();
+    std::size_t size() const;
+    gc_ctx* handle() { return ctx_; }
+
+private:
+    gc_ctx* ctx_{nullptr};
+};
+
+}  // namespace fmmgalaxy

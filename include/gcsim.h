@@ -1,0 +1,192 @@
+/*
+ * gcsim.h — C ABI of the MI355X (gfx950) force engine for the galaxy
+ * collision simulator's hot path: softened all-pairs gravity, the FMM
+ * near/far field, and kick-drift-kick leapfrog, with particles resident in
+ * HBM across steps.
+ *
+ * Plain C: opaque handles, int status codes, plain pointers and sizes, no
+ * exceptions and no torch types.  Host arrays are particle-major n x 3
+ * (x, y, z interleaved, the field order of the reference's Particle) and
+ * are only read or written during the call.  Device state is SoA FP64 and
+ * owned by the context.  One context per host thread; each context runs on
+ * its own HIP stream (replaceable with gc_set_stream for interop).
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   cpp/cuda/cuda_solver.hpp:10   bool cuda_solver_available()
+ *   cpp/cuda/cuda_solver.hpp:11   compute_cuda_direct_accelerations(particles, params)
+ *   cpp/cuda/cuda_solver.hpp:12   cuda_direct_leapfrog_step(particles, dt, params)
+ *   cpp/direct/direct_solver.hpp:22-28  compute_direct_accelerations[_for_targets]
+ *   cpp/fmm/fmm_solver.hpp:71-83  compute_fmm_accelerations[_for_targets]
+ *   cpp/core/integrator.hpp:13-21 kick / drift / leapfrog_step
+ *   cpp/core/diagnostics.hpp:21   compute_diagnostics
+ */
+#ifndef GCSIM_H
+#define GCSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCSIM_ABI_VERSION 1
+
+typedef enum {
+    GC_OK = 0,
+    GC_ERR_INVALID_ARGUMENT = 1,
+    GC_ERR_NO_DEVICE = 2,
+    GC_ERR_HIP = 3,
+    GC_ERR_OUT_OF_MEMORY = 4,
+    GC_ERR_UNSUPPORTED = 5,
+    GC_ERR_STATE = 6
+} gc_status;
+
+/* solver choice: the `solver = "hip-direct" | "hip-fmm"` config values */
+typedef enum { GC_SOLVER_DIRECT = 0, GC_SOLVER_FMM = 1 } gc_solver_kind;
+
+/* GC_PRECISION_FP64: FP64 pairs, per target ascending-j sums, the same
+ *   operation order as compute_direct_accelerations_for_targets
+ *   (cpp/direct/direct_solver.cpp:59-83).
+ * GC_PRECISION_MIXED: FP32 pair forces on FP32 offsets from FP64 tile
+ *   centres, per-tile FP32 partials merged by two-sum, FP64 state. */
+typedef enum { GC_PRECISION_FP64 = 0, GC_PRECISION_MIXED = 1 } gc_precision;
+
+/* PhysicsParams (cpp/core/simulation_params.hpp:5-8) + FmmOptions
+ * (cpp/fmm/fmm_solver.hpp:13-18) + solver/precision selection. */
+typedef struct {
+    double gravitational_constant; /* G, default 1 */
+    double softening;              /* Plummer epsilon, default 0.02 */
+    int solver;                    /* gc_solver_kind */
+    int precision;                 /* gc_precision */
+    double theta;                  /* FMM opening angle, default 0.6 */
+    int leaf_capacity;             /* FMM leaf size, default 16 */
+    int max_depth;                 /* FMM depth cap, default 32 */
+    int expansion_order;           /* FMM order 0..4, default 4 */
+} gc_params;
+
+/* Diagnostics (cpp/core/diagnostics.hpp:11-19), FP64 */
+typedef struct {
+    double kinetic_energy;
+    double potential_energy;
+    double total_energy;
+    double total_mass;
+    double momentum[3];
+    double center_of_mass[3];
+    double angular_momentum[3];
+} gc_diagnostics;
+
+/* Device addresses of the resident SoA state (for torch interop). */
+typedef struct {
+    size_t n;
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    double *ax, *ay, *az;
+    double* mass;
+    void* stream;
+} gc_device_view;
+
+typedef struct gc_ctx gc_ctx;
+
+/* Fills p with the reference defaults: G=1, softening=0.02, direct, fp64,
+ * theta=0.6, leaf 16, depth 32, order 4. */
+void gc_default_params(gc_params* p);
+
+/* Number of visible HIP devices (0 without a GPU). */
+int gc_device_count(void);
+
+/* cuda_solver_available (cpp/cuda/cuda_solver.hpp:10): 1 iff a gfx950
+ * device is visible.  There is no CPU fallback: callers must check. */
+int gc_solver_available(void);
+
+const char* gc_status_string(gc_status s);
+/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+const char* gc_last_error(const gc_ctx* ctx);
+
+gc_status gc_create(gc_ctx** out, int device, const gc_params* p);
+void gc_destroy(gc_ctx* ctx);
+gc_status gc_set_params(gc_ctx* ctx, const gc_params* p);
+gc_status gc_set_stream(gc_ctx* ctx, void* hip_stream); /* NULL: back to own */
+void* gc_get_stream(gc_ctx* ctx);
+
+/* Uploads n particles (pos, vel: n x 3; mass: n).  vel may be NULL (zero).
+ * Accelerations are zeroed; call gc_compute_accelerations before stepping. */
+gc_status gc_upload(gc_ctx* ctx, size_t n, const double* pos, const double* vel,
+                    const double* mass);
+/* Replaces the resident accelerations (n x 3), e.g. with a(x) the caller
+ * already holds, as cuda_direct_leapfrog_step starts from the particles'
+ * accelerations (cpp/cuda/cuda_solver.cu:212-246). */
+gc_status gc_set_accelerations(gc_ctx* ctx, const double* acc);
+/* Copies state back (any pointer may be NULL).  Synchronises the stream. */
+gc_status gc_download(gc_ctx* ctx, double* pos, double* vel, double* acc);
+size_t gc_particle_count(const gc_ctx* ctx);
+gc_status gc_device_state(gc_ctx* ctx, gc_device_view* out);
+
+/* a(x) on the resident state: compute_*_accelerations equivalent. */
+gc_status gc_compute_accelerations(gc_ctx* ctx);
+/* nsteps KDK steps (integrator.cpp:33-42): kick dt/2, drift dt, a(x),
+ * kick dt/2.  Requires a(x) of the current positions (gc_compute_accelerations
+ * or a previous step).  Asynchronous on the context's stream. */
+gc_status gc_step(gc_ctx* ctx, double dt, int nsteps);
+gc_status gc_synchronize(gc_ctx* ctx);
+
+/* FP64 diagnostics of the resident state on the device (wavefront
+ * reductions; O(N^2/2) softened potential).  Synchronises. */
+gc_status gc_diagnostics_compute(gc_ctx* ctx, gc_diagnostics* out);
+
+/* Accelerations of an explicit target range [begin, end) only
+ * (compute_*_accelerations_for_targets); other entries keep their values. */
+gc_status gc_compute_accelerations_for_targets(gc_ctx* ctx, size_t begin, size_t end);
+
+/* Measurement hooks: average duration (ms) of the dominant force kernel
+ * over `reps` back-to-back launches on the context's stream (HIP events),
+ * and the FP32-pair interactions one launch performs. */
+gc_status gc_time_force_kernel(gc_ctx* ctx, int reps, double* ms_per_launch,
+                               double* interactions_per_launch);
+
+/* Tuning: mixed-kernel launch shape (targets per thread 1/2/4, j-splits).
+ * 0 = automatic. */
+gc_status gc_set_launch_shape(gc_ctx* ctx, int targets_per_thread, int jsplit);
+gc_status gc_get_launch_shape(gc_ctx* ctx, int* targets_per_thread, int* jsplit);
+
+
+/* ---- synthetic initial conditions (host side) -------------------------
+ * No reference counterpart for these workloads: the reference's generator
+ * (cpp/core/initial_conditions.cpp:37-114) has no central mass and no
+ * Plummer sphere.  Deterministic per seed (std::mt19937_64 with explicit
+ * uniform/normal transforms). */
+typedef struct {
+    size_t n_particles;        /* 1 central mass + (n_particles - 1) disk */
+    double central_mass;       /* default 1.0 */
+    double disk_mass;          /* total disk mass, used when disk_particle_mass == 0 */
+    double disk_particle_mass; /* > 0 overrides disk_mass / (n_particles - 1) */
+    double scale_length;       /* exponential scale, default 0.3 */
+    double r_min, r_max;       /* radius truncation, default [0.05, 1.0] */
+    double z_sigma;            /* vertical N(0, z_sigma), default 0.02 */
+    double v_sigma;            /* per-component velocity dispersion, default 0.01 */
+    double inclination;        /* rotation about x (radians) */
+    double position[3], velocity[3];
+} gc_disk_galaxy;
+
+typedef struct {
+    size_t n_particles;      /* even, split equally between the two galaxies */
+    double inclination;      /* radians, of galaxy B about x */
+    double impact_parameter; /* y offset between the centres */
+    double speed;            /* relative approach speed along x */
+    uint64_t seed;           /* per-run generator seed */
+} gc_collision;
+
+void gc_default_disk_galaxy(gc_disk_galaxy* g);
+/* Writes sum(n_particles) particles; returns that count (0 on bad input). */
+size_t gc_generate_disks(const gc_disk_galaxy* galaxies, int count, double G, double eps,
+                         uint64_t seed, double* pos, double* vel, double* mass);
+size_t gc_generate_plummer(size_t n, double total_mass, double scale, double G, uint64_t seed,
+                           double* pos, double* vel, double* mass);
+/* Batch sampler: N log-uniform in [n_min, n_max] (even), inclination
+ * U[0, pi/2], impact parameter U[0.5, 2], speed U[0.2, 0.5]. */
+int gc_sample_collisions(int runs, uint64_t seed, int n_min, int n_max, gc_collision* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GCSIM_H */

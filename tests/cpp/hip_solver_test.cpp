@@ -1,0 +1,133 @@
+// hip_solver_test.cpp — the reference's smoke checks (cpp/tests/smoke_tests.cpp
+// :54-77, :83-123) run against the C++ hip-direct interface, with the CPU
+// oracle as the FP64 checker.  Without a gfx950 device it verifies the error
+// path instead: no silent CPU fallback, every call throws.
+#include "hip_solver.hpp"
+#include "gcoracle.h"
+
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <stdexcept>
+#include <vector>
+
+using namespace fmmgalaxy;
+
+namespace {
+
+int failures = 0;
+
+void require(bool ok, const char* what) {
+    if (!ok) {
+        std::fprintf(stderr, "FAILED: %s\n", what);
+        ++failures;
+    }
+}
+
+double rel_err_mean(const std::vector<Particle>& a, const std::vector<double>& ref) {
+    double sum = 0.0;
+    for (std::size_t i = 0; i < a.size(); ++i) {
+        const double dx = a[i].acceleration.x - ref[3 * i];
+        const double dy = a[i].acceleration.y - ref[3 * i + 1];
+        const double dz = a[i].acceleration.z - ref[3 * i + 2];
+        const double nr = std::sqrt(ref[3 * i] * ref[3 * i] + ref[3 * i + 1] * ref[3 * i + 1] +
+                                    ref[3 * i + 2] * ref[3 * i + 2]);
+        sum += std::sqrt(dx * dx + dy * dy + dz * dz) / std::max(nr, 1e-12);
+    }
+    return sum / static_cast<double>(a.size());
+}
+
+}  // namespace
+
+int main() {
+    if (!hip_solver_available()) {
+        std::vector<Particle> ps(2);
+        ps[1].position = {1.0, 0.0, 0.0};
+        bool threw = false;
+        try {
+            compute_hip_direct_accelerations(ps, PhysicsParams{});
+        } catch (const std::runtime_error& e) {
+            threw = true;
+            std::printf("no device: %s\n", e.what());
+        }
+        require(threw, "hip-direct without a device throws std::runtime_error");
+        require(uses_hip_direct_solver("hip-direct") && uses_hip_fmm_solver("hip-fmm") &&
+                    !uses_hip_direct_solver("direct"),
+                "solver name predicates");
+        bool bad = false;
+        try {
+            parse_hip_precision("fp16");
+        } catch (const std::runtime_error&) {
+            bad = true;
+        }
+        require(bad, "unknown precision throws");
+        std::printf(failures ? "hip_solver_test FAILED\n" : "hip_solver_test passed (no device)\n");
+        return failures ? 1 : 0;
+    }
+
+    // two-body, eps = 0: attraction and momentum balance (smoke_tests.cpp:54-69)
+    PhysicsParams exact;
+    exact.softening = 0.0;
+    std::vector<Particle> two(2);
+    two[0].position = {-0.5, 0.0, 0.2};
+    two[0].mass = 2.0;
+    two[1].position = {0.5, 0.0, -0.2};
+    two[1].mass = 3.0;
+    compute_hip_direct_accelerations(two, exact, HipPrecision::fp64);
+    require(two[0].acceleration.x > 0.0, "direct acceleration attracts particle 0");
+    require(two[1].acceleration.x < 0.0, "direct acceleration attracts particle 1");
+    require(std::fabs(two[0].mass * two[0].acceleration.x + two[1].mass * two[1].acceleration.x) <=
+                1e-12,
+            "pairwise direct force is symmetric");
+    two[0].velocity = {0.0, 0.5, 0.0};
+    two[1].velocity = {0.0, -1.0 / 3.0, 0.0};
+    hip_direct_leapfrog_step(two, 0.001, exact, HipPrecision::fp64);
+    require(std::isfinite(two[0].position.x), "leapfrog position stays finite");
+
+    // 80 particles in [-1,1]^3 from mt19937_64(7), eps 0.03 (smoke_tests.cpp:83-123)
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> uniform(-1.0, 1.0);
+    std::vector<Particle> cube(80);
+    for (auto& p : cube) {
+        p.position = {uniform(rng), uniform(rng), uniform(rng)};
+        p.mass = 1.0 / 80.0;
+    }
+    PhysicsParams soft;
+    soft.softening = 0.03;
+    std::vector<double> pos(3 * 80), mass(80), ref(3 * 80);
+    for (int i = 0; i < 80; ++i) {
+        pos[3 * i] = cube[i].position.x;
+        pos[3 * i + 1] = cube[i].position.y;
+        pos[3 * i + 2] = cube[i].position.z;
+        mass[i] = cube[i].mass;
+    }
+    orc_direct_symmetric(80, pos.data(), mass.data(), 1.0, 0.03, ref.data());
+    auto fp64 = cube, mixed = cube;
+    compute_hip_direct_accelerations(fp64, soft, HipPrecision::fp64);
+    compute_hip_direct_accelerations(mixed, soft, HipPrecision::mixed);
+    require(rel_err_mean(fp64, ref) < 1.0e-10, "hip-direct fp64 matches direct solver");
+    require(rel_err_mean(mixed, ref) < 1.0e-6, "hip-direct mixed matches direct solver");
+
+    // resident solver: 10 steps equal 10 single steps of the oracle targets form
+    HipForceSolver solver(soft, HipForceSolver::Kind::direct, HipPrecision::fp64);
+    solver.upload(cube);
+    solver.compute_accelerations();
+    solver.step(0.01, 10);
+    std::vector<Particle> out;
+    solver.download(out);
+    std::vector<double> vel(3 * 80, 0.0), acc(3 * 80);
+    orc_direct_targets(80, pos.data(), mass.data(), 1.0, 0.03, 0, 80, acc.data());
+    orc_solver s{1, 1.0, 0.03, 0.6, 16, 32, 4};
+    orc_leapfrog(&s, 80, pos.data(), vel.data(), acc.data(), mass.data(), 0.01, 10);
+    bool same = true;
+    for (int i = 0; i < 80; ++i) {
+        same = same && out[i].position.x == pos[3 * i] && out[i].position.y == pos[3 * i + 1] &&
+               out[i].position.z == pos[3 * i + 2] && out[i].velocity.x == vel[3 * i];
+    }
+    require(same, "resident hip-direct fp64 KDK is bit-identical to the oracle");
+    const Diagnostics d = solver.diagnostics();
+    require(d.total_mass > 0.0 && std::isfinite(d.total_energy), "device diagnostics finite");
+
+    std::printf(failures ? "hip_solver_test FAILED\n" : "hip_solver_test passed\n");
+    return failures ? 1 : 0;
+}

@@ -1,0 +1,208 @@
+"""hip-direct on the GPU, through the C ABI, against the oracle.
+
+FP64 path: bit-identical to the reference's FP64 targets form
+(cpp/direct/direct_solver.cpp:59-83), per force evaluation and over KDK
+steps (cpp/core/integrator.cpp:33-42).
+Mixed path: max relative force error <= 1e-5 (the bound the mixed-precision
+metric is quoted at) against the FP64 oracle.
+Diagnostics: <= 1e-12 relative against the oracle's FP64 restatement.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load_diagnostics, load_snapshot
+from helpers import rel_err, smoke_cube
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-5  # max relative force error, mixed path vs FP64
+
+
+def _engine(gpu, precision, eps, **kw):
+    return gpu.Engine("hip-direct", precision, softening=eps, **kw)
+
+
+def _accel(gpu, precision, pos, vel, mass, eps, G=1.0, shape=None):
+    with _engine(gpu, precision, eps, G=G) as eng:
+        if shape:
+            eng.set_launch_shape(*shape)
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations()
+        return eng.download()[2]
+
+
+@pytest.mark.parametrize("n", [250, 1000])
+def test_fp64_bit_exact_on_golden_inputs(gpu, n):
+    m, x, v, a = load_snapshot(f"direct_{n}_snapshot_000000.csv.gz")
+    got = _accel(gpu, "fp64", x, v, m, 0.025)
+    assert np.array_equal(got, O.direct_targets(x, m, 1.0, 0.025))
+    # the golden itself is the symmetric form: same sums, other order
+    assert rel_err(got, a).max() < 1e-13
+
+
+@pytest.mark.parametrize("n,eps,G", [(1, 0.02, 1.0), (2, 0.0, 1.0), (257, 0.01, 2.5),
+                                     (3001, 0.05, 1.0)])
+def test_fp64_bit_exact_random(gpu, n, eps, G):
+    rng = np.random.default_rng(n)
+    pos, m = rng.normal(size=(n, 3)), rng.uniform(0.1, 1.0, n)
+    got = _accel(gpu, "fp64", pos, None, m, eps, G)
+    assert np.array_equal(got, O.direct_targets(pos, m, G, eps))
+
+
+def test_fp64_coincident_particles_without_softening(gpu):
+    pos = np.array([[0.0, 0, 0], [0.0, 0, 0], [1.0, 0, 0], [0.5, 0.5, 0.5]])
+    m = np.array([1.0, 2.0, 3.0, 4.0])
+    got = _accel(gpu, "fp64", pos, None, m, 0.0)
+    assert np.isfinite(got).all()
+    assert np.array_equal(got, O.direct_targets(pos, m, 1.0, 0.0))
+    mixed = _accel(gpu, "mixed", pos, None, m, 0.0)
+    assert np.isfinite(mixed).all() and rel_err(mixed, got).max() < MIXED_TOL
+
+
+def test_fp64_kdk_steps_bit_exact(gpu):
+    m, x, v, a = load_snapshot("direct_1000_snapshot_000000.csv.gz")
+    solver = O.make_solver(O.SOLVER_DIRECT_TARGETS, 1.0, 0.025)
+    a0 = O.direct_targets(x, m, 1.0, 0.025)
+    want = O.leapfrog(solver, x, v, a0, m, 0.01, 20)
+    with _engine(gpu, "fp64", 0.025) as eng:
+        eng.upload(x, v, m)
+        eng.compute_accelerations()
+        eng.step(0.01, 20)
+        got = eng.download()
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+    # and the reference's golden (symmetric-form sums) to rounding
+    m20, x20, v20, a20 = load_snapshot("direct_1000_snapshot_000020.csv.gz")
+    assert np.abs(got[0] - x20).max() < 1e-12
+    assert rel_err(got[2], a20).max() < 1e-10
+
+
+def test_fp64_step_from_caller_accelerations(gpu):
+    # cuda_direct_leapfrog_step semantics: the step starts from the
+    # accelerations the particles carry, not from a fresh a(x)
+    m, x, v, a = load_snapshot("direct_250_snapshot_000000.csv.gz")
+    fake = a * 0.5
+    solver = O.make_solver(O.SOLVER_DIRECT_TARGETS, 1.0, 0.025)
+    want = O.leapfrog(solver, x, v, fake, m, 0.01, 1)
+    with _engine(gpu, "fp64", 0.025) as eng:
+        eng.upload(x, v, m, acc=fake)
+        eng.step(0.01, 1)
+        got = eng.download()
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+def test_targets_range_leaves_others(gpu):
+    rng = np.random.default_rng(5)
+    pos, m = rng.normal(size=(900, 3)), rng.uniform(0.1, 1.0, 900)
+    for prec in ("fp64", "mixed"):
+        with _engine(gpu, prec, 0.02) as eng:
+            eng.upload(pos, None, m)
+            eng.compute_accelerations(300, 650)
+            acc = eng.download()[2]
+        ref = O.direct_targets(pos, m, 1.0, 0.02)
+        assert not acc[:300].any() and not acc[650:].any()
+        tol = 0 if prec == "fp64" else MIXED_TOL
+        assert rel_err(acc[300:650], ref[300:650]).max() <= tol
+
+
+def test_reference_smoke_cube_thresholds(gpu):
+    # cpp/tests/smoke_tests.cpp:83-123: the device solver matches direct < 1e-10
+    pos, m = smoke_cube()
+    ref = O.direct_symmetric(pos, m, 1.0, 0.03)
+    assert rel_err(_accel(gpu, "fp64", pos, None, m, 0.03), ref).mean() < 1e-10
+    assert rel_err(_accel(gpu, "mixed", pos, None, m, 0.03), ref).max() < MIXED_TOL
+
+
+def test_mixed_config0_error(gpu):
+    pos, vel, m = gpu.generators.config0()
+    ref = O.direct_targets(pos, m, 1.0, 0.05)
+    err = rel_err(_accel(gpu, "mixed", pos, vel, m, 0.05), ref)
+    assert err.max() <= MIXED_TOL and np.median(err) < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 3), (4, 8), (1, 64), (2, 0), (0, 0)])
+def test_mixed_launch_shapes(gpu, shape):
+    pos, vel, m = gpu.generators.config1(n=6000)
+    ref = O.direct_targets(pos, m, 1.0, 0.02, threads=0)
+    err = rel_err(_accel(gpu, "mixed", pos, vel, m, 0.02, shape=shape), ref)
+    assert err.max() <= MIXED_TOL
+
+
+def test_mixed_full_size_against_fp64_gpu(gpu):
+    # N = 65,536 (the precision workload): the FP64 GPU path is the oracle
+    # here (bit-identical to the CPU FP64 form above); spot-check it on a
+    # CPU sample too.
+    pos, vel, m = gpu.generators.config1()
+    a64 = _accel(gpu, "fp64", pos, vel, m, 0.02)
+    sample = np.arange(0, 65536, 257)
+    cpu = O.direct_targets(pos, m, 1.0, 0.02, threads=0)
+    assert np.array_equal(a64[sample], cpu[sample])
+    err = rel_err(_accel(gpu, "mixed", pos, vel, m, 0.02), a64)
+    assert err.max() <= MIXED_TOL
+    assert np.median(err) < 1e-6
+
+
+def test_mixed_kdk_tracks_fp64(gpu):
+    pos, vel, m = gpu.generators.config0()
+    with _engine(gpu, "mixed", 0.05) as mx, _engine(gpu, "fp64", 0.05) as fp:
+        for eng in (mx, fp):
+            eng.upload(pos, vel, m)
+            eng.compute_accelerations()
+            eng.step(1e-3, 50)
+        pm, vm, am = mx.download()
+        pf, vf, af = fp.download()
+        e_m, e_f = mx.diagnostics(), fp.diagnostics()
+    assert np.abs(pm - pf).max() < 1e-6
+    assert abs(e_m["total_energy"] - e_f["total_energy"]) < 1e-7 * abs(e_f["total_energy"])
+
+
+@pytest.mark.parametrize("n", [250, 1000])
+def test_diagnostics_against_oracle_and_golden(gpu, n):
+    m, x, v, a = load_snapshot(f"direct_{n}_snapshot_000000.csv.gz")
+    rows = load_diagnostics(f"direct_{n}_diagnostics.csv")
+    with _engine(gpu, "fp64", 0.025) as eng:
+        eng.upload(x, v, m)
+        d = eng.diagnostics()
+    want = O.diagnostics(x, v, m, 1.0, 0.025)
+    for k, w in want.items():
+        assert abs(d[k] - w) <= 1e-12 * max(abs(w), 1e-3), k
+    got_row = np.array([d[k] for k in O.DIAG_FIELDS])
+    assert np.allclose(got_row, rows[0, 3:], rtol=1e-12, atol=1e-15)
+
+
+def test_diagnostics_large_n(gpu):
+    pos, vel, m = gpu.generators.config1(n=20000)
+    with _engine(gpu, "fp64", 0.02) as eng:
+        eng.upload(pos, vel, m)
+        d = eng.diagnostics()
+    want = O.diagnostics(pos, vel, m, 1.0, 0.02)
+    for k, w in want.items():
+        assert abs(d[k] - w) <= 1e-12 * max(abs(w), 1e-3), k
+
+
+def test_empty_and_reupload(gpu):
+    with _engine(gpu, "mixed", 0.02) as eng:
+        eng.upload(np.zeros((0, 3)), None, np.zeros(0))
+        eng.compute_accelerations()
+        eng.step(0.01, 2)
+        assert eng.n == 0
+        rng = np.random.default_rng(9)
+        for n in (700, 100, 1500):
+            pos, m = rng.normal(size=(n, 3)), rng.uniform(0.1, 1, n)
+            eng.upload(pos, None, m)
+            eng.compute_accelerations()
+            acc = eng.download()[2]
+            assert acc.shape == (n, 3)
+            assert rel_err(acc, O.direct_targets(pos, m, 1.0, 0.02)).max() <= MIXED_TOL
+
+
+def test_cpp_interface(gpu):
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parent / "cpp" / "hip_solver_test"
+    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "hip_solver_test passed" in res.stdout and "no device" not in res.stdout
