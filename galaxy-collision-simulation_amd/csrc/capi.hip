@@ -466,6 +466,18 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
     return launch_status(c, "timed force kernels");
 }
 
+gc_status gc_fmm_last_stats(gc_ctx* c, gc_fmm_stats* out) {
+    if (!c || !out) return fail(c, GC_ERR_INVALID_ARGUMENT, "fmm_last_stats: null argument");
+    if (!c->fmm) return fail(c, GC_ERR_STATE, "fmm_last_stats: no hip-fmm evaluation yet");
+    *out = c->fmm->stats();
+    float b = 0.f, e = 0.f;
+    if (c->fmm->phase_times(&b, &e) == GC_OK) {
+        out->build_ms = b;
+        out->eval_ms = e;
+    }
+    return GC_OK;
+}
+
 gc_status gc_set_launch_shape(gc_ctx* c, int ipt, int jsplit) {
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (ipt != 0 && ipt != 1 && ipt != 2 && ipt != 4)

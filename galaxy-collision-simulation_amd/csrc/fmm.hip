@@ -1,23 +1,933 @@
-// fmm.hip — hip-fmm engine.
+// fmm.hip — hip-fmm: the reference's octree FMM (cpp/fmm/fmm_solver.cpp,
+// cpp/fmm/multipole.cpp) on the resident particle state.
+//
+// Per force evaluation, all on the context's stream:
+//  1. bounding box -> root cube exactly as fmm_solver.cpp:60-85 (FP64);
+//  2. per particle a 63-bit octant key: 21 levels of the reference's
+//     ">=" tests against child centres recursed as c +- hw/2 in FP64
+//     (fmm_solver.cpp:106-134), so the key path equals the insertion path;
+//  3. stable radix sort of (key, index) (hipCUB);
+//  4. level-by-level split: a node at depth L is internal iff it holds more
+//     than leaf_capacity particles and L < max_depth — the tree the
+//     reference's insertion builds (fmm_solver.cpp:136-155).  Empty children
+//     are not materialised (they have zero mass and never enter a list);
+//  5. upward pass, deepest level first: mass, centre of mass, 35 raw
+//     moments about the COM in FP64 (P2M at leaves, binomial M2M shift),
+//     then FP32 normalised moments N_alpha = M_alpha / alpha!;
+//  6. per target leaf a preorder DFS with the reference's acceptance test
+//     2 (h_t + h_s) / |c_s - c_t| < theta on geometric centres
+//     (fmm_solver.cpp:228-277): far cells and near leaves, in the
+//     reference's list order;
+//  7. evaluation, one 64-lane wave per target leaf: every far cell by M2P
+//     (generated Cartesian derivative tensors, m2p_generated.inc), every
+//     near leaf by P2P over sources staged in LDS as offsets from the
+//     target leaf's centre; self pairs skipped (fmm_solver.cpp:295-310).
+// The evaluation is instantiated in FP32 (precision "mixed": FP32 pairs and
+// M2P on FP32 offsets, FP64 accumulation) and FP64.
 #include "gc_fmm.hpp"
 
-namespace gc {
+#include <hipcub/hipcub.hpp>
 
-struct FmmEngine::Impl {};
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace gc {
+namespace {
+
+constexpr int kLevels = 21;       // 3 key bits per level, 63 bits
+constexpr int kMom = 35;          // raw Cartesian moments, total degree <= 4
+constexpr int kN32 = 36;          // FP32 normalised moments per node (+pad)
+constexpr int kStack = 8 * (kLevels + 2);
+constexpr int kEvalLanes = 64;
+
+__device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double gc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float gc_rsq(float s) { return __builtin_amdgcn_rsqf(s); }
+__device__ __forceinline__ double gc_rsq(double s) { return 1.0 / __builtin_sqrt(s); }
+
+#define GC_M2P_INLINE __device__ __forceinline__
+#define GC_FMA(a, b, c) gc_fma((a), (b), (c))
+#define GC_RSQ(s) gc_rsq(s)
+#include "m2p_generated.inc"
+#undef GC_FMA
+#undef GC_RSQ
+
+// ---- multipole tables (graded exponent order of multipole.cpp:17-26) ----
+struct Tables {
+    int e[kMom][3];
+    double inv_fact[kMom];  // 1 / alpha!
+    // M2M pairs (alpha, beta <= alpha, 2 <= |alpha|), coefficient
+    // prod_i C(alpha_i, beta_i), shift exponent alpha - beta
+    int npair;
+    int pa[512], pb[512], sx[512], sy[512], sz[512];
+    double coef[512];
+};
+
+constexpr Tables make_tables() {
+    Tables t{};
+    int k = 0;
+    for (int d = 0; d <= 4; ++d)
+        for (int a = d; a >= 0; --a)
+            for (int b = d - a; b >= 0; --b) {
+                t.e[k][0] = a;
+                t.e[k][1] = b;
+                t.e[k][2] = d - a - b;
+                ++k;
+            }
+    int fact[5] = {1, 1, 2, 6, 24};
+    for (int i = 0; i < kMom; ++i)
+        t.inv_fact[i] = 1.0 / static_cast<double>(fact[t.e[i][0]] * fact[t.e[i][1]] * fact[t.e[i][2]]);
+    int binom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
+    int n = 0;
+    for (int a = 0; a < kMom; ++a) {
+        const int da = t.e[a][0] + t.e[a][1] + t.e[a][2];
+        if (da < 2) continue;
+        for (int b = 0; b < kMom; ++b) {
+            if (t.e[b][0] > t.e[a][0] || t.e[b][1] > t.e[a][1] || t.e[b][2] > t.e[a][2]) continue;
+            const int db = t.e[b][0] + t.e[b][1] + t.e[b][2];
+            if (db == 1) continue;  // dipole about the COM vanishes
+            t.pa[n] = a;
+            t.pb[n] = b;
+            t.sx[n] = t.e[a][0] - t.e[b][0];
+            t.sy[n] = t.e[a][1] - t.e[b][1];
+            t.sz[n] = t.e[a][2] - t.e[b][2];
+            t.coef[n] = static_cast<double>(binom[t.e[a][0]][t.e[b][0]] * binom[t.e[a][1]][t.e[b][1]] *
+                                            binom[t.e[a][2]][t.e[b][2]]);
+            ++n;
+        }
+    }
+    t.npair = n;
+    return t;
+}
+
+__constant__ Tables c_tables = make_tables();
+
+__device__ __forceinline__ double ipow(double v, int k) {
+    double r = 1.0;
+    for (int i = 0; i < k; ++i) r *= v;
+    return r;
+}
+
+// ---- 1. bounding box -----------------------------------------------------
+__global__ __launch_bounds__(256) void bbox_partial_kernel(int n, const double* __restrict__ x,
+                                                           const double* __restrict__ y,
+                                                           const double* __restrict__ z,
+                                                           double* __restrict__ part) {
+    __shared__ double red[6][256];
+    double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double p[3] = {x[i], y[i], z[i]};
+        for (int d = 0; d < 3; ++d) {
+            v[d] = fmin(v[d], p[d]);
+            v[3 + d] = fmax(v[3 + d], p[d]);
+        }
+    }
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int d = 0; d < 3; ++d) {
+                red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+                red[3 + d][threadIdx.x] = fmax(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + s]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Root cube of fmm_solver.cpp:76-85; node 0 of the node arrays.
+__global__ void bbox_final_kernel(int nblocks, const double* __restrict__ part, double eps, int n,
+                                  double* __restrict__ geom, int* __restrict__ nb,
+                                  int* __restrict__ ne) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nblocks; ++b)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], part[b * 6 + d]);
+            hi[d] = fmax(hi[d], part[b * 6 + 3 + d]);
+        }
+    double ext = hi[0] - lo[0];
+    if (ext < hi[1] - lo[1]) ext = hi[1] - lo[1];
+    if (ext < hi[2] - lo[2]) ext = hi[2] - lo[2];
+    double hw = 0.5 * ext;
+    if (hw < eps + 1.0e-6) hw = eps + 1.0e-6;
+    hw *= 1.0001;
+    for (int d = 0; d < 3; ++d) geom[d] = (lo[d] + hi[d]) * 0.5;
+    geom[3] = hw;
+    nb[0] = 0;
+    ne[0] = n;
+}
+
+// ---- 2. octant keys --------------------------------------------------------
+__global__ __launch_bounds__(256) void keys_kernel(int n, const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z,
+                                                   const double* __restrict__ root,
+                                                   unsigned long long* __restrict__ keys,
+                                                   int* __restrict__ idx) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double c0 = root[0], c1 = root[1], c2 = root[2], hw = root[3];
+    const double px = x[i], py = y[i], pz = z[i];
+    unsigned long long key = 0;
+    for (int l = 0; l < kLevels; ++l) {
+        const int ox = px >= c0 ? 1 : 0, oy = py >= c1 ? 1 : 0, oz = pz >= c2 ? 1 : 0;
+        key = (key << 3) | static_cast<unsigned long long>(ox + 2 * oy + 4 * oz);
+        const double chw = hw * 0.5;
+        c0 = c0 + (ox ? 1.0 : -1.0) * chw;
+        c1 = c1 + (oy ? 1.0 : -1.0) * chw;
+        c2 = c2 + (oz ? 1.0 : -1.0) * chw;
+        hw = chw;
+    }
+    keys[i] = key;
+    idx[i] = i;
+}
+
+// ---- 4. level split -------------------------------------------------------
+__device__ __forceinline__ int digit_at(unsigned long long key, int level /*1..kLevels*/) {
+    return static_cast<int>((key >> (3 * (kLevels - level))) & 7ull);
+}
+
+// Children of node k at level L: bounds[9] in sorted order, nonempty count.
+__global__ __launch_bounds__(256) void split_count_kernel(
+    int lb, int le, int level, int cap, int max_depth, const unsigned long long* __restrict__ keys,
+    const int* __restrict__ nb, const int* __restrict__ ne, int* __restrict__ bounds,
+    int* __restrict__ count, int* __restrict__ overflow) {
+    const int k = lb + blockIdx.x * 256 + threadIdx.x;
+    if (k >= le) return;
+    const int b = nb[k], e = ne[k];
+    const int j = k - lb;
+    int* bd = bounds + 9 * j;
+    const bool internal = (e - b) > cap && level < max_depth;
+    if (!internal) {
+        count[j] = 0;
+        return;
+    }
+    if (level >= kLevels) {
+        atomicOr(overflow, 1);
+        count[j] = 0;
+        return;
+    }
+    bd[0] = b;
+    bd[8] = e;
+    for (int d = 1; d < 8; ++d) {  // first position with digit >= d
+        int lo = b, hi = e;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (digit_at(keys[mid], level + 1) < d) lo = mid + 1;
+            else hi = mid;
+        }
+        bd[d] = lo;
+    }
+    int c = 0;
+    for (int d = 0; d < 8; ++d) c += bd[d + 1] > bd[d] ? 1 : 0;
+    count[j] = c;
+}
+
+__global__ __launch_bounds__(256) void split_emit_kernel(
+    int lb, int le, const int* __restrict__ bounds, const int* __restrict__ count,
+    const int* __restrict__ offset, double* __restrict__ geom, int* __restrict__ nb,
+    int* __restrict__ ne, int* __restrict__ first_child, int* __restrict__ nchild) {
+#pragma clang fp contract(off)
+    const int k = lb + blockIdx.x * 256 + threadIdx.x;
+    if (k >= le) return;
+    const int j = k - lb;
+    const int c = count[j];
+    nchild[k] = c;
+    if (c == 0) {
+        first_child[k] = -1;
+        return;
+    }
+    const int* bd = bounds + 9 * j;
+    int pos = le + offset[j];
+    first_child[k] = pos;
+    const double chw = geom[4 * k + 3] * 0.5;
+    for (int d = 0; d < 8; ++d) {
+        if (bd[d + 1] <= bd[d]) continue;
+        geom[4 * pos + 0] = geom[4 * k + 0] + ((d & 1) ? 1.0 : -1.0) * chw;
+        geom[4 * pos + 1] = geom[4 * k + 1] + ((d & 2) ? 1.0 : -1.0) * chw;
+        geom[4 * pos + 2] = geom[4 * k + 2] + ((d & 4) ? 1.0 : -1.0) * chw;
+        geom[4 * pos + 3] = chw;
+        nb[pos] = bd[d];
+        ne[pos] = bd[d + 1];
+        ++pos;
+    }
+}
+
+// ---- 5. upward pass -------------------------------------------------------
+__global__ __launch_bounds__(64) void upward_kernel(
+    int lb, int le, const int* __restrict__ nb, const int* __restrict__ ne,
+    const int* __restrict__ first_child, const int* __restrict__ nchild,
+    const double* __restrict__ geom, const int* __restrict__ sidx, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    double* __restrict__ mcom, double* __restrict__ mom, float* __restrict__ n32) {
+#pragma clang fp contract(off)
+    const int k = lb + blockIdx.x * 64 + threadIdx.x;
+    if (k >= le) return;
+    double mass = 0.0, w[3] = {0.0, 0.0, 0.0}, M[kMom];
+    for (int a = 0; a < kMom; ++a) M[a] = 0.0;
+    const int nc = nchild[k];
+    if (nc == 0) {
+        for (int s = nb[k]; s < ne[k]; ++s) {
+            const int i = sidx[s];
+            mass += m[i];
+            w[0] += x[i] * m[i];
+            w[1] += y[i] * m[i];
+            w[2] += z[i] * m[i];
+        }
+    } else {
+        for (int c = first_child[k]; c < first_child[k] + nc; ++c) {
+            const double cm = mcom[4 * c];
+            mass += cm;
+            for (int d = 0; d < 3; ++d) w[d] += mcom[4 * c + 1 + d] * cm;
+        }
+    }
+    double com[3];
+    for (int d = 0; d < 3; ++d) com[d] = mass > 0.0 ? w[d] / mass : geom[4 * k + d];
+    if (nc == 0) {
+        for (int s = nb[k]; s < ne[k]; ++s) {
+            const int i = sidx[s];
+            const double o[3] = {x[i] - com[0], y[i] - com[1], z[i] - com[2]};
+            for (int a = 4; a < kMom; ++a)
+                M[a] += m[i] * (ipow(o[0], c_tables.e[a][0]) * ipow(o[1], c_tables.e[a][1]) *
+                                ipow(o[2], c_tables.e[a][2]));
+        }
+    } else {
+        for (int c = first_child[k]; c < first_child[k] + nc; ++c) {
+            const double cm = mcom[4 * c];
+            const double sh[3] = {mcom[4 * c + 1] - com[0], mcom[4 * c + 2] - com[1],
+                                  mcom[4 * c + 3] - com[2]};
+            const double* cmom = mom + static_cast<size_t>(kMom) * c;
+            for (int t = 0; t < c_tables.npair; ++t) {
+                const int b = c_tables.pb[t];
+                const double operand = b == 0 ? cm : cmom[b];
+                M[c_tables.pa[t]] += c_tables.coef[t] * ipow(sh[0], c_tables.sx[t]) *
+                                     ipow(sh[1], c_tables.sy[t]) * ipow(sh[2], c_tables.sz[t]) *
+                                     operand;
+            }
+        }
+    }
+    mcom[4 * k] = mass;
+    for (int d = 0; d < 3; ++d) mcom[4 * k + 1 + d] = com[d];
+    double* out = mom + static_cast<size_t>(kMom) * k;
+    float* nf = n32 + static_cast<size_t>(kN32) * k;
+    for (int a = 0; a < kMom; ++a) {
+        out[a] = M[a];
+        nf[a] = static_cast<float>(a == 0 ? mass : M[a] * c_tables.inv_fact[a]);
+    }
+    nf[kMom] = 0.f;
+}
+
+// leaf flags for the scan of target leaves: non-empty leaves with mass > 0
+__global__ __launch_bounds__(256) void leaf_flag_kernel(int nnodes, const int* __restrict__ nchild,
+                                                        const double* __restrict__ mcom,
+                                                        int* __restrict__ flag) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnodes) return;
+    flag[k] = (nchild[k] == 0 && mcom[4 * k] > 0.0) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void leaf_collect_kernel(int nnodes, const int* __restrict__ flag,
+                                                           const int* __restrict__ pos,
+                                                           int* __restrict__ leaves) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnodes) return;
+    if (flag[k]) leaves[pos[k]] = k;
+}
+
+// ---- 6. interaction lists ----------------------------------------------
+__device__ __forceinline__ bool well_separated(const double* __restrict__ geom, int t, int s,
+                                               double theta) {
+#pragma clang fp contract(off)
+    const double dx = geom[4 * s] - geom[4 * t], dy = geom[4 * s + 1] - geom[4 * t + 1],
+                 dz = geom[4 * s + 2] - geom[4 * t + 2];
+    const double dist = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
+    if (dist == 0.0) return false;
+    const double w = 2.0 * (geom[4 * t + 3] + geom[4 * s + 3]);
+    return w / dist < theta;
+}
+
+// FILL == false: count far cells, near leaves and near source particles
+// per target leaf; FILL == true: write the lists at the scanned offsets.
+template <bool FILL>
+__global__ __launch_bounds__(64) void lists_kernel(
+    int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
+    const double* __restrict__ mcom, const int* __restrict__ first_child,
+    const int* __restrict__ nchild, const int* __restrict__ nb, const int* __restrict__ ne,
+    double theta, int* __restrict__ nfar, int* __restrict__ nnear, int* __restrict__ nsrc,
+    const long long* __restrict__ far_off, const long long* __restrict__ near_off,
+    int* __restrict__ far, int* __restrict__ near) {
+    const int li = blockIdx.x * 64 + threadIdx.x;
+    if (li >= nleaves) return;
+    const int t = leaves[li];
+    int stack[kStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    int cf = 0, cn = 0, cs = 0;
+    long long fo = FILL ? far_off[li] : 0, no = FILL ? near_off[li] : 0;
+    while (sp > 0) {
+        const int s = stack[--sp];
+        if (mcom[4 * s] <= 0.0) continue;
+        if (s != t && well_separated(geom, t, s, theta)) {
+            if (FILL) far[fo + cf] = s;
+            ++cf;
+            continue;
+        }
+        const int nc = nchild[s];
+        if (nc == 0) {
+            if (FILL) near[no + cn] = s;
+            ++cn;
+            cs += ne[s] - nb[s];
+            continue;
+        }
+        for (int c = first_child[s] + nc - 1; c >= first_child[s]; --c) stack[sp++] = c;
+    }
+    if (!FILL) {
+        nfar[li] = cf;
+        nnear[li] = cn;
+        nsrc[li] = cs;
+    }
+}
+
+// ---- 7. evaluation ----------------------------------------------------------
+template <typename F>
+struct Src {
+    F x, y, z, gm;
+    int id;
+};
+
+template <typename F, int ORDER, bool SAFE>
+__global__ __launch_bounds__(kEvalLanes) void eval_kernel(
+    int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
+    const double* __restrict__ mcom, const float* __restrict__ n32, const int* __restrict__ nb,
+    const int* __restrict__ ne, const int* __restrict__ sidx, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    const long long* __restrict__ far_off, const int* __restrict__ nfar,
+    const int* __restrict__ far, const long long* __restrict__ near_off,
+    const int* __restrict__ nnear, const int* __restrict__ near, double G, double eps, int begin,
+    int end, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, int close_kick,
+    double half_dt) {
+#pragma clang fp contract(off)
+    __shared__ Src<F> tile[kEvalLanes];
+    const int leaf = leaves[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    const F eps2 = static_cast<F>(eps * eps);
+    const int tb = nb[leaf], te = ne[leaf];
+    const long long fo = far_off[blockIdx.x], no = near_off[blockIdx.x];
+    const int nf = nfar[blockIdx.x], nn = nnear[blockIdx.x];
+    for (int base = tb; base < te; base += kEvalLanes) {
+        const int s = base + lane;
+        const bool active = s < te;
+        const int me = active ? sidx[s] : -1;
+        const F tx = active ? static_cast<F>(x[me] - cx) : F(0);
+        const F ty = active ? static_cast<F>(y[me] - cy) : F(0);
+        const F tz = active ? static_cast<F>(z[me] - cz) : F(0);
+        double far_a[3] = {0.0, 0.0, 0.0}, near_a[3] = {0.0, 0.0, 0.0};
+        for (int f = 0; f < nf; ++f) {
+            const int cell = far[fo + f];
+            const F yx = static_cast<F>(mcom[4 * cell + 1] - cx) - tx;
+            const F yy = static_cast<F>(mcom[4 * cell + 2] - cy) - ty;
+            const F yz = static_cast<F>(mcom[4 * cell + 3] - cz) - tz;
+            F N[kMom];
+            const float* src = n32 + static_cast<size_t>(kN32) * cell;
+#pragma unroll
+            for (int a = 0; a < kMom; ++a) N[a] = static_cast<F>(src[a]);
+            F fx = 0, fy = 0, fz = 0;
+            if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
+            else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
+            else m2p_order4<F>(yx, yy, yz, eps2, N, fx, fy, fz);
+            far_a[0] += static_cast<double>(fx);
+            far_a[1] += static_cast<double>(fy);
+            far_a[2] += static_cast<double>(fz);
+        }
+        for (int q = 0; q < nn; ++q) {
+            const int src_leaf = near[no + q];
+            const int sb = nb[src_leaf], se = ne[src_leaf];
+            const bool self_leaf = src_leaf == leaf;
+            for (int cb = sb; cb < se; cb += kEvalLanes) {
+                __syncthreads();
+                const int j = cb + lane;
+                if (j < se) {
+                    const int id = sidx[j];
+                    Src<F> t;
+                    t.x = static_cast<F>(x[id] - cx);
+                    t.y = static_cast<F>(y[id] - cy);
+                    t.z = static_cast<F>(z[id] - cz);
+                    t.gm = static_cast<F>(G * m[id]);
+                    t.id = id;
+                    tile[lane] = t;
+                }
+                __syncthreads();
+                const int cnt = min(kEvalLanes, se - cb);
+                F px = 0, py = 0, pz = 0;
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const Src<F> sj = tile[jj];
+                    if (self_leaf && sj.id == me) continue;
+                    const F dx = sj.x - tx, dy = sj.y - ty, dz = sj.z - tz;
+                    const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+                    F r;
+                    if (SAFE) {
+                        r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
+                    } else {
+                        r = gc_rsq(s2);
+                    }
+                    const F k = (sj.gm * r) * (r * r);
+                    px = gc_fma(dx, k, px);
+                    py = gc_fma(dy, k, py);
+                    pz = gc_fma(dz, k, pz);
+                }
+                near_a[0] += static_cast<double>(px);
+                near_a[1] += static_cast<double>(py);
+                near_a[2] += static_cast<double>(pz);
+            }
+        }
+        if (!active || me < begin || me >= end) continue;
+        const double a0 = G * far_a[0] + near_a[0];
+        const double a1 = G * far_a[1] + near_a[1];
+        const double a2 = G * far_a[2] + near_a[2];
+        ax[me] = a0;
+        ay[me] = a1;
+        az[me] = a2;
+        if (close_kick) {
+            vx[me] = vx[me] + a0 * half_dt;
+            vy[me] = vy[me] + a1 * half_dt;
+            vz[me] = vz[me] + a2 * half_dt;
+        }
+    }
+}
+
+// zero accelerations of [begin, end) (particles in massless leaves keep 0,
+// as in fmm_solver.cpp:31 + :210-226)
+__global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, double* __restrict__ ax,
+                                                         double* __restrict__ ay,
+                                                         double* __restrict__ az) {
+    const int i = begin + blockIdx.x * 256 + threadIdx.x;
+    if (i >= end) return;
+    ax[i] = 0.0;
+    ay[i] = 0.0;
+    az[i] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void stats_kernel(
+    int nleaves, const int* __restrict__ leaves, const int* __restrict__ nb,
+    const int* __restrict__ ne, const int* __restrict__ nfar, const int* __restrict__ nnear,
+    const int* __restrict__ nsrc, unsigned long long* __restrict__ out) {
+    const int li = blockIdx.x * 256 + threadIdx.x;
+    if (li >= nleaves) return;
+    const long long cnt = ne[leaves[li]] - nb[leaves[li]];
+    atomicAdd(&out[0], static_cast<unsigned long long>(nfar[li]));
+    atomicAdd(&out[1], static_cast<unsigned long long>(nnear[li]));
+    atomicAdd(&out[2], static_cast<unsigned long long>(cnt * nfar[li]));
+    atomicAdd(&out[3], static_cast<unsigned long long>(cnt * nsrc[li] - cnt));
+}
+
+struct ToI64 {
+    __host__ __device__ long long operator()(int v) const { return static_cast<long long>(v); }
+};
+
+template <typename T>
+hipError_t grow(T** p, size_t* cap, size_t need, size_t keep, hipStream_t st) {
+    if (need <= *cap) return hipSuccess;
+    size_t ncap = std::max(need, *cap * 2);
+    T* q = nullptr;
+    hipError_t e = hipMalloc(&q, sizeof(T) * ncap);
+    if (e != hipSuccess) return e;
+    if (*p && keep) {
+        e = hipMemcpyAsync(q, *p, sizeof(T) * keep, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return e;
+    }
+    (void)hipFree(*p);
+    *p = q;
+    *cap = ncap;
+    return hipSuccess;
+}
+
+}  // namespace
+
+struct FmmEngine::Impl {
+    // particles
+    size_t pcap = 0;
+    unsigned long long *keys = nullptr, *keys_sorted = nullptr;
+    int *idx = nullptr, *idx_sorted = nullptr;
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    double* bbox_part = nullptr;
+    // nodes (one growth unit per array)
+    size_t ncap = 0;
+    double* geom = nullptr;  // 4 per node: cx, cy, cz, hw
+    double* mcom = nullptr;  // 4 per node: mass, com
+    double* mom = nullptr;   // 35 per node
+    float* n32 = nullptr;    // 36 per node
+    int *nb = nullptr, *ne = nullptr, *first_child = nullptr, *nchild = nullptr;
+    // per-level scratch
+    size_t lcap = 0;
+    int *bounds = nullptr, *count = nullptr, *offset = nullptr;
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    int* overflow = nullptr;
+    // leaves and lists
+    size_t leafcap = 0;
+    int *flag = nullptr, *flag_pos = nullptr, *leaves = nullptr;
+    int *nfar = nullptr, *nnear = nullptr, *nsrc = nullptr;
+    long long *far_off = nullptr, *near_off = nullptr;
+    size_t farcap = 0, nearcap = 0;
+    int *far = nullptr, *near = nullptr;
+    unsigned long long* stats_dev = nullptr;
+    // host-side results of the last build
+    std::vector<int> level_begin;
+    int nnodes = 0, nleaves = 0;
+    long long total_far = 0, total_near = 0;
+    gc_fmm_stats last{};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+
+    ~Impl() {
+        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sort_tmp, bbox_part, geom, mcom, mom,
+                        n32, nb, ne, first_child, nchild, bounds, count, offset, scan_tmp,
+                        overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off, near_off,
+                        far, near, stats_dev};
+        for (void* p : ptrs) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    // Grows every node array to `nodes` entries keeping the first `keep`
+    // (geometry and topology; mass/moments are rewritten by the upward pass).
+    hipError_t grow_nodes(size_t nodes, size_t keep, hipStream_t st) {
+        size_t c;
+        hipError_t e;
+        c = 4 * ncap; if ((e = grow(&geom, &c, 4 * nodes, 4 * keep, st)) != hipSuccess) return e;
+        c = 4 * ncap; if ((e = grow(&mcom, &c, 4 * nodes, 0, st)) != hipSuccess) return e;
+        c = kMom * ncap; if ((e = grow(&mom, &c, kMom * nodes, 0, st)) != hipSuccess) return e;
+        c = kN32 * ncap; if ((e = grow(&n32, &c, kN32 * nodes, 0, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&nb, &c, nodes, keep, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&ne, &c, nodes, keep, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&first_child, &c, nodes, keep, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&nchild, &c, nodes, keep, st)) != hipSuccess) return e;
+        ncap = nodes;
+        return hipSuccess;
+    }
+
+    int build(const DeviceState& s, const gc_params& p, hipStream_t st, std::string* err);
+    int launch_eval(const DeviceState& s, const gc_params& p, size_t begin, size_t end,
+                    bool close_kick, double half_dt, hipStream_t st, std::string* err);
+};
+
+#define FMM_HIP(call, what)                                                                     \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            if (err) *err = std::string(what) + ": " + hipGetErrorString(e_);                   \
+            return e_ == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP;               \
+        }                                                                                       \
+    } while (0)
+
+int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t st,
+                           std::string* err) {
+    const int n = static_cast<int>(s.n);
+    const int cap = std::max(1, p.leaf_capacity);
+    const int max_depth = std::max(1, p.max_depth);
+    const double theta = std::max(1.0e-6, p.theta);
+    const int bblocks = std::min(1024, (n + 255) / 256);
+
+    // particle buffers
+    if (static_cast<size_t>(n) > pcap) {
+        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sort_tmp, bbox_part};
+        for (void* q : ptrs) (void)hipFree(q);
+        pcap = static_cast<size_t>(n);
+        FMM_HIP(hipMalloc(&keys, sizeof(unsigned long long) * pcap), "allocate keys");
+        FMM_HIP(hipMalloc(&keys_sorted, sizeof(unsigned long long) * pcap), "allocate keys");
+        FMM_HIP(hipMalloc(&idx, sizeof(int) * pcap), "allocate indices");
+        FMM_HIP(hipMalloc(&idx_sorted, sizeof(int) * pcap), "allocate indices");
+        FMM_HIP(hipMalloc(&bbox_part, sizeof(double) * 6 * 1024), "allocate bbox partials");
+        sort_tmp_bytes = 0;
+        FMM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp_bytes, keys, keys_sorted, idx,
+                                                   idx_sorted, n, 0, 3 * kLevels, st),
+                "size radix sort");
+        FMM_HIP(hipMalloc(&sort_tmp, sort_tmp_bytes), "allocate radix sort storage");
+        if (!overflow) FMM_HIP(hipMalloc(&overflow, sizeof(int)), "allocate flag");
+        if (!stats_dev) FMM_HIP(hipMalloc(&stats_dev, sizeof(unsigned long long) * 4), "allocate stats");
+    }
+    const size_t need_nodes = std::max<size_t>(1024, 2 * static_cast<size_t>(n) / cap + 64);
+    if (need_nodes > ncap) FMM_HIP(grow_nodes(need_nodes, 0, st), "allocate nodes");
+
+    // 1-3: root, keys, sort
+    hipLaunchKernelGGL(bbox_partial_kernel, dim3(bblocks), dim3(256), 0, st, n, s.x, s.y, s.z,
+                       bbox_part);
+    hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, st, bblocks, bbox_part,
+                       p.softening, n, geom, nb, ne);
+    hipLaunchKernelGGL(keys_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, s.x, s.y, s.z,
+                       geom, keys, idx);
+    FMM_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_tmp_bytes, keys, keys_sorted, idx,
+                                               idx_sorted, n, 0, 3 * kLevels, st),
+            "radix sort keys");
+    FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
+
+    // 4: levels
+    level_begin.assign(1, 0);
+    level_begin.push_back(1);
+    int internal = 0;
+    for (int level = 0;; ++level) {
+        const int lb = level_begin[level], le = level_begin[level + 1];
+        const int cnt = le - lb;
+        if (cnt == 0) break;
+        if (static_cast<size_t>(cnt) > lcap) {
+            (void)hipFree(bounds);
+            (void)hipFree(count);
+            (void)hipFree(offset);
+            (void)hipFree(scan_tmp);
+            lcap = static_cast<size_t>(cnt) * 2;
+            FMM_HIP(hipMalloc(&bounds, sizeof(int) * 9 * lcap), "allocate level scratch");
+            FMM_HIP(hipMalloc(&count, sizeof(int) * lcap), "allocate level scratch");
+            FMM_HIP(hipMalloc(&offset, sizeof(int) * lcap), "allocate level scratch");
+            scan_tmp_bytes = 0;
+            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp_bytes, count, offset,
+                                                     static_cast<int>(lcap), st),
+                    "size scan");
+            FMM_HIP(hipMalloc(&scan_tmp, scan_tmp_bytes), "allocate scan storage");
+        }
+        const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
+        hipLaunchKernelGGL(split_count_kernel, dim3(blocks), dim3(256), 0, st, lb, le, level, cap,
+                           max_depth, keys_sorted, nb, ne, bounds, count, overflow);
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, count, offset, cnt, st),
+                "scan children");
+        int last[2] = {0, 0};
+        FMM_HIP(hipMemcpyAsync(&last[0], offset + cnt - 1, sizeof(int), hipMemcpyDeviceToHost, st),
+                "read level size");
+        FMM_HIP(hipMemcpyAsync(&last[1], count + cnt - 1, sizeof(int), hipMemcpyDeviceToHost, st),
+                "read level size");
+        FMM_HIP(hipStreamSynchronize(st), "level size");
+        const int next = last[0] + last[1];
+        const size_t need = static_cast<size_t>(le) + next;
+        if (need > ncap)
+            FMM_HIP(grow_nodes(std::max(need, ncap * 2), static_cast<size_t>(le), st), "grow nodes");
+        hipLaunchKernelGGL(split_emit_kernel, dim3(blocks), dim3(256), 0, st, lb, le, bounds, count,
+                           offset, geom, nb, ne, first_child, nchild);
+        level_begin.push_back(le + next);
+    }
+    int ovf = 0;
+    FMM_HIP(hipMemcpyAsync(&ovf, overflow, sizeof(int), hipMemcpyDeviceToHost, st), "read flag");
+    FMM_HIP(hipStreamSynchronize(st), "tree build");
+    if (ovf) {
+        if (err)
+            *err = "octree deeper than 21 levels (coincident or extremely clustered particles); "
+                   "lower max_depth or raise leaf_capacity";
+        return GC_ERR_UNSUPPORTED;
+    }
+    nnodes = level_begin.back();
+    const int depth = static_cast<int>(level_begin.size()) - 3;  // deepest non-empty level
+
+    // 5: upward, deepest level first
+    for (int level = depth; level >= 0; --level) {
+        const int lb = level_begin[level], le = level_begin[level + 1];
+        if (le <= lb) continue;
+        hipLaunchKernelGGL(upward_kernel, dim3((le - lb + 63) / 64), dim3(64), 0, st, lb, le, nb, ne,
+                           first_child, nchild, geom, idx_sorted, s.x, s.y, s.z, s.m, mcom, mom, n32);
+    }
+
+    // leaves (mass > 0)
+    if (static_cast<size_t>(nnodes) > leafcap) {
+        void* ptrs[] = {flag, flag_pos, leaves, nfar, nnear, nsrc, far_off, near_off};
+        for (void* q : ptrs) (void)hipFree(q);
+        leafcap = static_cast<size_t>(nnodes) * 2;
+        FMM_HIP(hipMalloc(&flag, sizeof(int) * leafcap), "allocate leaves");
+        FMM_HIP(hipMalloc(&flag_pos, sizeof(int) * leafcap), "allocate leaves");
+        FMM_HIP(hipMalloc(&leaves, sizeof(int) * leafcap), "allocate leaves");
+        FMM_HIP(hipMalloc(&nfar, sizeof(int) * leafcap), "allocate lists");
+        FMM_HIP(hipMalloc(&nnear, sizeof(int) * leafcap), "allocate lists");
+        FMM_HIP(hipMalloc(&nsrc, sizeof(int) * leafcap), "allocate lists");
+        FMM_HIP(hipMalloc(&far_off, sizeof(long long) * leafcap), "allocate lists");
+        FMM_HIP(hipMalloc(&near_off, sizeof(long long) * leafcap), "allocate lists");
+    }
+    {
+        size_t need = 0;
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, flag_pos, nnodes, st), "size scan");
+        size_t need2 = 0;
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, nfar, far_off, nnodes, st), "size scan");
+        need = std::max(need, need2);
+        if (need > scan_tmp_bytes) {
+            (void)hipFree(scan_tmp);
+            scan_tmp_bytes = need;
+            FMM_HIP(hipMalloc(&scan_tmp, scan_tmp_bytes), "allocate scan storage");
+        }
+    }
+    const unsigned nblk = static_cast<unsigned>((nnodes + 255) / 256);
+    hipLaunchKernelGGL(leaf_flag_kernel, dim3(nblk), dim3(256), 0, st, nnodes, nchild, mcom, flag);
+    FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, flag, flag_pos, nnodes, st),
+            "scan leaves");
+    hipLaunchKernelGGL(leaf_collect_kernel, dim3(nblk), dim3(256), 0, st, nnodes, flag, flag_pos,
+                       leaves);
+    int lastf[2] = {0, 0};
+    FMM_HIP(hipMemcpyAsync(&lastf[0], flag_pos + nnodes - 1, sizeof(int), hipMemcpyDeviceToHost, st), "leaf count");
+    FMM_HIP(hipMemcpyAsync(&lastf[1], flag + nnodes - 1, sizeof(int), hipMemcpyDeviceToHost, st), "leaf count");
+    FMM_HIP(hipStreamSynchronize(st), "leaf count");
+    nleaves = lastf[0] + lastf[1];
+    if (nleaves == 0) {
+        total_far = total_near = 0;
+        return GC_OK;
+    }
+
+    // 6: lists (count, scan, fill)
+    const unsigned lblk = static_cast<unsigned>((nleaves + 63) / 64);
+    hipLaunchKernelGGL((lists_kernel<false>), dim3(lblk), dim3(64), 0, st, nleaves, leaves, geom,
+                       mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
+                       near_off, far, near);
+    // 64-bit offsets: widen the counts with a scan over long long
+    {
+        // far_off/near_off hold exclusive sums of nfar/nnear
+        auto widen = [&](const int* in, long long* out) -> hipError_t {
+            size_t need = 0;
+            hipcub::TransformInputIterator<long long, ToI64, const int*> it(in, ToI64());
+            hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, out, nleaves, st);
+            if (e != hipSuccess) return e;
+            if (need > scan_tmp_bytes) {
+                (void)hipFree(scan_tmp);
+                scan_tmp_bytes = need;
+                e = hipMalloc(&scan_tmp, scan_tmp_bytes);
+                if (e != hipSuccess) return e;
+            }
+            return hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, it, out, nleaves, st);
+        };
+        FMM_HIP(widen(nfar, far_off), "scan far lists");
+        FMM_HIP(widen(nnear, near_off), "scan near lists");
+    }
+    long long lo[2] = {0, 0};
+    int lc[2] = {0, 0};
+    FMM_HIP(hipMemcpyAsync(&lo[0], far_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "list size");
+    FMM_HIP(hipMemcpyAsync(&lo[1], near_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "list size");
+    FMM_HIP(hipMemcpyAsync(&lc[0], nfar + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
+    FMM_HIP(hipMemcpyAsync(&lc[1], nnear + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
+    FMM_HIP(hipStreamSynchronize(st), "list sizes");
+    total_far = lo[0] + lc[0];
+    total_near = lo[1] + lc[1];
+    if (static_cast<size_t>(total_far) + 1 > farcap) {
+        (void)hipFree(far);
+        farcap = static_cast<size_t>(total_far) + 1 + static_cast<size_t>(total_far) / 4;
+        FMM_HIP(hipMalloc(&far, sizeof(int) * farcap), "allocate far lists");
+    }
+    if (static_cast<size_t>(total_near) + 1 > nearcap) {
+        (void)hipFree(near);
+        nearcap = static_cast<size_t>(total_near) + 1 + static_cast<size_t>(total_near) / 4;
+        FMM_HIP(hipMalloc(&near, sizeof(int) * nearcap), "allocate near lists");
+    }
+    hipLaunchKernelGGL((lists_kernel<true>), dim3(lblk), dim3(64), 0, st, nleaves, leaves, geom,
+                       mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
+                       near_off, far, near);
+
+    // statistics (FmmStats, fmm_solver.hpp:20-25, plus pair counts)
+    FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
+    hipLaunchKernelGGL(stats_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves, leaves,
+                       nb, ne, nfar, nnear, nsrc, stats_dev);
+    unsigned long long sv[4];
+    FMM_HIP(hipMemcpyAsync(sv, stats_dev, sizeof(sv), hipMemcpyDeviceToHost, st), "read stats");
+    std::vector<int> nch(static_cast<size_t>(nnodes));
+    FMM_HIP(hipMemcpyAsync(nch.data(), nchild, sizeof(int) * nnodes, hipMemcpyDeviceToHost, st), "read tree");
+    FMM_HIP(hipStreamSynchronize(st), "stats");
+    internal = 0;
+    for (int v : nch) internal += v > 0 ? 1 : 0;
+    last.node_count = 1 + 8 * static_cast<size_t>(internal);
+    last.leaf_count = static_cast<size_t>(nleaves);
+    last.far_interactions = sv[0];
+    last.near_interactions = sv[1];
+    last.m2p_evaluations = sv[2];
+    last.p2p_pairs = sv[3];
+    last.max_depth = depth;
+    last.tree_nodes = static_cast<size_t>(nnodes);
+    return GC_OK;
+}
+
+int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_t begin, size_t end,
+                                 bool close_kick, double half_dt, hipStream_t st, std::string* err) {
+    int order = std::min(4, std::max(0, p.expansion_order));
+    if (order == 1 || order == 3) ++order;
+    const bool mixed = p.precision == GC_PRECISION_MIXED;
+    const bool safe = !(static_cast<float>(p.softening * p.softening) >= 1e-30f);
+    const dim3 grid(static_cast<unsigned>(nleaves)), block(kEvalLanes);
+#define GC_EVAL(F, O, S)                                                                         \
+    hipLaunchKernelGGL((eval_kernel<F, O, S>), grid, block, 0, st, nleaves, leaves, geom, mcom,  \
+                       n32, nb, ne, idx_sorted, s.x, s.y, s.z, s.m, far_off, nfar, far,          \
+                       near_off, nnear, near, p.gravitational_constant, p.softening,             \
+                       static_cast<int>(begin), static_cast<int>(end), s.ax, s.ay, s.az, s.vx,   \
+                       s.vy, s.vz, close_kick ? 1 : 0, half_dt)
+#define GC_EVAL_ORDER(F, S)                         \
+    do {                                            \
+        if (order == 0) GC_EVAL(F, 0, S);           \
+        else if (order == 2) GC_EVAL(F, 2, S);      \
+        else GC_EVAL(F, 4, S);                      \
+    } while (0)
+    if (mixed) {
+        if (safe) GC_EVAL_ORDER(float, true);
+        else GC_EVAL_ORDER(float, false);
+    } else {
+        if (safe) GC_EVAL_ORDER(double, true);
+        else GC_EVAL_ORDER(double, false);
+    }
+#undef GC_EVAL_ORDER
+#undef GC_EVAL
+    FMM_HIP(hipGetLastError(), "launch FMM evaluation");
+    return GC_OK;
+}
 
 FmmEngine::FmmEngine() : impl_(new Impl()) {}
 FmmEngine::~FmmEngine() { delete impl_; }
 
-int FmmEngine::compute(const DeviceState&, const gc_params&, size_t, size_t, bool, double,
-                       hipStream_t, std::string* err) {
-    if (err) *err = "the FMM solver is not available in this build";
-    return GC_ERR_UNSUPPORTED;
+const gc_fmm_stats& FmmEngine::stats() const { return impl_->last; }
+
+int FmmEngine::compute(const DeviceState& s, const gc_params& p, size_t begin, size_t end,
+                       bool close_kick, double half_dt, hipStream_t st, std::string* err) {
+    end = std::min(end, s.n);
+    if (s.n == 0 || begin >= end) return GC_OK;
+    for (auto& e : impl_->ev)
+        if (!e) FMM_HIP(hipEventCreate(&e), "create FMM events");
+    FMM_HIP(hipEventRecord(impl_->ev[0], st), "record build start");
+    int rc = impl_->build(s, p, st, err);
+    if (rc != GC_OK) return rc;
+    FMM_HIP(hipEventRecord(impl_->ev[1], st), "record build end");
+    const unsigned zb = static_cast<unsigned>((end - begin + 255) / 256);
+    hipLaunchKernelGGL(zero_range_kernel, dim3(zb), dim3(256), 0, st, static_cast<int>(begin),
+                       static_cast<int>(end), s.ax, s.ay, s.az);
+    if (impl_->nleaves == 0) return GC_OK;
+    rc = impl_->launch_eval(s, p, begin, end, close_kick, half_dt, st, err);
+    FMM_HIP(hipEventRecord(impl_->ev[2], st), "record eval end");
+    return rc;
 }
 
-int FmmEngine::time_kernels(const DeviceState&, const gc_params&, int, hipStream_t, double*,
-                            double*, std::string* err) {
-    if (err) *err = "the FMM solver is not available in this build";
-    return GC_ERR_UNSUPPORTED;
+int FmmEngine::time_kernels(const DeviceState& s, const gc_params& p, int reps, hipStream_t st,
+                            double* ms, double* interactions, std::string* err) {
+    int rc = compute(s, p, 0, s.n, false, 0.0, st, err);
+    if (rc != GC_OK) return rc;
+    if (impl_->nleaves == 0) {
+        *ms = 0.0;
+        if (interactions) *interactions = 0.0;
+        return GC_OK;
+    }
+    FMM_HIP(hipEventRecord(impl_->ev[3], st), "record");
+    for (int r = 0; r < reps; ++r) {
+        rc = impl_->launch_eval(s, p, 0, s.n, false, 0.0, st, err);
+        if (rc != GC_OK) return rc;
+    }
+    FMM_HIP(hipEventRecord(impl_->ev[4], st), "record");
+    FMM_HIP(hipEventSynchronize(impl_->ev[4]), "wait");
+    float t = 0.f;
+    FMM_HIP(hipEventElapsedTime(&t, impl_->ev[3], impl_->ev[4]), "elapsed");
+    *ms = static_cast<double>(t) / reps;
+    if (interactions) *interactions = static_cast<double>(impl_->last.p2p_pairs);
+    return GC_OK;
+}
+
+int FmmEngine::phase_times(float* build_ms, float* eval_ms) const {
+    if (!impl_->ev[2]) return GC_ERR_STATE;
+    if (hipEventSynchronize(impl_->ev[2]) != hipSuccess) return GC_ERR_HIP;
+    if (hipEventElapsedTime(build_ms, impl_->ev[0], impl_->ev[1]) != hipSuccess) return GC_ERR_HIP;
+    if (hipEventElapsedTime(eval_ms, impl_->ev[1], impl_->ev[2]) != hipSuccess) return GC_ERR_HIP;
+    return GC_OK;
 }
 
 }  // namespace gc

@@ -21,6 +21,10 @@ public:
                 bool close_kick, double half_dt, hipStream_t st, std::string* err);
     int time_kernels(const DeviceState& s, const gc_params& p, int reps, hipStream_t st,
                      double* ms, double* interactions, std::string* err);
+    // Tree and list statistics of the last compute().
+    const gc_fmm_stats& stats() const;
+    // Build (tree + lists) and evaluation times of the last compute() (ms).
+    int phase_times(float* build_ms, float* eval_ms) const;
 
 private:
     struct Impl;

@@ -59,6 +59,13 @@ void launch_scatter_xyz(const double* interleaved, double* x, double* y, double*
                         hipStream_t st);
 void launch_gather_xyz(const double* x, const double* y, const double* z, double* interleaved,
                        size_t n, hipStream_t st);
+// adjoint.hip: lam / xbar are SoA with strides lstride / ostride, mbar n.
+void launch_vjp_fp64(const DeviceState& s, double G, double eps, const double* lam, int lstride,
+                     double* xbar, double* mbar, int ostride, bool accumulate, hipStream_t st);
+void launch_vjp_mixed(const DeviceState& s, MixedLayout& L, float4* lpack, double* part,
+                      int jsplit, double G, double eps, const double* lam, int lstride,
+                      double* xbar, double* mbar, int ostride, bool accumulate, hipStream_t st);
+int vjp_mixed_splits(size_t n);
 // diagnostics.hip: out[13] as gc_diagnostics order (M, KE, P3, C3, L3, PE)
 void launch_diagnostics(const DeviceState& s, double G, double eps, double* scratch,
                         size_t scratch_doubles, double* out_dev, hipStream_t st);

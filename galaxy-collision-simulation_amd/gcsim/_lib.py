@@ -46,6 +46,14 @@ class DeviceView(C.Structure):
         + [("stream", C.c_void_p)]
 
 
+class FmmStats(C.Structure):
+    _fields_ = [("node_count", C.c_size_t), ("leaf_count", C.c_size_t),
+                ("far_interactions", C.c_size_t), ("near_interactions", C.c_size_t),
+                ("m2p_evaluations", C.c_size_t), ("p2p_pairs", C.c_size_t),
+                ("tree_nodes", C.c_size_t), ("max_depth", C.c_int), ("build_ms", C.c_float),
+                ("eval_ms", C.c_float)]
+
+
 class DiskGalaxy(C.Structure):
     _fields_ = [("n_particles", C.c_size_t), ("central_mass", C.c_double),
                 ("disk_mass", C.c_double), ("disk_particle_mass", C.c_double),
@@ -83,6 +91,7 @@ SIGNATURES = {
     "gc_synchronize": (C.c_int, [C.c_void_p]),
     "gc_diagnostics_compute": (C.c_int, [C.c_void_p, C.POINTER(Diagnostics)]),
     "gc_time_force_kernel": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "gc_fmm_last_stats": (C.c_int, [C.c_void_p, C.POINTER(FmmStats)]),
     "gc_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gc_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),

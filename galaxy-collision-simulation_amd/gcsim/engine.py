@@ -186,6 +186,13 @@ class Engine:
             "angular_momentum_z": d.angular_momentum[2],
         }
 
+    def fmm_stats(self) -> dict:
+        """FmmStats of the last hip-fmm evaluation (fmm_solver.hpp:20-25) plus
+        M2P/P2P work counts and build/evaluation times in ms."""
+        s = L.FmmStats()
+        self._check(self._lib.gc_fmm_last_stats(self._ctx, C.byref(s)), "fmm stats")
+        return {k: getattr(s, k) for k, _ in L.FmmStats._fields_}
+
     def time_force_kernel(self, reps=20):
         """(ms per launch, interactions per launch) of the dominant force kernel."""
         ms, inter = C.c_double(), C.c_double()

@@ -144,6 +144,23 @@ gc_status gc_compute_accelerations_for_targets(gc_ctx* ctx, size_t begin, size_t
 gc_status gc_time_force_kernel(gc_ctx* ctx, int reps, double* ms_per_launch,
                                double* interactions_per_launch);
 
+/* FmmStats (cpp/fmm/fmm_solver.hpp:20-25) of the last hip-fmm force
+ * evaluation, plus the particle-level work it implied. */
+typedef struct {
+    size_t node_count;        /* the reference's count: 1 + 8 * internal nodes */
+    size_t leaf_count;        /* leaves with mass > 0 (target leaves) */
+    size_t far_interactions;  /* sum over target leaves of far-list length */
+    size_t near_interactions; /* sum over target leaves of near-list length */
+    size_t m2p_evaluations;   /* particle x far-cell evaluations */
+    size_t p2p_pairs;         /* directed particle pairs in near leaves, self excluded */
+    size_t tree_nodes;        /* non-empty nodes materialised on the device */
+    int max_depth;            /* deepest level reached */
+    float build_ms;           /* tree, moments and lists */
+    float eval_ms;            /* M2P + P2P evaluation */
+} gc_fmm_stats;
+
+gc_status gc_fmm_last_stats(gc_ctx* ctx, gc_fmm_stats* out);
+
 /* Tuning: mixed-kernel launch shape (targets per thread 1/2/4, j-splits).
  * 0 = automatic. */
 gc_status gc_set_launch_shape(gc_ctx* ctx, int targets_per_thread, int jsplit);

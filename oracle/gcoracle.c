@@ -656,10 +656,17 @@ int orc_fmm(size_t n, const double* pos, const double* mass, double G,
     tree_mass(&t, root);
     tree_moments(&t, root);
 
-    lvec leaves = {0}, far = {0}, near = {0};
+    lvec leaves = {0};
     collect_leaves(&t, root, &leaves);
     size_t nfar = 0, nnear = 0;
-    for (size_t li = 0; li < leaves.n; ++li) {
+    /* leaves are independent: each target's sums keep the reference order,
+     * so the result does not depend on the thread count */
+    const long long nleaves = (long long)leaves.n;
+#pragma omp parallel reduction(+ : nfar, nnear)
+    {
+    lvec far = {0}, near = {0};
+#pragma omp for schedule(dynamic, 4)
+    for (long long li = 0; li < nleaves; ++li) {
         const long L = leaves.v[li];
         far.n = near.n = 0;
         leaf_lists(&t, L, root, theta, &far, &near);
@@ -694,6 +701,9 @@ int orc_fmm(size_t n, const double* pos, const double* mass, double G,
             acc[3 * id + 2] = a2;
         }
     }
+    free(far.v);
+    free(near.v);
+    }
     if (stats) {
         stats->node_count = t.count;
         stats->leaf_count = leaves.n;
@@ -702,8 +712,6 @@ int orc_fmm(size_t n, const double* pos, const double* mass, double G,
         stats->max_depth_reached = t.max_depth_reached;
     }
     free(leaves.v);
-    free(far.v);
-    free(near.v);
     free(t.nodes);
     free(t.idx);
     free(t.scratch);

@@ -1,0 +1,119 @@
+"""hip-fmm on the GPU against the oracle's FMM (the reference's algorithm).
+
+The device tree, moments and lists follow the reference's geometric rules,
+so the FmmStats counts (node, leaf, far and near list totals) must equal the
+oracle's exactly.  The FP64 evaluation then agrees with the oracle to
+rounding (different but exact formulas: derivative tensors vs polynomial
+products), the FP32 ("mixed") evaluation to FP32 rounding.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load_snapshot
+from helpers import rel_err, smoke_cube
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11     # max relative difference, FP64 device FMM vs oracle FMM
+MIXED_TOL = 1e-5     # max relative difference, FP32 device FMM vs oracle FMM
+STAT_KEYS = ("node_count", "leaf_count", "far_interactions", "near_interactions")
+
+
+def _fmm(gpu, precision, pos, mass, eps, theta, leaf, order=4, vel=None, begin=None, end=None):
+    with gpu.Engine("hip-fmm", precision, softening=eps, theta=theta, leaf_capacity=leaf,
+                    expansion_order=order) as eng:
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations(begin, end)
+        acc = eng.download()[2]
+        return acc, eng.fmm_stats()
+
+
+def _check(gpu, pos, m, eps, theta, leaf, order=4):
+    want, st = O.fmm(pos, m, 1.0, eps, theta, leaf, 32, order)
+    a64, s64 = _fmm(gpu, "fp64", pos, m, eps, theta, leaf, order)
+    for k in STAT_KEYS:
+        assert s64[k] == st[k], k
+    assert rel_err(a64, want).max() < FP64_TOL
+    a32, s32 = _fmm(gpu, "mixed", pos, m, eps, theta, leaf, order)
+    assert rel_err(a32, want).max() < MIXED_TOL
+    return want, a64, a32, s64
+
+
+def test_readme_golden_step0(gpu):
+    m, x, v, a = load_snapshot("readme1000_snapshot_000000.csv.gz")
+    want, a64, a32, st = _check(gpu, x, m, 0.025, 0.58, 16)
+    assert np.array_equal(want, a)  # the oracle is the golden here
+    assert st["max_depth"] >= 1 and st["eval_ms"] > 0
+
+
+@pytest.mark.parametrize("theta,leaf,order,eps", [(0.5, 64, 4, 0.01), (0.35, 4, 4, 0.03),
+                                                  (0.8, 32, 2, 0.02), (0.6, 8, 0, 0.02),
+                                                  (0.7, 16, 3, 0.0)])
+def test_plummer_against_oracle(gpu, theta, leaf, order, eps):
+    pos, vel, m = gpu.generators.plummer(3000, seed=11)
+    _check(gpu, pos, m, eps, theta, leaf, order)
+
+
+def test_two_disk_and_reference_smoke_threshold(gpu):
+    pos, vel, m = gpu.generators.config0()
+    _check(gpu, pos, m, 0.05, 0.5, 16)
+    # cpp/tests/smoke_tests.cpp:96-123: p=4 FMM (theta 0.35, leaf 4) vs direct
+    cube, cm = smoke_cube()
+    acc, _ = _fmm(gpu, "mixed", cube, cm, 0.03, 0.35, 4)
+    assert rel_err(acc, O.direct_symmetric(cube, cm, 1.0, 0.03)).mean() < 0.25
+
+
+def test_massless_particles_and_target_range(gpu):
+    rng = np.random.default_rng(3)
+    pos, m = rng.normal(size=(2000, 3)), rng.uniform(0.5, 1.0, 2000)
+    m[:300] = 0.0
+    want, _ = O.fmm(pos, m, 1.0, 0.02, 0.5, 8, 32, 4, begin=500, end=1500)
+    got, _ = _fmm(gpu, "fp64", pos, m, 0.02, 0.5, 8, begin=500, end=1500)
+    assert not got[:500].any() and not got[1500:].any()
+    assert rel_err(got[500:1500], want[500:1500]).max() < FP64_TOL
+
+
+def test_fmm_kdk_steps_track_golden(gpu):
+    m, x, v, a = load_snapshot("fmm_1000_snapshot_000000.csv.gz")
+    m20, x20, v20, a20 = load_snapshot("fmm_1000_snapshot_000020.csv.gz")
+    with gpu.Engine("hip-fmm", "fp64", softening=0.025, theta=0.58, leaf_capacity=16) as eng:
+        eng.upload(x, v, m)
+        eng.compute_accelerations()
+        eng.step(0.01, 20)
+        p, vv, acc = eng.download()
+    assert np.abs(p - x20).max() < 1e-9
+    assert rel_err(acc, a20).max() < 1e-6
+
+
+def test_depth_beyond_key_is_reported(gpu):
+    pos = np.zeros((100, 3))
+    pos[50:] = 1.0  # 50 coincident particles cannot be separated by any split
+    m = np.ones(100)
+    with pytest.raises(gpu.GcError) as e:
+        _fmm(gpu, "mixed", pos, m, 0.02, 0.5, 8)
+    assert "21 levels" in str(e.value)
+    # with a depth cap the reference keeps them in one leaf: supported
+    with gpu.Engine("hip-fmm", "fp64", softening=0.02, theta=0.5, leaf_capacity=8,
+                    max_depth=6) as eng:
+        eng.upload(pos, None, m)
+        eng.compute_accelerations()
+        got = eng.download()[2]
+    want, _ = O.fmm(pos, m, 1.0, 0.02, 0.5, 8, 6, 4)
+    assert rel_err(got, want).max() < FP64_TOL
+
+
+def test_plummer_full_size_sampled(gpu):
+    # N = 2^20 Plummer sphere, leaves <= 64, p = 4, theta 0.5, eps 0.01: the
+    # FP32 device FMM against the oracle FMM and FP64 direct sums on 4096
+    # targets (indices are i.i.d. draws, so a contiguous range is a random
+    # sample)
+    pos, vel, m = gpu.generators.plummer(1 << 20, seed=1)
+    got, st = _fmm(gpu, "mixed", pos, m, 0.01, 0.5, 64)
+    want, ost = O.fmm(pos, m, 1.0, 0.01, 0.5, 64, 32, 4, begin=0, end=4096)
+    for k in STAT_KEYS:
+        assert st[k] == ost[k], k
+    assert rel_err(got[:4096], want[:4096]).max() < MIXED_TOL
+    direct = O.direct_targets(pos, m, 1.0, 0.01, 0, 4096, threads=0)
+    err = rel_err(got[:4096], direct[:4096])
+    assert np.median(err) < 1e-4 and err.max() < 1e-2
