@@ -27,6 +27,9 @@ struct gc_ctx {
     double* diag_out = nullptr;  // 13 doubles
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     gc::FmmEngine* fmm = nullptr;
+    float4* vjp_lpack = nullptr;  // FP32 cotangents, cap entries
+    double* vjp_part = nullptr;   // per-split VJP partials
+    size_t vjp_lpack_n = 0, vjp_part_n = 0;
     std::string err;
 };
 
@@ -259,6 +262,8 @@ void gc_destroy(gc_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     delete c->fmm;
     free_state(c);
+    (void)hipFree(c->vjp_lpack);
+    (void)hipFree(c->vjp_part);
     (void)hipFree(c->diag_out);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -464,6 +469,99 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
         launch_merge_mixed(c->s, c->mixed, js, 0, n, false, 0.0, c->stream);
     }
     return launch_status(c, "timed force kernels");
+}
+
+gc_status gc_set_state_device(gc_ctx* c, size_t n, const double* pos, const double* vel,
+                              const double* mass, const double* acc) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (n > 0 && (!pos || !mass)) return fail(c, GC_ERR_INVALID_ARGUMENT, "set_state_device: null positions or masses");
+    if (n > static_cast<size_t>(1) << 30) return fail(c, GC_ERR_INVALID_ARGUMENT, "set_state_device: too many particles");
+    st = ensure_capacity(c, n);
+    if (st != GC_OK) return st;
+    if (n == 0) return GC_OK;
+    const size_t b = sizeof(double) * n;
+    double* dst[3][3] = {{c->s.x, c->s.y, c->s.z}, {c->s.vx, c->s.vy, c->s.vz}, {c->s.ax, c->s.ay, c->s.az}};
+    const double* src[3] = {pos, vel, acc};
+    for (int k = 0; k < 3; ++k) {
+        for (int d = 0; d < 3; ++d) {
+            if (src[k]) {
+                GC_HIP(c, hipMemcpyAsync(dst[k][d], src[k] + d * n, b, hipMemcpyDeviceToDevice, c->stream),
+                       "copy device state in");
+            } else {
+                GC_HIP(c, hipMemsetAsync(dst[k][d], 0, b, c->stream), "clear device state");
+            }
+        }
+    }
+    GC_HIP(c, hipMemcpyAsync(c->s.m, mass, b, hipMemcpyDeviceToDevice, c->stream), "copy masses in");
+    return GC_OK;
+}
+
+gc_status gc_get_state_device(gc_ctx* c, double* pos, double* vel, double* acc) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    const size_t n = c->s.n;
+    if (n == 0) return GC_OK;
+    const size_t b = sizeof(double) * n;
+    const double* src[3][3] = {{c->s.x, c->s.y, c->s.z}, {c->s.vx, c->s.vy, c->s.vz}, {c->s.ax, c->s.ay, c->s.az}};
+    double* dst[3] = {pos, vel, acc};
+    for (int k = 0; k < 3; ++k) {
+        if (!dst[k]) continue;
+        for (int d = 0; d < 3; ++d)
+            GC_HIP(c, hipMemcpyAsync(dst[k] + d * n, src[k][d], b, hipMemcpyDeviceToDevice, c->stream),
+                   "copy device state out");
+    }
+    return GC_OK;
+}
+
+gc_status gc_step_reverse(gc_ctx* c, double dt, int nsteps) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (!(dt > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "dt must be positive");
+    if (nsteps < 0) return fail(c, GC_ERR_INVALID_ARGUMENT, "steps must be non-negative");
+    if (c->s.n == 0) return GC_OK;
+    const double half = -0.5 * dt;
+    for (int k = 0; k < nsteps; ++k) {
+        launch_open_step(c->s, half, -dt, c->stream);
+        st = force(c, 0, c->s.n, true, half);
+        if (st != GC_OK) return st;
+    }
+    return launch_status(c, "launch reverse step kernels");
+}
+
+gc_status gc_force_vjp(gc_ctx* c, const double* lambda, double* xbar, double* mbar,
+                       int accumulate) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (!lambda || !xbar || !mbar) return fail(c, GC_ERR_INVALID_ARGUMENT, "force_vjp: null array");
+    if (c->params.solver != GC_SOLVER_DIRECT)
+        return fail(c, GC_ERR_UNSUPPORTED, "force_vjp: only the direct solver has an adjoint");
+    const size_t n = c->s.n;
+    if (n == 0) return GC_OK;
+    const int stride = static_cast<int>(n);
+    if (c->params.precision == GC_PRECISION_FP64) {
+        launch_vjp_fp64(c->s, c->params.gravitational_constant, c->params.softening, lambda, stride,
+                        xbar, mbar, stride, accumulate != 0, c->stream);
+        return launch_status(c, "launch FP64 force VJP");
+    }
+    const int js = vjp_mixed_splits(n);
+    const size_t need_part = static_cast<size_t>(js) * 4 * c->s.cap;
+    if (c->vjp_lpack_n < c->s.cap || c->vjp_part_n < need_part) {
+        GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize before VJP scratch");
+        (void)hipFree(c->vjp_lpack);
+        (void)hipFree(c->vjp_part);
+        c->vjp_lpack = nullptr;
+        c->vjp_part = nullptr;
+        c->vjp_lpack_n = c->vjp_part_n = 0;
+        GC_HIP(c, hipMalloc(&c->vjp_lpack, sizeof(float4) * c->s.cap), "allocate VJP scratch");
+        GC_HIP(c, hipMalloc(&c->vjp_part, sizeof(double) * need_part), "allocate VJP scratch");
+        c->vjp_lpack_n = c->s.cap;
+        c->vjp_part_n = need_part;
+    }
+    launch_vjp_mixed(c->s, c->mixed, c->vjp_lpack, c->vjp_part, js, c->params.gravitational_constant,
+                     c->params.softening, lambda, stride, xbar, mbar, stride, accumulate != 0,
+                     c->stream);
+    return launch_status(c, "launch mixed force VJP");
 }
 
 gc_status gc_fmm_last_stats(gc_ctx* c, gc_fmm_stats* out) {

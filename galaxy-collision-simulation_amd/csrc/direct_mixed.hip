@@ -25,6 +25,7 @@
 // reference's s2 == 0 skip (cpp/direct/direct_solver.cpp:17-19); without it
 // r^3 would overflow and 0 * inf would poison the sum.
 #include "gc_internal.hpp"
+#include "mixed_tiles.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -32,15 +33,6 @@
 namespace gc {
 namespace {
 
-constexpr float kTinySeparation2 = 1e-30f;
-
-__device__ __forceinline__ void two_sum_acc(float& hi, float& lo, float v) {
-    const float s = hi + v;
-    const float bb = s - hi;
-    const float err = (hi - (s - bb)) + (v - bb);
-    hi = s;
-    lo += err;
-}
 
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
@@ -137,49 +129,7 @@ __global__ __launch_bounds__(kTile) void direct_mixed_kernel(
         hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
     }
 
-    float4 next = t0 < t1 ? jp[static_cast<size_t>(t0) * kTile + t] : make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int tl = t0; tl < t1; ++tl) {
-        __syncthreads();
-        tile[t] = next;
-        __syncthreads();
-        if (tl + 1 < t1) next = jp[static_cast<size_t>(tl + 1) * kTile + t];
-        const double cx = centre[tl], cy = centre[ntiles + tl], cz = centre[2 * ntiles + tl];
-        float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            tx[k] = static_cast<float>(xi[k] - cx);
-            ty[k] = static_cast<float>(yi[k] - cy);
-            tz[k] = static_cast<float>(zi[k] - cz);
-            px[k] = py[k] = pz[k] = 0.f;
-        }
-#pragma unroll 8
-        for (int jj = 0; jj < kTile; ++jj) {
-            const float4 s = tile[jj];
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) {
-                const float dx = s.x - tx[k];
-                const float dy = s.y - ty[k];
-                const float dz = s.z - tz[k];
-                const float s2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-                float r;
-                if (SAFE) {
-                    r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
-                } else {
-                    r = __builtin_amdgcn_rsqf(s2);
-                }
-                const float kk = (s.w * r) * (r * r);
-                px[k] = __builtin_fmaf(dx, kk, px[k]);
-                py[k] = __builtin_fmaf(dy, kk, py[k]);
-                pz[k] = __builtin_fmaf(dz, kk, pz[k]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            two_sum_acc(hx[k], lx[k], px[k]);
-            two_sum_acc(hy[k], ly[k], py[k]);
-            two_sum_acc(hz[k], lz[k], pz[k]);
-        }
-    }
+    tile_sweep<IPT, SAFE>(tile, t0, t1, jp, centre, ntiles, eps2, xi, yi, zi, hx, hy, hz, lx, ly, lz);
     float* base = part + static_cast<size_t>(split) * 6 * cap;
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {

@@ -91,7 +91,27 @@ SIGNATURES = {
     "gc_synchronize": (C.c_int, [C.c_void_p]),
     "gc_diagnostics_compute": (C.c_int, [C.c_void_p, C.POINTER(Diagnostics)]),
     "gc_time_force_kernel": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "gc_set_state_device": (C.c_int, [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "gc_get_state_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gc_step_reverse": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "gc_force_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "gc_fmm_last_stats": (C.c_int, [C.c_void_p, C.POINTER(FmmStats)]),
+    "gc_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params), C.c_int,
+                                  C.POINTER(C.c_size_t)]),
+    "gc_batch_destroy": (None, [C.c_void_p]),
+    "gc_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "gc_batch_particle_count": (C.c_size_t, [C.c_void_p]),
+    "gc_batch_upload": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "gc_batch_download": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "gc_batch_compute_accelerations": (C.c_int, [C.c_void_p]),
+    "gc_batch_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "gc_batch_synchronize": (C.c_int, [C.c_void_p]),
+    "gc_batch_set_schedule": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "gc_batch_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
+    "gc_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gc_batch_time_force_kernel": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
     "gc_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gc_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),

@@ -186,6 +186,27 @@ class Engine:
             "angular_momentum_z": d.angular_momentum[2],
         }
 
+    # -- device-resident interop (addresses of caller-owned device arrays,
+    #    FP64, SoA [3][n]; used by the autograd layer)
+    def set_state_device(self, n, pos_ptr, vel_ptr, mass_ptr, acc_ptr=None):
+        self._check(self._lib.gc_set_state_device(self._ctx, n, C.c_void_p(pos_ptr),
+                                                  C.c_void_p(vel_ptr or 0),
+                                                  C.c_void_p(mass_ptr),
+                                                  C.c_void_p(acc_ptr or 0)), "set state device")
+
+    def get_state_device(self, pos_ptr=None, vel_ptr=None, acc_ptr=None):
+        self._check(self._lib.gc_get_state_device(self._ctx, C.c_void_p(pos_ptr or 0),
+                                                  C.c_void_p(vel_ptr or 0),
+                                                  C.c_void_p(acc_ptr or 0)), "get state device")
+
+    def step_reverse(self, dt, steps=1):
+        self._check(self._lib.gc_step_reverse(self._ctx, float(dt), int(steps)), "reverse step")
+
+    def force_vjp(self, lambda_ptr, xbar_ptr, mbar_ptr, accumulate=False):
+        self._check(self._lib.gc_force_vjp(self._ctx, C.c_void_p(lambda_ptr), C.c_void_p(xbar_ptr),
+                                           C.c_void_p(mbar_ptr), 1 if accumulate else 0),
+                    "force vjp")
+
     def fmm_stats(self) -> dict:
         """FmmStats of the last hip-fmm evaluation (fmm_solver.hpp:20-25) plus
         M2P/P2P work counts and build/evaluation times in ms."""

@@ -144,6 +144,56 @@ gc_status gc_compute_accelerations_for_targets(gc_ctx* ctx, size_t begin, size_t
 gc_status gc_time_force_kernel(gc_ctx* ctx, int reps, double* ms_per_launch,
                                double* interactions_per_launch);
 
+/* ---- differentiation (no reference counterpart: the reference has no
+ * adjoint; these serve gradient-based callers such as torch.autograd) ----
+ * Device arrays below are caller-owned, FP64, SoA "[3][n]" (all x, then all
+ * y, then all z) and are read/written on the context's stream. */
+
+/* Replaces the resident state from device memory; vel/acc may be NULL
+ * (zeroed). */
+gc_status gc_set_state_device(gc_ctx* ctx, size_t n, const double* pos, const double* vel,
+                              const double* mass, const double* acc);
+/* Copies the resident state to device memory (any pointer may be NULL). */
+gc_status gc_get_state_device(gc_ctx* ctx, double* pos, double* vel, double* acc);
+/* nsteps time-reversed KDK steps (the leapfrog with -dt), the recomputation
+ * the adjoint uses to walk a trajectory backwards between checkpoints. */
+gc_status gc_step_reverse(gc_ctx* ctx, double dt, int nsteps);
+/* Vector-Jacobian product of the direct force at the resident positions and
+ * masses: given lambda = dL/da ([3][n]), writes (accumulate == 0) or adds
+ * xbar = (da/dx)^T lambda ([3][n]) and mbar = (da/dm)^T lambda ([n]).
+ * FP64 or mixed per the context's precision; solver must be direct. */
+gc_status gc_force_vjp(gc_ctx* ctx, const double* lambda, double* xbar, double* mbar,
+                       int accumulate);
+
+/* ---- batched runs (no reference counterpart: the reference runs one
+ * simulation per process) ----
+ * Many independent collisions of uneven N advanced together on one device
+ * with the mixed-precision direct solver: one persistent force launch per
+ * step over a static work list, multi-step chunks replayed from a captured
+ * hipGraph.  Host arrays are the runs concatenated in run order (n_total x 3
+ * and n_total). */
+typedef struct gc_batch gc_batch;
+
+gc_status gc_batch_create(gc_batch** out, int device, const gc_params* p, int runs,
+                          const size_t* counts);
+void gc_batch_destroy(gc_batch* b);
+const char* gc_batch_last_error(const gc_batch* b);
+size_t gc_batch_particle_count(const gc_batch* b);
+gc_status gc_batch_upload(gc_batch* b, const double* pos, const double* vel, const double* mass);
+gc_status gc_batch_download(gc_batch* b, double* pos, double* vel, double* acc);
+gc_status gc_batch_compute_accelerations(gc_batch* b);
+gc_status gc_batch_step(gc_batch* b, double dt, int nsteps);
+gc_status gc_batch_synchronize(gc_batch* b);
+/* chunk_tiles: source tiles (of 256 particles) per work item; blocks:
+ * persistent workgroups; graph_steps: steps per captured graph (0/1: none). */
+gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps);
+gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps);
+/* Times (chunk_tiles, blocks) candidates on this batch's force kernel and
+ * keeps the fastest; results are cached per process by batch shape. */
+gc_status gc_batch_autotune(gc_batch* b, int* chunk_tiles, int* blocks);
+gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms_per_launch,
+                                     double* interactions_per_launch);
+
 /* FmmStats (cpp/fmm/fmm_solver.hpp:20-25) of the last hip-fmm force
  * evaluation, plus the particle-level work it implied. */
 typedef struct {
