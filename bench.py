@@ -47,6 +47,206 @@ def _traffic(kernel: str):
     return None
 
 
+def _rel_err(a, ref):
+    return np.linalg.norm(a - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-12)
+
+
+def _oracle():
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle  # accuracy checker and CPU baseline only
+
+    return oracle
+
+
+def workload_fmm(args):
+    """Plummer sphere N = 2^20, leaves <= 64, p = 4, theta 0.5, eps 0.01, 10 KDK
+    steps at dt 1e-3 with FP32 evaluation and FP64 state; accuracy on 4096
+    sampled targets against FP64 direct sums."""
+    import gcsim
+
+    n, eps, dt, steps = 1 << 20, 0.01, 1e-3, 10
+    pos, vel, mass = gcsim.generators.plummer(n, seed=1)
+    eng = gcsim.Engine("hip-fmm", "mixed", softening=eps, theta=0.5, leaf_capacity=64,
+                       expansion_order=4)
+    eng.upload(pos, vel, mass)
+    eng.compute_accelerations()
+    _, _, a0 = eng.download()
+    st0 = eng.fmm_stats()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.step(dt, steps)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    st = eng.fmm_stats()
+    k_ms, _ = eng.time_force_kernel(reps=3)
+    eng.close()
+    O = _oracle()
+    threads = O.direct_targets_omp_threads(0)
+    sample = 4096
+    direct = O.direct_targets(pos, mass, 1.0, eps, 0, sample, threads=threads)
+    err = _rel_err(a0[:sample], direct[:sample])
+    tc = time.perf_counter()
+    acc_cpu, _ = O.fmm(pos, mass, 1.0, eps, 0.5, 64, 32, 4, begin=0, end=sample)
+    cpu_s = time.perf_counter() - tc
+    err_vs_cpu_fmm = _rel_err(a0[:sample], acc_cpu[:sample])
+    flops = 20.0 * st["p2p_pairs"] + 440.0 * st["m2p_evaluations"]
+    return {
+        "workload": "fmm: Plummer N=2^20, leaf<=64, p=4, theta=0.5, eps=0.01, dt=1e-3, 10 KDK",
+        "ms_per_step": wall * 1e3 / steps,
+        "eval_kernel_ms": k_ms,
+        "build_ms": st["build_ms"], "eval_ms": st["eval_ms"],
+        "p2p_pairs": st["p2p_pairs"], "m2p_evaluations": st["m2p_evaluations"],
+        "Gp2p_per_s_eval": st["p2p_pairs"] / (k_ms * 1e-3) / 1e9,
+        "Gm2p_per_s_eval": st["m2p_evaluations"] / (k_ms * 1e-3) / 1e9,
+        "tflops_eval_20_440": flops / (k_ms * 1e-3) / 1e12,
+        "stats": {k: st0[k] for k in ("node_count", "leaf_count", "far_interactions",
+                                      "near_interactions", "max_depth")},
+        "accuracy_vs_fp64_direct_4096": {"max": float(err.max()), "median": float(np.median(err)),
+                                         "p95": float(np.percentile(err, 95))},
+        "fp32_vs_oracle_fmm_4096_max": float(err_vs_cpu_fmm.max()),
+        "cpu_baseline": {"kind": "port", "cores": threads,
+                         "sample": f"oracle FMM, full tree + lists, {sample} targets",
+                         "seconds": cpu_s,
+                         "extrapolated_s_per_evaluation": cpu_s * n / sample},
+    }
+
+
+def workload_batch(args):
+    """256 collisions, N log-uniform in [512, 8192], eps 0.05, dt 1e-3: one
+    persistent launch per step, hipGraph 10-step chunks, autotuned schedule;
+    against the same runs stepped one at a time."""
+    import gcsim
+    from gcsim.batch import Batch
+
+    runs = gcsim.generators.collisions(256, seed=2026)
+    parts = [gcsim.generators.collision_particles(r, softening=0.05) for r in runs]
+    counts = [len(p[2]) for p in parts]
+    pairs = float(sum(c * (c - 1) for c in counts))
+    data = tuple(np.concatenate([p[i] for p in parts]) for i in range(3))
+    steps = max(10, args.steps // 10 * 10)
+    b = Batch(counts, softening=0.05)
+    b.upload(*data)
+    chunk, blocks = b.autotune()
+    b.compute_accelerations()
+    b.step(1e-3, 10)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(1e-3, steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    k_ms, _ = b.time_force_kernel(5)
+    b.close()
+    # one at a time: every run through a resident single-run engine
+    single_steps = 5
+    eng = gcsim.Engine("hip-direct", "mixed", softening=0.05)
+    t_single = 0.0
+    for p in parts:
+        eng.upload(*p)
+        eng.compute_accelerations()
+        eng.synchronize()
+        t1 = time.perf_counter()
+        eng.step(1e-3, single_steps)
+        eng.synchronize()
+        t_single += time.perf_counter() - t1
+    eng.close()
+    batched = pairs * steps / wall / 1e9
+    single = pairs * single_steps / t_single / 1e9
+    return {
+        "workload": "batch: 256 two-disk collisions, N in [512, 8192] log-uniform, eps=0.05, dt=1e-3",
+        "runs": len(counts), "particles": sum(counts), "pairs_per_step": pairs,
+        "schedule": {"chunk_tiles": chunk, "blocks": blocks, "graph_steps": 10},
+        "batched_Gpair_per_s": batched, "ms_per_step": wall * 1e3 / steps,
+        "force_kernel_ms": k_ms,
+        "force_kernel_frac_fp32_peak": FLOP_PER_PAIR * pairs / (k_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+        "one_at_a_time_Gpair_per_s": single, "speedup": batched / single,
+    }
+
+
+def workload_adjoint(args):
+    """Two-disk N = 16,384, 200 KDK steps at dt 1e-3, checkpoints every 20:
+    forward + reversed-leapfrog adjoint, FP32 forces/VJP with FP64 state."""
+    import torch
+
+    import gcsim
+    from gcsim import adjoint
+
+    n, steps, dt = 16384, 200, 1e-3
+    pos, vel, mass = gcsim.generators.two_disk(n // 2, softening=0.05, seed=42)
+    dev = torch.device("cuda:0")
+    x0, v0, m = (torch.tensor(a, dtype=torch.float64, device=dev) for a in (pos, vel, mass))
+    g = torch.Generator(device=dev).manual_seed(8)
+    out = {"workload": "adjoint: two-disk N=16384, 200 KDK steps, dt=1e-3, checkpoints 20"}
+    for prec in ("mixed", "fp64"):
+        eng = gcsim.Engine("hip-direct", prec, softening=0.05)
+        with torch.no_grad():
+            vt = v0 + 0.005 * torch.randn(v0.shape, dtype=torch.float64, device=dev, generator=g)
+            target, _ = adjoint.trajectory(x0, vt, m, dt, steps, engine=eng)
+        va, ma = v0.clone().requires_grad_(), m.clone().requires_grad_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xT, _ = adjoint.trajectory(x0, va, ma, dt, steps, engine=eng, checkpoint_every=20)
+        loss = ((xT - target) ** 2).sum(1).mean()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "loss": loss.item(),
+                     "grad_v_norm": va.grad.norm().item(), "grad_m_norm": ma.grad.norm().item()}
+        out[prec + "_grad"] = (va.grad.detach().clone(), ma.grad.detach().clone())
+        eng.close()
+    gv64, gm64 = out.pop("fp64_grad")
+    gv32, gm32 = out.pop("mixed_grad")
+    out["mixed_vs_fp64_grad_rel"] = {
+        "v": ((gv32 - gv64).norm() / gv64.norm()).item(),
+        "m": ((gm32 - gm64).norm() / gm64.norm()).item()}
+    return out
+
+
+def workload_drift(args):
+    """Relative energy drift over 1000 KDK steps of the N = 65,536 two-disk
+    collision, FP64 and mixed, and the per-step force error of mixed vs FP64."""
+    import gcsim
+
+    n, eps, dt, steps, every = 65536, 0.02, 5e-4, 1000, 50
+    pos, vel, mass = gcsim.generators.config1(n=n)
+    out = {"workload": "drift: two-disk N=65536, eps=0.02, dt=5e-4, 1000 KDK steps"}
+    engines = {p: gcsim.Engine("hip-direct", p, softening=eps) for p in ("fp64", "mixed")}
+    for e in engines.values():
+        e.upload(pos, vel, mass)
+        e.compute_accelerations()
+    series = {p: [] for p in engines}
+    errs = []
+    t0 = time.perf_counter()
+    for k in range(0, steps + 1, every):
+        if k:
+            for e in engines.values():
+                e.step(dt, every)
+        for p, e in engines.items():
+            series[p].append(e.diagnostics()["total_energy"])
+        # force error of mixed against FP64 on the mixed trajectory's state
+        pm, vm, am = engines["mixed"].download()
+        probe = gcsim.Engine("hip-direct", "fp64", softening=eps)
+        probe.upload(pm, vm, mass)
+        probe.compute_accelerations()
+        a64 = probe.download()[2]
+        probe.close()
+        err = _rel_err(am, a64)
+        errs.append((float(err.max()), float(np.median(err))))
+        print(f"drift step {k}", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    for p, s in series.items():
+        s = np.array(s)
+        rel = np.abs(s - s[0]) / abs(s[0])
+        out[p] = {"max_rel_energy_drift": float(rel.max()), "final_rel_energy_drift": float(rel[-1])}
+    out["mixed_over_fp64_drift"] = (out["mixed"]["max_rel_energy_drift"]
+                                    / max(out["fp64"]["max_rel_energy_drift"], 1e-300))
+    out["mixed_force_err_vs_fp64"] = {"max_over_steps": max(e[0] for e in errs),
+                                      "median_of_medians": float(np.median([e[1] for e in errs]))}
+    out["wall_s"] = wall
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +256,9 @@ def main() -> int:
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
+    ap.add_argument("--workload", default="direct",
+                    choices=["direct", "fmm", "batch", "adjoint", "drift"],
+                    help="direct = the headline line; the others print their own records")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -69,6 +272,11 @@ def main() -> int:
     if not gcsim.solver_available():
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         return 1
+    if args.workload != "direct":
+        fn = {"fmm": workload_fmm, "batch": workload_batch, "adjoint": workload_adjoint,
+              "drift": workload_drift}[args.workload]
+        print(json.dumps(fn(args)))
+        return 0
 
     n, eps, dt = args.n, 0.02, 5e-4
     pos, vel, mass = generators.config1(n=n)

@@ -227,13 +227,17 @@ gc_status build_schedule(gc_batch* b, int chunk) {
     std::vector<int4> items;
     int slot = 0;
     for (int r = 0; r < b->runs; ++r) {
+        // the run's source split of a single-run launch with
+        // jsplit = ceil(nt / chunk): equal ranges of tps tiles
         const int t0 = b->run_tile0[r], nt = b->run_ntiles[r];
-        const int nc = (nt + chunk - 1) / chunk;
+        const int js = (nt + chunk - 1) / chunk;
+        const int tps = (nt + js - 1) / js;
+        const int nc = (nt + tps - 1) / tps;
         for (int ti = t0; ti < t0 + nt; ++ti) {
             slot0[ti] = slot;
             nchunk[ti] = nc;
             for (int c = 0; c < nc; ++c) {
-                const int j0 = t0 + c * chunk, j1 = std::min(j0 + chunk, t0 + nt);
+                const int j0 = t0 + c * tps, j1 = std::min(j0 + tps, t0 + nt);
                 items.push_back(make_int4(ti, j0, j1, slot++));
             }
         }
