@@ -247,6 +247,28 @@ def workload_drift(args):
     return out
 
 
+def workload_copy(args):
+    """Calibration for the HBM counters: a plain device-to-device copy kernel
+    (torch elementwise copy) of 256 MiB, so FETCH_SIZE / WRITE_SIZE per
+    dispatch can be scaled to known bytes."""
+    import torch
+
+    nbytes = 256 << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"workload": "copy calibration", "bytes_read_per_copy": nbytes,
+            "bytes_written_per_copy": nbytes, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,7 +279,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
     ap.add_argument("--workload", default="direct",
-                    choices=["direct", "fmm", "batch", "adjoint", "drift"],
+                    choices=["direct", "fmm", "batch", "adjoint", "drift", "copy"],
                     help="direct = the headline line; the others print their own records")
     args = ap.parse_args()
 
@@ -274,7 +296,7 @@ def main() -> int:
         return 1
     if args.workload != "direct":
         fn = {"fmm": workload_fmm, "batch": workload_batch, "adjoint": workload_adjoint,
-              "drift": workload_drift}[args.workload]
+              "drift": workload_drift, "copy": workload_copy}[args.workload]
         print(json.dumps(fn(args)))
         return 0
 

@@ -29,6 +29,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -502,6 +503,127 @@ __global__ __launch_bounds__(kEvalLanes) void eval_kernel(
     }
 }
 
+// Grouped evaluation: a leaf chunk of c <= 64 targets is served by
+// G = floor(64 / c) lane groups; group g takes far cells f = g, g + G, ...
+// and staged near sources jj = g, g + G, ...; the per-lane partials are then
+// summed over groups in group order through LDS (deterministic).  Keeps
+// ~c*G/64 of the lanes busy instead of c/64.
+template <typename F, int ORDER, bool SAFE>
+__global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
+    int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
+    const double* __restrict__ mcom, const float* __restrict__ n32, const int* __restrict__ nb,
+    const int* __restrict__ ne, const int* __restrict__ sidx, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    const long long* __restrict__ far_off, const int* __restrict__ nfar,
+    const int* __restrict__ far, const long long* __restrict__ near_off,
+    const int* __restrict__ nnear, const int* __restrict__ near, double G, double eps, int begin,
+    int end, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, int close_kick,
+    double half_dt) {
+#pragma clang fp contract(off)
+    __shared__ Src<F> tile[kEvalLanes];
+    __shared__ double red[3][kEvalLanes];
+    const int leaf = leaves[blockIdx.x];
+    const int lane = threadIdx.x;
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    const F eps2 = static_cast<F>(eps * eps);
+    const int tb = nb[leaf], te = ne[leaf];
+    const long long fo = far_off[blockIdx.x], no = near_off[blockIdx.x];
+    const int nf = nfar[blockIdx.x], nn = nnear[blockIdx.x];
+    for (int base = tb; base < te; base += kEvalLanes) {
+        const int c = min(kEvalLanes, te - base);
+        const int groups = kEvalLanes / c;
+        const int g = lane / c;
+        const int tl = lane - g * c;
+        const bool active = g < groups;
+        const int me = sidx[base + tl];
+        const F tx = static_cast<F>(x[me] - cx);
+        const F ty = static_cast<F>(y[me] - cy);
+        const F tz = static_cast<F>(z[me] - cz);
+        double far_a[3] = {0.0, 0.0, 0.0}, near_a[3] = {0.0, 0.0, 0.0};
+        if (active) {
+            for (int f = g; f < nf; f += groups) {
+                const int cell = far[fo + f];
+                const F yx = static_cast<F>(mcom[4 * cell + 1] - cx) - tx;
+                const F yy = static_cast<F>(mcom[4 * cell + 2] - cy) - ty;
+                const F yz = static_cast<F>(mcom[4 * cell + 3] - cz) - tz;
+                F N[kMom];
+                const float* src = n32 + static_cast<size_t>(kN32) * cell;
+#pragma unroll
+                for (int a = 0; a < kMom; ++a) N[a] = static_cast<F>(src[a]);
+                F fx = 0, fy = 0, fz = 0;
+                if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
+                else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
+                else m2p_order4<F>(yx, yy, yz, eps2, N, fx, fy, fz);
+                far_a[0] += static_cast<double>(fx);
+                far_a[1] += static_cast<double>(fy);
+                far_a[2] += static_cast<double>(fz);
+            }
+        }
+        for (int q = 0; q < nn; ++q) {
+            const int src_leaf = near[no + q];
+            const int sb = nb[src_leaf], se = ne[src_leaf];
+            const bool self_leaf = src_leaf == leaf;
+            for (int cb = sb; cb < se; cb += kEvalLanes) {
+                __syncthreads();
+                const int j = cb + lane;
+                if (j < se) {
+                    const int id = sidx[j];
+                    Src<F> t;
+                    t.x = static_cast<F>(x[id] - cx);
+                    t.y = static_cast<F>(y[id] - cy);
+                    t.z = static_cast<F>(z[id] - cz);
+                    t.gm = static_cast<F>(G * m[id]);
+                    t.id = id;
+                    tile[lane] = t;
+                }
+                __syncthreads();
+                if (!active) continue;
+                const int cnt = min(kEvalLanes, se - cb);
+                F px = 0, py = 0, pz = 0;
+                for (int jj = g; jj < cnt; jj += groups) {
+                    const Src<F> sj = tile[jj];
+                    if (self_leaf && sj.id == me) continue;
+                    const F dx = sj.x - tx, dy = sj.y - ty, dz = sj.z - tz;
+                    const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+                    F r;
+                    if (SAFE) {
+                        r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
+                    } else {
+                        r = gc_rsq(s2);
+                    }
+                    const F k = (sj.gm * r) * (r * r);
+                    px = gc_fma(dx, k, px);
+                    py = gc_fma(dy, k, py);
+                    pz = gc_fma(dz, k, pz);
+                }
+                near_a[0] += static_cast<double>(px);
+                near_a[1] += static_cast<double>(py);
+                near_a[2] += static_cast<double>(pz);
+            }
+        }
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) red[d][lane] = active ? G * far_a[d] + near_a[d] : 0.0;
+        __syncthreads();
+        if (g != 0) continue;
+        double a[3];
+        for (int d = 0; d < 3; ++d) {
+            double s = red[d][tl];
+            for (int k = 1; k < groups; ++k) s += red[d][tl + k * c];
+            a[d] = s;
+        }
+        if (me < begin || me >= end) continue;
+        ax[me] = a[0];
+        ay[me] = a[1];
+        az[me] = a[2];
+        if (close_kick) {
+            vx[me] = vx[me] + a[0] * half_dt;
+            vy[me] = vy[me] + a[1] * half_dt;
+            vz[me] = vz[me] + a[2] * half_dt;
+        }
+    }
+}
+
 // zero accelerations of [begin, end) (particles in massless leaves keep 0,
 // as in fmm_solver.cpp:31 + :210-226)
 __global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, double* __restrict__ ax,
@@ -850,12 +972,19 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const bool mixed = p.precision == GC_PRECISION_MIXED;
     const bool safe = !(static_cast<float>(p.softening * p.softening) >= 1e-30f);
     const dim3 grid(static_cast<unsigned>(nleaves)), block(kEvalLanes);
-#define GC_EVAL(F, O, S)                                                                         \
-    hipLaunchKernelGGL((eval_kernel<F, O, S>), grid, block, 0, st, nleaves, leaves, geom, mcom,  \
-                       n32, nb, ne, idx_sorted, s.x, s.y, s.z, s.m, far_off, nfar, far,          \
-                       near_off, nnear, near, p.gravitational_constant, p.softening,             \
-                       static_cast<int>(begin), static_cast<int>(end), s.ax, s.ay, s.az, s.vx,   \
-                       s.vy, s.vz, close_kick ? 1 : 0, half_dt)
+    // lane-per-target evaluation only when asked for (comparison runs)
+    const char* mode = std::getenv("GCSIM_FMM_EVAL");
+    const bool grouped = !(mode && std::strcmp(mode, "simple") == 0);
+#define GC_EVAL_ARGS                                                                             \
+    grid, block, 0, st, nleaves, leaves, geom, mcom, n32, nb, ne, idx_sorted, s.x, s.y, s.z, s.m, \
+        far_off, nfar, far, near_off, nnear, near, p.gravitational_constant, p.softening,        \
+        static_cast<int>(begin), static_cast<int>(end), s.ax, s.ay, s.az, s.vx, s.vy, s.vz,      \
+        close_kick ? 1 : 0, half_dt
+#define GC_EVAL(F, O, S)                                                   \
+    do {                                                                   \
+        if (grouped) hipLaunchKernelGGL((eval_grouped_kernel<F, O, S>), GC_EVAL_ARGS); \
+        else hipLaunchKernelGGL((eval_kernel<F, O, S>), GC_EVAL_ARGS);     \
+    } while (0)
 #define GC_EVAL_ORDER(F, S)                         \
     do {                                            \
         if (order == 0) GC_EVAL(F, 0, S);           \
@@ -871,6 +1000,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     }
 #undef GC_EVAL_ORDER
 #undef GC_EVAL
+#undef GC_EVAL_ARGS
     FMM_HIP(hipGetLastError(), "launch FMM evaluation");
     return GC_OK;
 }
