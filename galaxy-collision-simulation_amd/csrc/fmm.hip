@@ -395,6 +395,25 @@ __global__ __launch_bounds__(64) void lists_kernel(
 }
 
 // ---- 7. evaluation ----------------------------------------------------------
+// Normalised moments N_alpha = M_alpha / alpha! of a cell: the FP32 copy for
+// the FP32 evaluation, recomputed from the FP64 moments for the FP64 one.
+template <typename F>
+__device__ __forceinline__ void load_moments(int cell, const float* __restrict__ n32,
+                                             const double* __restrict__ mom,
+                                             const double* __restrict__ mcom, F* N) {
+    if constexpr (sizeof(F) == sizeof(double)) {
+        const double* src = mom + static_cast<size_t>(kMom) * cell;
+        N[0] = mcom[4 * cell];
+        N[1] = N[2] = N[3] = 0.0;
+#pragma unroll
+        for (int a = 4; a < kMom; ++a) N[a] = src[a] * c_tables.inv_fact[a];
+    } else {
+        const float* src = n32 + static_cast<size_t>(kN32) * cell;
+#pragma unroll
+        for (int a = 0; a < kMom; ++a) N[a] = src[a];
+    }
+}
+
 template <typename F>
 struct Src {
     F x, y, z, gm;
@@ -404,8 +423,9 @@ struct Src {
 template <typename F, int ORDER, bool SAFE>
 __global__ __launch_bounds__(kEvalLanes) void eval_kernel(
     int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
-    const double* __restrict__ mcom, const float* __restrict__ n32, const int* __restrict__ nb,
-    const int* __restrict__ ne, const int* __restrict__ sidx, const double* __restrict__ x,
+    const double* __restrict__ mcom, const float* __restrict__ n32, const double* __restrict__ mom,
+    const int* __restrict__ nb, const int* __restrict__ ne, const int* __restrict__ sidx,
+    const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
     const long long* __restrict__ far_off, const int* __restrict__ nfar,
     const int* __restrict__ far, const long long* __restrict__ near_off,
@@ -436,9 +456,7 @@ __global__ __launch_bounds__(kEvalLanes) void eval_kernel(
             const F yy = static_cast<F>(mcom[4 * cell + 2] - cy) - ty;
             const F yz = static_cast<F>(mcom[4 * cell + 3] - cz) - tz;
             F N[kMom];
-            const float* src = n32 + static_cast<size_t>(kN32) * cell;
-#pragma unroll
-            for (int a = 0; a < kMom; ++a) N[a] = static_cast<F>(src[a]);
+            load_moments<F>(cell, n32, mom, mcom, N);
             F fx = 0, fy = 0, fz = 0;
             if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
             else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
@@ -511,8 +529,9 @@ __global__ __launch_bounds__(kEvalLanes) void eval_kernel(
 template <typename F, int ORDER, bool SAFE>
 __global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
     int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
-    const double* __restrict__ mcom, const float* __restrict__ n32, const int* __restrict__ nb,
-    const int* __restrict__ ne, const int* __restrict__ sidx, const double* __restrict__ x,
+    const double* __restrict__ mcom, const float* __restrict__ n32, const double* __restrict__ mom,
+    const int* __restrict__ nb, const int* __restrict__ ne, const int* __restrict__ sidx,
+    const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
     const long long* __restrict__ far_off, const int* __restrict__ nfar,
     const int* __restrict__ far, const long long* __restrict__ near_off,
@@ -548,9 +567,7 @@ __global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
                 const F yy = static_cast<F>(mcom[4 * cell + 2] - cy) - ty;
                 const F yz = static_cast<F>(mcom[4 * cell + 3] - cz) - tz;
                 F N[kMom];
-                const float* src = n32 + static_cast<size_t>(kN32) * cell;
-#pragma unroll
-                for (int a = 0; a < kMom; ++a) N[a] = static_cast<F>(src[a]);
+                load_moments<F>(cell, n32, mom, mcom, N);
                 F fx = 0, fy = 0, fz = 0;
                 if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
                 else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
@@ -976,7 +993,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const char* mode = std::getenv("GCSIM_FMM_EVAL");
     const bool grouped = !(mode && std::strcmp(mode, "simple") == 0);
 #define GC_EVAL_ARGS                                                                             \
-    grid, block, 0, st, nleaves, leaves, geom, mcom, n32, nb, ne, idx_sorted, s.x, s.y, s.z, s.m, \
+    grid, block, 0, st, nleaves, leaves, geom, mcom, n32, mom, nb, ne, idx_sorted, s.x, s.y, s.z, s.m, \
         far_off, nfar, far, near_off, nnear, near, p.gravitational_constant, p.softening,        \
         static_cast<int>(begin), static_cast<int>(end), s.ax, s.ay, s.az, s.vx, s.vy, s.vz,      \
         close_kick ? 1 : 0, half_dt
