@@ -281,7 +281,13 @@ gc_status gc_set_params(gc_ctx* c, const gc_params* p) {
 
 gc_status gc_set_stream(gc_ctx* c, void* stream) {
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
-    c->stream = stream ? static_cast<hipStream_t>(stream) : c->own_stream;
+    c->stream = static_cast<hipStream_t>(stream);
+    return GC_OK;
+}
+
+gc_status gc_use_own_stream(gc_ctx* c) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    c->stream = c->own_stream;
     return GC_OK;
 }
 

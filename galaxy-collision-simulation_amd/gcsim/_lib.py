@@ -79,6 +79,7 @@ SIGNATURES = {
     "gc_destroy": (None, [C.c_void_p]),
     "gc_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "gc_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gc_use_own_stream": (C.c_int, [C.c_void_p]),
     "gc_get_stream": (C.c_void_p, [C.c_void_p]),
     "gc_upload": (C.c_int, [C.c_void_p, C.c_size_t, _dp, _dp, _dp]),
     "gc_set_accelerations": (C.c_int, [C.c_void_p, _dp]),

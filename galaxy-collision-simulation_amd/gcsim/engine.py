@@ -113,8 +113,13 @@ class Engine:
         self.params = p
 
     def set_stream(self, stream_handle: int | None):
-        self._check(self._lib.gc_set_stream(self._ctx, C.c_void_p(stream_handle or 0)),
-                    "set stream")
+        """Run on the given HIP stream (0 = the null stream, e.g. torch's
+        default stream); None returns to the engine's own stream."""
+        if stream_handle is None:
+            self._check(self._lib.gc_use_own_stream(self._ctx), "use own stream")
+        else:
+            self._check(self._lib.gc_set_stream(self._ctx, C.c_void_p(stream_handle)),
+                        "set stream")
 
     def set_launch_shape(self, targets_per_thread=0, jsplit=0):
         self._check(self._lib.gc_set_launch_shape(self._ctx, targets_per_thread, jsplit),

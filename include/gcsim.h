@@ -106,7 +106,11 @@ const char* gc_last_error(const gc_ctx* ctx);
 gc_status gc_create(gc_ctx** out, int device, const gc_params* p);
 void gc_destroy(gc_ctx* ctx);
 gc_status gc_set_params(gc_ctx* ctx, const gc_params* p);
-gc_status gc_set_stream(gc_ctx* ctx, void* hip_stream); /* NULL: back to own */
+/* Runs the context's work on hip_stream (NULL = the null stream, e.g.
+ * torch's default stream); gc_use_own_stream returns to the context's own
+ * non-blocking stream. */
+gc_status gc_set_stream(gc_ctx* ctx, void* hip_stream);
+gc_status gc_use_own_stream(gc_ctx* ctx);
 void* gc_get_stream(gc_ctx* ctx);
 
 /* Uploads n particles (pos, vel: n x 3; mass: n).  vel may be NULL (zero).
