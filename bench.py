@@ -248,25 +248,21 @@ def workload_drift(args):
 
 
 def workload_copy(args):
-    """Calibration for the HBM counters: a plain device-to-device copy kernel
-    (torch elementwise copy) of 256 MiB, so FETCH_SIZE / WRITE_SIZE per
-    dispatch can be scaled to known bytes."""
-    import torch
+    """Calibration for the HBM counters: calibration_copy_kernel (plain
+    float4 copy) over 256 MiB, so FETCH_SIZE / WRITE_SIZE per dispatch can
+    be scaled to known bytes."""
+    import ctypes as C
+
+    import gcsim
 
     nbytes = 256 << 20
-    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    reps = 10
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
-    return {"workload": "copy calibration", "bytes_read_per_copy": nbytes,
-            "bytes_written_per_copy": nbytes, "GBps_read_plus_write": 2 * nbytes / dt / 1e9}
+    ms = C.c_double()
+    st = gcsim.load().gc_calibration_copy(0, nbytes, 10, C.byref(ms))
+    if st != 0:
+        raise gcsim.GcError(st, "calibration copy failed")
+    return {"workload": "copy calibration", "kernel": "calibration_copy_kernel",
+            "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes,
+            "ms_per_copy": ms.value, "GBps_read_plus_write": 2 * nbytes / (ms.value * 1e-3) / 1e9}
 
 
 def main() -> int:

@@ -582,6 +582,18 @@ gc_status gc_fmm_last_stats(gc_ctx* c, gc_fmm_stats* out) {
     return GC_OK;
 }
 
+gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms) {
+    if (!ms || bytes == 0 || bytes % 16 != 0 || reps <= 0)
+        return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "calibration_copy: bad arguments");
+    if (device < 0 || device >= gc_device_count())
+        return fail(nullptr, GC_ERR_NO_DEVICE, "calibration_copy: no such device");
+    const hipError_t e = run_calibration_copy(device, bytes, reps, ms);
+    if (e != hipSuccess)
+        return fail(nullptr, e == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
+                    std::string("calibration_copy: ") + hipGetErrorString(e));
+    return GC_OK;
+}
+
 gc_status gc_set_launch_shape(gc_ctx* c, int ipt, int jsplit) {
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (ipt != 0 && ipt != 1 && ipt != 2 && ipt != 4)

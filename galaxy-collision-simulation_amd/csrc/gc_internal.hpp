@@ -59,6 +59,7 @@ void launch_scatter_xyz(const double* interleaved, double* x, double* y, double*
                         hipStream_t st);
 void launch_gather_xyz(const double* x, const double* y, const double* z, double* interleaved,
                        size_t n, hipStream_t st);
+hipError_t run_calibration_copy(int device, size_t bytes, int reps, double* ms);
 // adjoint.hip: lam / xbar are SoA with strides lstride / ostride, mbar n.
 void launch_vjp_fp64(const DeviceState& s, double G, double eps, const double* lam, int lstride,
                      double* xbar, double* mbar, int ostride, bool accumulate, hipStream_t st);

@@ -65,7 +65,44 @@ __global__ __launch_bounds__(256) void gather_xyz_kernel(size_t n, const double*
 
 inline unsigned blocks_for(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
+// Plain streaming copy (16 B per lane), the reference kernel for scaling
+// the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void calibration_copy_kernel(size_t n, const float4* __restrict__ in,
+                                                               float4* __restrict__ out) {
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * 256)
+        out[i] = in[i];
+}
+
 }  // namespace
+
+hipError_t run_calibration_copy(int device, size_t bytes, int reps, double* ms) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const size_t n = bytes / sizeof(float4);
+    if ((e = hipMalloc(&a, bytes)) == hipSuccess && (e = hipMalloc(&b, bytes)) == hipSuccess &&
+        (e = hipMemset(a, 0, bytes)) == hipSuccess && (e = hipEventCreate(&e0)) == hipSuccess &&
+        (e = hipEventCreate(&e1)) == hipSuccess) {
+        const unsigned grid = 4096;
+        hipLaunchKernelGGL(calibration_copy_kernel, dim3(grid), dim3(256), 0, nullptr, n, a, b);
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(calibration_copy_kernel, dim3(grid), dim3(256), 0, nullptr, n, a, b);
+        (void)hipEventRecord(e1, nullptr);
+        e = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e == hipSuccess) *ms = static_cast<double>(t) / std::max(1, reps);
+        if (e == hipSuccess) e = hipGetLastError();
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return e;
+}
 
 void launch_open_step(const DeviceState& s, double half_dt, double dt, hipStream_t st) {
     if (s.n == 0) return;

@@ -113,6 +113,7 @@ SIGNATURES = {
                                         C.POINTER(C.c_int)]),
     "gc_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_batch_time_force_kernel": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp]),
+    "gc_calibration_copy": (C.c_int, [C.c_int, C.c_size_t, C.c_int, _dp]),
     "gc_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gc_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),

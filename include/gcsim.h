@@ -215,6 +215,11 @@ typedef struct {
 
 gc_status gc_fmm_last_stats(gc_ctx* ctx, gc_fmm_stats* out);
 
+/* Calibration for HBM counters: copies `bytes` (multiple of 16) between two
+ * fresh device buffers `reps` times with a plain float4 copy kernel
+ * (calibration_copy_kernel) on `device`; returns ms per copy. */
+gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms_per_copy);
+
 /* Tuning: mixed-kernel launch shape (targets per thread 1/2/4, j-splits).
  * 0 = automatic. */
 gc_status gc_set_launch_shape(gc_ctx* ctx, int targets_per_thread, int jsplit);
