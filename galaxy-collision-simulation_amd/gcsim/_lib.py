@@ -137,6 +137,14 @@ def load():
         raise RuntimeError(f"gcsim: native library missing at {LIB_PATH}; "
                            "build it with __graft_entry__.build() or make -C "
                            "galaxy-collision-simulation_amd")
+    # One HIP runtime per process: libgcsim and torch both resolve
+    # libamdhip64.so.7 and the first one loaded serves both.  torch only
+    # initialises on the runtime it ships, so when torch is installed it is
+    # loaded first; libgcsim runs on either.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
