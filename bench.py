@@ -175,11 +175,11 @@ def workload_adjoint(args):
     dev = torch.device("cuda:0")
     x0, v0, m = (torch.tensor(a, dtype=torch.float64, device=dev) for a in (pos, vel, mass))
     g = torch.Generator(device=dev).manual_seed(8)
+    vt = v0 + 0.005 * torch.randn(v0.shape, dtype=torch.float64, device=dev, generator=g)
     out = {"workload": "adjoint: two-disk N=16384, 200 KDK steps, dt=1e-3, checkpoints 20"}
     for prec in ("mixed", "fp64"):
         eng = gcsim.Engine("hip-direct", prec, softening=0.05)
-        with torch.no_grad():
-            vt = v0 + 0.005 * torch.randn(v0.shape, dtype=torch.float64, device=dev, generator=g)
+        with torch.no_grad():  # targets from the perturbed-velocity run
             target, _ = adjoint.trajectory(x0, vt, m, dt, steps, engine=eng)
         va, ma = v0.clone().requires_grad_(), m.clone().requires_grad_()
         torch.cuda.synchronize()
@@ -317,6 +317,7 @@ def main() -> int:
     k_ms, k_pairs = eng.time_force_kernel(reps=20)
     achieved = FLOP_PER_PAIR * k_pairs / (k_ms * 1e-3) / 1e12
     ipt, jsplit = eng.launch_shape()
+    sources = eng.sources() if args.precision == "mixed" else None
     diag = eng.diagnostics()
     eng.close()
 
@@ -338,6 +339,7 @@ def main() -> int:
             "workload": "two-disk collision, direct all-pairs KDK, N=65536, eps=0.02, dt=5e-4",
             "n": n, "eps": eps, "dt": dt, "precision": args.precision,
             "state": "fp64 resident in HBM", "targets_per_thread": ipt, "jsplit": jsplit,
+            "sources": sources,
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -106,7 +106,8 @@ gc_status ensure_capacity(gc_ctx* c, size_t n) {
     c->s.cap = cap;
     c->s.n = n;
     const size_t ntiles_cap = cap / kTile;
-    GC_HIP(c, hipMalloc(&c->mixed.jp, sizeof(float4) * cap), "allocate mixed source tiles");
+    GC_HIP(c, hipMalloc(&c->mixed.jp, sizeof(float4) * (cap + kStreamBatch)),
+           "allocate mixed source tiles");
     GC_HIP(c, hipMalloc(&c->mixed.centre, sizeof(double) * 3 * ntiles_cap), "allocate tile centres");
     c->mixed.ntiles = (n + kTile - 1) / kTile;
     GC_HIP(c, hipMalloc(&c->stage, sizeof(double) * 3 * cap), "allocate staging buffer");
@@ -117,11 +118,19 @@ gc_status ensure_capacity(gc_ctx* c, size_t n) {
     return GC_OK;
 }
 
-// Mixed-kernel launch shape: ~8192 blocks of 4 waves (32 waves per SIMD
-// over the launch, 8 resident), split over source tile ranges.  Measured at
-// N = 65,536: 1 target per lane with 32 splits (8192 blocks) runs 1.083 ms
-// against 1.219 ms with 8 splits and 1.489 ms with 4; 2 or 4 targets per
-// lane are slower at every split count.
+// Source delivery of the mixed kernel: per-wave scalar loads (default) or
+// LDS tiles shared by the block's four waves.  Measured at N = 65,536 with
+// 64 splits: 1.0855 ms per launch with scalar loads, 1.0932 ms with LDS
+// tiles (whole KDK step 1.118 ms against 1.122 ms).
+bool scalar_sources(const gc_ctx* c) { return c->shape.sources != GC_SOURCES_LDS; }
+
+// Mixed-kernel launch shape: ~16384 blocks of 4 waves (64 waves per SIMD
+// over the launch, 8 resident), split over source tile ranges, so the last
+// wave of blocks is short.  Measured at N = 65,536 (scalar loads): 1 target
+// per lane with 64 splits runs 1.0855 ms, 32 splits 1.1107 ms, 16 splits
+// 1.136 ms, 8 splits 1.195 ms; 128 splits 1.0785 ms but its partial sums
+// cost more in the merge than they save (step 1.137 ms against 1.118 ms);
+// 2 targets per lane are slower at every split count.
 void mixed_shape(const gc_ctx* c, size_t ntargets, int* ipt_out, int* jsplit_out) {
     int ipt = c->shape.ipt;
     if (ipt != 1 && ipt != 2 && ipt != 4) ipt = 1;
@@ -129,7 +138,7 @@ void mixed_shape(const gc_ctx* c, size_t ntargets, int* ipt_out, int* jsplit_out
     const size_t iblocks = std::max<size_t>(1, (ntargets + span - 1) / span);
     const int ntiles = static_cast<int>(std::max<size_t>(1, c->mixed.ntiles));
     int js = c->shape.jsplit;
-    if (js <= 0) js = static_cast<int>(std::min<size_t>(64, (8192 + iblocks - 1) / iblocks));
+    if (js <= 0) js = static_cast<int>(std::min<size_t>(64, (16384 + iblocks - 1) / iblocks));
     js = std::max(1, std::min(js, ntiles));
     const int tps = (ntiles + js - 1) / js;
     *ipt_out = ipt;
@@ -174,7 +183,8 @@ gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double hal
     gc_status st = ensure_partials(c, js);
     if (st != GC_OK) return st;
     launch_pack_mixed(c->s, c->mixed, p.gravitational_constant, c->stream);
-    launch_direct_mixed(c->s, c->mixed, p.softening, ipt, js, begin, end, c->stream);
+    launch_direct_mixed(c->s, c->mixed, p.softening, ipt, js, scalar_sources(c), begin, end,
+                        c->stream);
     launch_merge_mixed(c->s, c->mixed, js, begin, end, close_kick, half_dt, c->stream);
     return launch_status(c, "launch mixed direct kernels");
 }
@@ -459,7 +469,8 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
     GC_HIP(c, hipEventRecord(c->ev0, c->stream), "record start event");
     for (int r = 0; r < reps; ++r) {
         if (c->params.precision == GC_PRECISION_MIXED) {
-            launch_direct_mixed(c->s, c->mixed, c->params.softening, ipt, js, 0, n, c->stream);
+            launch_direct_mixed(c->s, c->mixed, c->params.softening, ipt, js, scalar_sources(c), 0, n,
+                                c->stream);
         } else {
             launch_direct_fp64(c->s, c->params.gravitational_constant, c->params.softening, 0, n,
                                false, 0.0, c->stream);
@@ -611,6 +622,19 @@ gc_status gc_get_launch_shape(gc_ctx* c, int* ipt, int* jsplit) {
     if (!c || !ipt || !jsplit) return fail(c, GC_ERR_INVALID_ARGUMENT, "get_launch_shape: null argument");
     mixed_shape(c, c->s.n, ipt, jsplit);
     return GC_OK;
+}
+
+gc_status gc_set_sources(gc_ctx* c, int mode) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (mode != GC_SOURCES_AUTO && mode != GC_SOURCES_LDS && mode != GC_SOURCES_SCALAR)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "source mode must be 0, 1 or 2");
+    c->shape.sources = mode;
+    return GC_OK;
+}
+
+int gc_get_sources(const gc_ctx* c) {
+    if (!c) return -1;
+    return scalar_sources(c) ? GC_SOURCES_SCALAR : GC_SOURCES_LDS;
 }
 
 }  // extern "C"

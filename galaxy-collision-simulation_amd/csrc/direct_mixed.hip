@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kTile) void pack_mixed_kernel(
     }
 }
 
-template <int IPT, bool SAFE>
+template <int IPT, bool SAFE, bool STREAM>
 __global__ __launch_bounds__(kTile) void direct_mixed_kernel(
     int begin, int end, int ntiles, int tiles_per_split, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ jp,
@@ -129,7 +129,12 @@ __global__ __launch_bounds__(kTile) void direct_mixed_kernel(
         hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
     }
 
-    tile_sweep<IPT, SAFE>(tile, t0, t1, jp, centre, ntiles, eps2, xi, yi, zi, hx, hy, hz, lx, ly, lz);
+    if (STREAM) {
+        tile_stream<IPT, SAFE>(t0, t1, jp, centre, ntiles, eps2, xi, yi, zi, hx, hy, hz, lx, ly, lz);
+    } else {
+        tile_sweep<IPT, SAFE>(tile, t0, t1, jp, centre, ntiles, eps2, xi, yi, zi, hx, hy, hz, lx,
+                              ly, lz);
+    }
     float* base = part + static_cast<size_t>(split) * 6 * cap;
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
@@ -169,15 +174,17 @@ __global__ __launch_bounds__(256) void merge_mixed_kernel(
     }
 }
 
-template <int IPT>
+template <int IPT, bool STREAM>
 void launch_ipt(bool safe, dim3 grid, hipStream_t st, int begin, int end, int ntiles, int tps,
                 const DeviceState& s, const MixedLayout& L, float eps2, int cap) {
     if (safe) {
-        hipLaunchKernelGGL((direct_mixed_kernel<IPT, true>), grid, dim3(kTile), 0, st, begin,
-                           end, ntiles, tps, s.x, s.y, s.z, L.jp, L.centre, eps2, L.part, cap);
+        hipLaunchKernelGGL((direct_mixed_kernel<IPT, true, STREAM>), grid, dim3(kTile), 0, st,
+                           begin, end, ntiles, tps, s.x, s.y, s.z, L.jp, L.centre, eps2, L.part,
+                           cap);
     } else {
-        hipLaunchKernelGGL((direct_mixed_kernel<IPT, false>), grid, dim3(kTile), 0, st, begin,
-                           end, ntiles, tps, s.x, s.y, s.z, L.jp, L.centre, eps2, L.part, cap);
+        hipLaunchKernelGGL((direct_mixed_kernel<IPT, false, STREAM>), grid, dim3(kTile), 0, st,
+                           begin, end, ntiles, tps, s.x, s.y, s.z, L.jp, L.centre, eps2, L.part,
+                           cap);
     }
 }
 
@@ -191,7 +198,7 @@ void launch_pack_mixed(const DeviceState& s, MixedLayout& L, double G, hipStream
 
 // jsplit here is the effective split count (ceil(ntiles / tiles_per_split)).
 void launch_direct_mixed(const DeviceState& s, MixedLayout& L, double eps, int ipt, int jsplit,
-                         size_t begin, size_t end, hipStream_t st) {
+                         bool stream, size_t begin, size_t end, hipStream_t st) {
     if (end <= begin) return;
     const int ntiles = static_cast<int>(L.ntiles);
     const int tps = (ntiles + jsplit - 1) / jsplit;
@@ -200,10 +207,18 @@ void launch_direct_mixed(const DeviceState& s, MixedLayout& L, double eps, int i
     const size_t span = static_cast<size_t>(kTile) * ipt;
     dim3 grid(static_cast<unsigned>((end - begin + span - 1) / span), static_cast<unsigned>(jsplit));
     const int b = static_cast<int>(begin), e = static_cast<int>(end), cap = static_cast<int>(s.cap);
-    switch (ipt) {
-    case 4: launch_ipt<4>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
-    case 2: launch_ipt<2>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
-    default: launch_ipt<1>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+    if (stream) {
+        switch (ipt) {
+        case 4: launch_ipt<4, true>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+        case 2: launch_ipt<2, true>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+        default: launch_ipt<1, true>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+        }
+    } else {
+        switch (ipt) {
+        case 4: launch_ipt<4, false>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+        case 2: launch_ipt<2, false>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+        default: launch_ipt<1, false>(safe, grid, st, b, e, ntiles, tps, s, L, eps2, cap); break;
+        }
     }
 }
 

@@ -14,6 +14,9 @@ namespace gc {
 // Source tile: the LDS j-tile of the all-pairs kernels, and the group of
 // particles that shares one FP64 centre in the mixed-precision layout.
 constexpr int kTile = 256;
+// Sources the scalar-load sweep reads ahead; the source array carries this
+// many entries of padding past its last tile.
+constexpr int kStreamBatch = 8;
 
 inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
@@ -31,15 +34,16 @@ struct DeviceState {
 // FP32 relative to its tile's FP64 centre c; per tile the FP64 centre.
 struct MixedLayout {
     size_t ntiles = 0;
-    float4* jp = nullptr;      // ntiles * kTile entries, zero-mass padding
+    float4* jp = nullptr;      // ntiles * kTile entries (zero-mass padding) + kStreamBatch
     double* centre = nullptr;  // 3 * ntiles (SoA: cx | cy | cz)
     float* part = nullptr;     // jsplit * 6 * cap (hi x,y,z | lo x,y,z)
     size_t part_floats = 0;
 };
 
 struct LaunchShape {
-    int ipt = 0;     // targets per thread (1, 2, 4); 0 = automatic
-    int jsplit = 0;  // source-range splits; 0 = automatic
+    int ipt = 0;      // targets per thread (1, 2, 4); 0 = automatic
+    int jsplit = 0;   // source-range splits; 0 = automatic
+    int sources = 0;  // GC_SOURCES_*: 0 automatic, 1 LDS tiles, 2 scalar loads
 };
 
 // ---- kernel launchers (defined in the .hip files) ------------------------
@@ -48,8 +52,9 @@ void launch_direct_fp64(const DeviceState& s, double G, double eps, size_t begin
                         bool close_kick, double half_dt, hipStream_t st);
 // direct_mixed.hip
 void launch_pack_mixed(const DeviceState& s, MixedLayout& L, double G, hipStream_t st);
+// stream: sources by wave-uniform (scalar) loads instead of LDS tiles
 void launch_direct_mixed(const DeviceState& s, MixedLayout& L, double eps, int ipt, int jsplit,
-                         size_t begin, size_t end, hipStream_t st);
+                         bool stream, size_t begin, size_t end, hipStream_t st);
 void launch_merge_mixed(const DeviceState& s, const MixedLayout& L, int jsplit, size_t begin,
                         size_t end, bool close_kick, double half_dt, hipStream_t st);
 // integrate.hip

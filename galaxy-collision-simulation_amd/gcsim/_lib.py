@@ -22,6 +22,7 @@ GC_ERR_STATE = 6
 
 GC_SOLVER_DIRECT, GC_SOLVER_FMM = 0, 1
 GC_PRECISION_FP64, GC_PRECISION_MIXED = 0, 1
+GC_SOURCES_AUTO, GC_SOURCES_LDS, GC_SOURCES_SCALAR = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -116,6 +117,8 @@ SIGNATURES = {
     "gc_calibration_copy": (C.c_int, [C.c_int, C.c_size_t, C.c_int, _dp]),
     "gc_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gc_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gc_set_sources": (C.c_int, [C.c_void_p, C.c_int]),
+    "gc_get_sources": (C.c_int, [C.c_void_p]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),
     "gc_generate_disks": (C.c_size_t, [C.POINTER(DiskGalaxy), C.c_int, C.c_double, C.c_double,
                                        C.c_uint64, _dp, _dp, _dp]),

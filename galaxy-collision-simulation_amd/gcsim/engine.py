@@ -23,6 +23,7 @@ _dp = C.POINTER(C.c_double)
 SOLVERS = {"hip-direct": L.GC_SOLVER_DIRECT, "hip": L.GC_SOLVER_DIRECT,
            "rocm-direct": L.GC_SOLVER_DIRECT, "hip-fmm": L.GC_SOLVER_FMM,
            "rocm-fmm": L.GC_SOLVER_FMM}
+SOURCES = {"auto": L.GC_SOURCES_AUTO, "lds": L.GC_SOURCES_LDS, "scalar": L.GC_SOURCES_SCALAR}
 PRECISIONS = {"fp64": L.GC_PRECISION_FP64, "double": L.GC_PRECISION_FP64,
               "mixed": L.GC_PRECISION_MIXED, "fp32": L.GC_PRECISION_MIXED}
 
@@ -130,6 +131,15 @@ class Engine:
         self._check(self._lib.gc_get_launch_shape(self._ctx, C.byref(a), C.byref(b)),
                     "get launch shape")
         return a.value, b.value
+
+    def set_sources(self, mode="auto"):
+        """Mixed-kernel source reads: "auto", "lds" (block-shared LDS tiles) or
+        "scalar" (wave-uniform scalar loads)."""
+        self._check(self._lib.gc_set_sources(self._ctx, SOURCES[mode]), "set sources")
+
+    def sources(self) -> str:
+        return {L.GC_SOURCES_LDS: "lds",
+                L.GC_SOURCES_SCALAR: "scalar"}[self._lib.gc_get_sources(self._ctx)]
 
     # -- state
     @property

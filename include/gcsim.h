@@ -225,6 +225,17 @@ gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms_per
 gc_status gc_set_launch_shape(gc_ctx* ctx, int targets_per_thread, int jsplit);
 gc_status gc_get_launch_shape(gc_ctx* ctx, int* targets_per_thread, int* jsplit);
 
+/* Tuning: how the mixed kernel's waves read source particles.  LDS: the
+ * block stages each source tile in LDS (two barriers per tile) and lanes
+ * read it by broadcast.  SCALAR: each wave reads the tile with wave-uniform
+ * scalar loads into SGPRs, no LDS and no barrier.  Same arithmetic in the
+ * same order, so both give identical sums.  AUTO (the default) is SCALAR,
+ * the faster form on gfx950; gc_get_sources returns the form in effect
+ * (1 or 2), -1 for a null context. */
+enum { GC_SOURCES_AUTO = 0, GC_SOURCES_LDS = 1, GC_SOURCES_SCALAR = 2 };
+gc_status gc_set_sources(gc_ctx* ctx, int mode);
+int gc_get_sources(const gc_ctx* ctx);
+
 
 /* ---- synthetic initial conditions (host side) -------------------------
  * No reference counterpart for these workloads: the reference's generator

@@ -23,10 +23,13 @@ def _engine(gpu, precision, eps, **kw):
     return gpu.Engine("hip-direct", precision, softening=eps, **kw)
 
 
-def _accel(gpu, precision, pos, vel, mass, eps, G=1.0, shape=None):
+def _accel(gpu, precision, pos, vel, mass, eps, G=1.0, shape=None, sources=None):
     with _engine(gpu, precision, eps, G=G) as eng:
         if shape:
             eng.set_launch_shape(*shape)
+        if sources:
+            eng.set_sources(sources)
+            assert eng.sources() == sources
         eng.upload(pos, vel, mass)
         eng.compute_accelerations()
         return eng.download()[2]
@@ -126,8 +129,32 @@ def test_mixed_config0_error(gpu):
 def test_mixed_launch_shapes(gpu, shape):
     pos, vel, m = gpu.generators.config1(n=6000)
     ref = O.direct_targets(pos, m, 1.0, 0.02, threads=0)
-    err = rel_err(_accel(gpu, "mixed", pos, vel, m, 0.02, shape=shape), ref)
-    assert err.max() <= MIXED_TOL
+    got = {src: _accel(gpu, "mixed", pos, vel, m, 0.02, shape=shape, sources=src)
+           for src in ("lds", "scalar")}
+    assert rel_err(got["lds"], ref).max() <= MIXED_TOL
+    # the two source paths run the same arithmetic in the same order
+    assert np.array_equal(got["lds"], got["scalar"])
+
+
+@pytest.mark.parametrize("sources", ["lds", "scalar"])
+def test_mixed_sources_edge_cases(gpu, sources):
+    rng = np.random.default_rng(17)
+    pos, m = rng.normal(size=(1100, 3)), rng.uniform(0.1, 1.0, 1100)
+    ref = O.direct_targets(pos, m, 1.0, 0.02)
+    for lo, hi in ((0, 1100), (256, 768), (300, 650), (1099, 1100)):
+        with _engine(gpu, "mixed", 0.02) as eng:
+            eng.set_sources(sources)
+            eng.upload(pos, None, m)
+            eng.compute_accelerations(lo, hi)
+            acc = eng.download()[2]
+        assert not acc[:lo].any() and not acc[hi:].any()
+        assert rel_err(acc[lo:hi], ref[lo:hi]).max() <= MIXED_TOL
+    # coincident particles without softening: finite, the pair adds nothing
+    pos2, m2 = np.concatenate([pos[:300], pos[:1]]), m[:301]
+    got = _accel(gpu, "mixed", pos2, None, m2, 0.0, sources=sources)
+    assert np.isfinite(got).all()
+    assert rel_err(got, O.direct_targets(pos2, m2, 1.0, 0.0)).max() <= MIXED_TOL
+    assert not _accel(gpu, "mixed", pos[:1], None, m[:1], 0.02, sources=sources).any()
 
 
 def test_mixed_full_size_against_fp64_gpu(gpu):
