@@ -318,10 +318,16 @@ def main() -> int:
     achieved = FLOP_PER_PAIR * k_pairs / (k_ms * 1e-3) / 1e12
     ipt, jsplit = eng.launch_shape()
     sources = eng.sources() if args.precision == "mixed" else None
+    pair_form = eng.pairs() if args.precision == "mixed" else "targets"
     diag = eng.diagnostics()
     eng.close()
 
-    kernel_name = "direct_mixed_kernel" if args.precision == "mixed" else "direct_fp64_kernel"
+    if args.precision == "fp64":
+        kernel_name = "direct_fp64_kernel"
+    elif pair_form == "symmetric":
+        kernel_name = "direct_sym_kernel"
+    else:
+        kernel_name = "direct_mixed_kernel"
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -339,7 +345,7 @@ def main() -> int:
             "workload": "two-disk collision, direct all-pairs KDK, N=65536, eps=0.02, dt=5e-4",
             "n": n, "eps": eps, "dt": dt, "precision": args.precision,
             "state": "fp64 resident in HBM", "targets_per_thread": ipt, "jsplit": jsplit,
-            "sources": sources,
+            "sources": sources, "pairs": pair_form,
             "parallelism": "single GPU",
         },
         "roofline": {

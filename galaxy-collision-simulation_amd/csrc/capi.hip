@@ -21,6 +21,7 @@ struct gc_ctx {
     DeviceState s;
     MixedLayout mixed;
     LaunchShape shape;
+    SymPlan sym;
     double* stage = nullptr;  // 3 * cap doubles: interleaved host transfers
     double* diag_scratch = nullptr;
     size_t diag_scratch_doubles = 0;
@@ -69,6 +70,7 @@ gc_status check_params(gc_ctx* c, const gc_params* p) {
 }
 
 void free_state(gc_ctx* c) {
+    sym_release(c->sym);
     (void)hipFree(c->s.base);
     (void)hipFree(c->mixed.jp);
     (void)hipFree(c->mixed.centre);
@@ -145,6 +147,32 @@ void mixed_shape(const gc_ctx* c, size_t ntargets, int* ipt_out, int* jsplit_out
     *jsplit_out = (ntiles + tps - 1) / tps;
 }
 
+// Each unordered pair once (direct_sym.hip) for full-range forces of up to
+// kSymMaxTiles tiles (the default); target ranges run the per-target
+// kernel.  Measured at N = 65,536: 1.003 ms per launch (2 targets per lane)
+// against 1.090 ms for the per-target kernel; KDK step 1.088 ms against
+// 1.124-1.155 ms.
+bool symmetric_pairs(const gc_ctx* c, size_t begin, size_t end) {
+    if (c->shape.pairs == GC_PAIRS_TARGETS) return false;
+    return begin == 0 && end == c->s.n && c->mixed.ntiles >= 1 &&
+           c->mixed.ntiles <= static_cast<size_t>(kSymMaxTiles);
+}
+
+// Targets per lane of the symmetric kernel: 2 unless set (1 ran 1.074 ms,
+// 4 ran 1.0025 ms against 1.0033 ms for 2 at N = 65,536; 2 keeps more
+// blocks for small N).
+int sym_ipt(const gc_ctx* c) {
+    const int ipt = c->shape.ipt;
+    return (ipt == 1 || ipt == 2 || ipt == 4) ? ipt : 2;
+}
+
+gc_status ensure_sym(gc_ctx* c) {
+    std::string msg;
+    const int rc = sym_prepare(c->sym, static_cast<int>(c->mixed.ntiles), c->stream, &msg);
+    if (rc == 0) return GC_OK;
+    return fail(c, rc == 2 ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP, msg);
+}
+
 gc_status ensure_partials(gc_ctx* c, int jsplit) {
     const size_t need = static_cast<size_t>(jsplit) * 6 * c->s.cap;
     if (need <= c->mixed.part_floats) return GC_OK;
@@ -177,6 +205,14 @@ gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double hal
         launch_direct_fp64(c->s, p.gravitational_constant, p.softening, begin, end, close_kick,
                            half_dt, c->stream);
         return launch_status(c, "launch FP64 direct kernel");
+    }
+    if (symmetric_pairs(c, begin, end)) {
+        gc_status st = ensure_sym(c);
+        if (st != GC_OK) return st;
+        launch_pack_mixed(c->s, c->mixed, p.gravitational_constant, c->stream);
+        launch_direct_sym(c->s, c->mixed, c->sym, p.softening, sym_ipt(c), c->stream);
+        launch_merge_sym(c->s, c->sym, close_kick, half_dt, c->stream);
+        return launch_status(c, "launch symmetric mixed kernels");
     }
     int ipt = 1, js = 1;
     mixed_shape(c, end - begin, &ipt, &js);
@@ -460,15 +496,23 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
         return GC_OK;
     }
     int ipt = 1, js = 1;
+    const bool sym = c->params.precision == GC_PRECISION_MIXED && symmetric_pairs(c, 0, n);
     if (c->params.precision == GC_PRECISION_MIXED) {
-        mixed_shape(c, n, &ipt, &js);
-        st = ensure_partials(c, js);
+        if (sym) {
+            st = ensure_sym(c);
+        } else {
+            mixed_shape(c, n, &ipt, &js);
+            st = ensure_partials(c, js);
+        }
         if (st != GC_OK) return st;
         launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
     }
     GC_HIP(c, hipEventRecord(c->ev0, c->stream), "record start event");
     for (int r = 0; r < reps; ++r) {
-        if (c->params.precision == GC_PRECISION_MIXED) {
+        if (sym) {
+            launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, sym_ipt(c),
+                              c->stream);
+        } else if (c->params.precision == GC_PRECISION_MIXED) {
             launch_direct_mixed(c->s, c->mixed, c->params.softening, ipt, js, scalar_sources(c), 0, n,
                                 c->stream);
         } else {
@@ -484,8 +528,10 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
     GC_HIP(c, hipEventElapsedTime(&total, c->ev0, c->ev1), "elapsed time");
     *ms = static_cast<double>(total) / reps;
     if (inter) *inter = static_cast<double>(n) * static_cast<double>(n - 1);
-    if (c->params.precision == GC_PRECISION_MIXED) {
+    if (sym) {
         // restore accelerations consistent with the state
+        launch_merge_sym(c->s, c->sym, false, 0.0, c->stream);
+    } else if (c->params.precision == GC_PRECISION_MIXED) {
         launch_merge_mixed(c->s, c->mixed, js, 0, n, false, 0.0, c->stream);
     }
     return launch_status(c, "timed force kernels");
@@ -635,6 +681,19 @@ gc_status gc_set_sources(gc_ctx* c, int mode) {
 int gc_get_sources(const gc_ctx* c) {
     if (!c) return -1;
     return scalar_sources(c) ? GC_SOURCES_SCALAR : GC_SOURCES_LDS;
+}
+
+gc_status gc_set_pairs(gc_ctx* c, int mode) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (mode != GC_PAIRS_AUTO && mode != GC_PAIRS_TARGETS && mode != GC_PAIRS_SYMMETRIC)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "pair mode must be 0, 1 or 2");
+    c->shape.pairs = mode;
+    return GC_OK;
+}
+
+int gc_get_pairs(const gc_ctx* c) {
+    if (!c) return -1;
+    return symmetric_pairs(c, 0, c->s.n) ? GC_PAIRS_SYMMETRIC : GC_PAIRS_TARGETS;
 }
 
 }  // extern "C"

@@ -1,0 +1,313 @@
+// direct_sym.hip — mixed-precision all-pairs forces, each unordered pair
+// evaluated once (Newton's third law), as the reference's symmetric direct
+// sum does (cpp/direct/direct_solver.cpp:37-57: one softened kernel per
+// i < j, added to a_i and subtracted from a_j).
+//
+// Work is the upper triangle of source-tile pairs (I, J), J >= I, cut into
+// items of a few consecutive J tiles of one row I.  A block of four waves
+// holds the 256 targets of tile I, one per lane; the sources of tile J
+// are staged in LDS (the mixed layout of direct_mixed.hip: FP32 offsets from
+// the tile's FP64 centre and G*m).  A wave meets the 64 sources of a chunk
+// in 64 steps; at step s lane l takes source (l + s) mod 64, so the 64 lanes
+// see 64 distinct sources and every (target, source) pair of the chunk comes
+// up once.  Per step and lane:
+//   d = o_j - t_i, s2 = |d|^2 + eps^2, r = rsq(s2), r3 = r^3
+//   a_i += (G m_j r3) d          (3 FMA into the lane's own target)
+//   A_j -= (G m_i r3) d          (3 FMA into the source's running sum)
+//   A <- A of lane l + 1         (3 DPP wave_rol:1 moves)
+// The source sums ride along the wave with their sources, so one v_rsq_f32
+// serves two interactions without any cross-lane reduction; after 64 steps
+// lane l holds source l's sum.  The four waves' sums are added in wave
+// order through LDS, and each (I, J) pair leaves one FP32 partial per
+// target of tile J.  The diagonal tile J == I runs the
+// one-sided form (tile_pairs of mixed_tiles.hpp).  The i-side sum of each J
+// tile is an FP32 partial folded into (hi, lo) by two-sum, as in the
+// one-sided kernel.  The merge adds, per target, its row's item partials in
+// J order and its column's pair partials in I order, in FP64: the result
+// does not depend on scheduling.
+#include "gc_internal.hpp"
+#include "mixed_tiles.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace gc {
+namespace {
+
+// DPP wave_rol:1 (GFX9 encoding): lane l receives lane (l + 1) mod 64.
+constexpr int kWaveRol1 = 0x134;
+
+__device__ __forceinline__ float wave_rol1(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), kWaveRol1, 0xF, 0xF, true));
+}
+
+__host__ __device__ inline size_t pair_slot(int I, int J) {  // I < J
+    return static_cast<size_t>(J) * static_cast<size_t>(J - 1) / 2 + static_cast<size_t>(I);
+}
+
+constexpr int kChunk = 64;                // sources per rotation (one wave)
+constexpr int kChunks = kTile / kChunk;   // 4
+
+// IPT targets per lane: a block of kTile / IPT threads holds the kTile
+// targets of tile I (thread t: targets t + k kTile / IPT).  The rotating
+// source sums are shared by a lane's IPT targets, so the three DPP moves of
+// a step serve 2 * IPT interactions.
+template <int IPT, bool SAFE>
+__global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
+    const int4* __restrict__ items, int n, int ntiles, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ jp,
+    const double* __restrict__ centre, float eps2, float* __restrict__ ipart,
+    float* __restrict__ jpart) {
+    constexpr int kThreads = kTile / IPT;
+    constexpr int kWaves = kThreads / 64;
+    // Source tile J in LDS, each 64-source chunk stored twice in a row so
+    // that lane l at step s reads entry l + s of its chunk without wrapping.
+    __shared__ float4 src2[kChunks][2 * kChunk];
+    // per wave: its lanes' sums of -G m_i r3 d for each source of the tile
+    __shared__ float4 red[kWaves][kTile];
+    const int4 item = items[blockIdx.x];
+    const int I = item.x;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    double xi[IPT], yi[IPT], zi[IPT];
+    float mi[IPT];
+    float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = I * kTile + k * kThreads + t;
+        const bool active = i < n;
+        xi[k] = active ? x[i] : 0.0;
+        yi[k] = active ? y[i] : 0.0;
+        zi[k] = active ? z[i] : 0.0;
+        mi[k] = jp[i].w;  // G m_i; zero past n
+        hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
+    }
+    for (int J = item.y; J < item.z; ++J) {
+        float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
+        tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
+        const float4* src = jp + static_cast<size_t>(J) * kTile;
+        if (J == I) {
+            tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, px, py, pz);
+        } else {
+            for (int e = t; e < kTile; e += kThreads) {
+                const float4 v = src[e];
+                src2[e / kChunk][e % kChunk] = v;
+                src2[e / kChunk][kChunk + e % kChunk] = v;
+            }
+            __syncthreads();
+            for (int c = 0; c < kChunks; ++c) {
+                // lane l holds the running sum of source c*64 + (l + s) % 64
+                // at step s; wave_rol1 after each step keeps that true.
+                float ax = 0.f, ay = 0.f, az = 0.f;
+                const float4* row = &src2[c][lane];
+#pragma unroll 8
+                for (int s = 0; s < kChunk; ++s) {
+                    const float4 q = row[s];
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        const float dx = q.x - tx[k], dy = q.y - ty[k], dz = q.z - tz[k];
+                        const float s2 = __builtin_fmaf(
+                            dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+                        float r;
+                        if (SAFE) {
+                            r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
+                        } else {
+                            r = __builtin_amdgcn_rsqf(s2);
+                        }
+                        const float r3 = (r * r) * r;
+                        const float kj = q.w * r3, ki = mi[k] * r3;
+                        px[k] = __builtin_fmaf(dx, kj, px[k]);
+                        py[k] = __builtin_fmaf(dy, kj, py[k]);
+                        pz[k] = __builtin_fmaf(dz, kj, pz[k]);
+                        ax = __builtin_fmaf(-dx, ki, ax);
+                        ay = __builtin_fmaf(-dy, ki, ay);
+                        az = __builtin_fmaf(-dz, ki, az);
+                    }
+                    ax = wave_rol1(ax);
+                    ay = wave_rol1(ay);
+                    az = wave_rol1(az);
+                }
+                // after 64 steps lane l holds source c*64 + l
+                red[wave][c * kChunk + lane] = make_float4(ax, ay, az, 0.f);
+            }
+            __syncthreads();
+            float* out = jpart + pair_slot(I, J) * 3 * kTile;
+            for (int e = t; e < kTile; e += kThreads) {
+                float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) {
+                    const float4 v = red[w][e];
+                    sx += v.x;
+                    sy += v.y;
+                    sz += v.z;
+                }
+                out[e] = sx;
+                out[kTile + e] = sy;
+                out[2 * kTile + e] = sz;
+            }
+        }
+        fold_tile<IPT>(px, py, pz, hx, hy, hz, lx, ly, lz);
+    }
+    float* base = ipart + static_cast<size_t>(item.w) * 6 * kTile;
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int e = k * kThreads + t;
+        base[e] = hx[k];
+        base[kTile + e] = hy[k];
+        base[2 * kTile + e] = hz[k];
+        base[3 * kTile + e] = lx[k];
+        base[4 * kTile + e] = ly[k];
+        base[5 * kTile + e] = lz[k];
+    }
+}
+
+// a_i = sum over the items of row T (J order) of (hi + lo) + sum over I < T
+// of the (I, T) pair partials, in FP64; optional closing half kick.  Four
+// thread groups per target split the column: group g adds the pair partials
+// of I in [g T / 4, (g + 1) T / 4) (group 0 also the row items) and the four
+// group sums are added in group order, so the result is deterministic.
+// Blocks run the longest columns (largest T) first.
+constexpr int kMergeGroups = 4;
+
+__global__ __launch_bounds__(kTile * kMergeGroups) void merge_sym_kernel(
+    int n, int ntiles, const int* __restrict__ row_start, const float* __restrict__ ipart,
+    const float* __restrict__ jpart, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, int close_kick, double half_dt) {
+#pragma clang fp contract(off)
+    __shared__ double part[kMergeGroups - 1][3][kTile];
+    const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
+    const int l = threadIdx.x % kTile;
+    const int g = threadIdx.x / kTile;
+    double a[3] = {0.0, 0.0, 0.0};
+    if (g == 0) {
+        for (int s = row_start[T]; s < row_start[T + 1]; ++s) {
+            const float* b = ipart + static_cast<size_t>(s) * 6 * kTile;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                a[c] += static_cast<double>(b[c * kTile + l]) +
+                        static_cast<double>(b[(c + 3) * kTile + l]);
+        }
+    }
+    const float* col = jpart + pair_slot(0, T) * 3 * kTile;
+    const int i0 = g * T / kMergeGroups, i1 = (g + 1) * T / kMergeGroups;
+    for (int I = i0; I < i1; ++I) {
+        const float* b = col + static_cast<size_t>(I) * 3 * kTile;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[c] += static_cast<double>(b[c * kTile + l]);
+    }
+    if (g > 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) part[g - 1][c][l] = a[c];
+    }
+    __syncthreads();
+    if (g > 0) return;
+#pragma unroll
+    for (int k = 0; k < kMergeGroups - 1; ++k) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[c] += part[k][c][l];
+    }
+    const int i = T * kTile + l;
+    if (i >= n) return;
+    ax[i] = a[0];
+    ay[i] = a[1];
+    az[i] = a[2];
+    if (close_kick) {
+        vx[i] = vx[i] + a[0] * half_dt;
+        vy[i] = vy[i] + a[1] * half_dt;
+        vz[i] = vz[i] + a[2] * half_dt;
+    }
+}
+
+}  // namespace
+
+int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err) {
+    if (ntiles == plan.ntiles && plan.items) return 0;
+    if (plan.items) (void)hipStreamSynchronize(st);
+    sym_release(plan);
+    if (ntiles < 1 || ntiles > kSymMaxTiles) {
+        *err = "symmetric form: tile count out of range";
+        return 1;
+    }
+    // items of `len` consecutive J tiles per row, ~8192 items in all
+    const size_t tri = static_cast<size_t>(ntiles) * (ntiles + 1) / 2;
+    const int len = static_cast<int>(std::max<size_t>(1, tri / 8192));
+    std::vector<int4> items;
+    std::vector<int> row_start(ntiles + 1, 0);
+    for (int I = 0; I < ntiles; ++I) {
+        row_start[I] = static_cast<int>(items.size());
+        for (int J = I; J < ntiles; J += len)
+            items.push_back(make_int4(I, J, std::min(J + len, ntiles), static_cast<int>(items.size())));
+    }
+    row_start[ntiles] = static_cast<int>(items.size());
+    // longest items first; ties keep row-major order
+    std::stable_sort(items.begin(), items.end(),
+                     [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+    plan.nitems = static_cast<int>(items.size());
+    plan.ipart_floats = static_cast<size_t>(plan.nitems) * 6 * kTile;
+    plan.jpart_floats = std::max<size_t>(1, pair_slot(0, ntiles)) * 3 * kTile;
+    hipError_t e = hipMalloc(&plan.items, sizeof(int4) * items.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.row_start, sizeof(int) * row_start.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.ipart, sizeof(float) * plan.ipart_floats);
+    if (e == hipSuccess) e = hipMalloc(&plan.jpart, sizeof(float) * plan.jpart_floats);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan.items, items.data(), sizeof(int4) * items.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan.row_start, row_start.data(), sizeof(int) * row_start.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // host vectors end here
+    if (e != hipSuccess) {
+        *err = std::string("symmetric form buffers: ") + hipGetErrorString(e);
+        sym_release(plan);
+        return e == hipErrorOutOfMemory ? 2 : 3;
+    }
+    plan.ntiles = ntiles;
+    return 0;
+}
+
+void sym_release(SymPlan& plan) {
+    (void)hipFree(plan.items);
+    (void)hipFree(plan.row_start);
+    (void)hipFree(plan.ipart);
+    (void)hipFree(plan.jpart);
+    plan = SymPlan{};
+}
+
+template <int IPT>
+void launch_sym_ipt(bool safe, const DeviceState& s, const MixedLayout& L, const SymPlan& plan,
+                    float eps2, hipStream_t st) {
+    const int n = static_cast<int>(s.n), ntiles = static_cast<int>(L.ntiles);
+    if (safe) {
+        hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), dim3(plan.nitems), dim3(kTile / IPT), 0,
+                           st, plan.items, n, ntiles, s.x, s.y, s.z, L.jp, L.centre, eps2,
+                           plan.ipart, plan.jpart);
+    } else {
+        hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), dim3(plan.nitems), dim3(kTile / IPT), 0,
+                           st, plan.items, n, ntiles, s.x, s.y, s.z, L.jp, L.centre, eps2,
+                           plan.ipart, plan.jpart);
+    }
+}
+
+void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
+                       int ipt, hipStream_t st) {
+    const float eps2 = static_cast<float>(eps * eps);
+    const bool safe = !(eps2 >= kTinySeparation2);
+    switch (ipt) {
+    case 4: launch_sym_ipt<4>(safe, s, L, plan, eps2, st); break;
+    case 2: launch_sym_ipt<2>(safe, s, L, plan, eps2, st); break;
+    default: launch_sym_ipt<1>(safe, s, L, plan, eps2, st); break;
+    }
+}
+
+void launch_merge_sym(const DeviceState& s, const SymPlan& plan, bool close_kick, double half_dt,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(merge_sym_kernel, dim3(plan.ntiles), dim3(kTile * kMergeGroups), 0, st,
+                       static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.ipart, plan.jpart,
+                       s.ax, s.ay, s.az, s.vx, s.vy, s.vz, close_kick ? 1 : 0, half_dt);
+}
+
+}  // namespace gc

@@ -40,10 +40,24 @@ struct MixedLayout {
     size_t part_floats = 0;
 };
 
+// Symmetric (each unordered pair once) mixed form: the work list over the
+// upper triangle of source-tile pairs and its partial sums.
+constexpr int kSymMaxTiles = 1024;  // N <= 262,144: pair partials <= 1.6 GB
+struct SymPlan {
+    int ntiles = 0;             // plan built for this tile count
+    int nitems = 0;
+    int4* items = nullptr;      // (row I, J begin, J end, slot), longest first
+    int* row_start = nullptr;   // ntiles + 1 slot offsets; a row's slots in J order
+    float* ipart = nullptr;     // nitems * 6 * kTile (hi x,y,z | lo x,y,z)
+    float* jpart = nullptr;     // ntiles (ntiles - 1) / 2 pairs * 3 * kTile
+    size_t ipart_floats = 0, jpart_floats = 0;
+};
+
 struct LaunchShape {
     int ipt = 0;      // targets per thread (1, 2, 4); 0 = automatic
     int jsplit = 0;   // source-range splits; 0 = automatic
     int sources = 0;  // GC_SOURCES_*: 0 automatic, 1 LDS tiles, 2 scalar loads
+    int pairs = 0;    // GC_PAIRS_*: 0 automatic, 1 per target, 2 each pair once
 };
 
 // ---- kernel launchers (defined in the .hip files) ------------------------
@@ -57,6 +71,14 @@ void launch_direct_mixed(const DeviceState& s, MixedLayout& L, double eps, int i
                          bool stream, size_t begin, size_t end, hipStream_t st);
 void launch_merge_mixed(const DeviceState& s, const MixedLayout& L, int jsplit, size_t begin,
                         size_t end, bool close_kick, double half_dt, hipStream_t st);
+// direct_sym.hip: 0 on success, 2 out of memory, 1/3 other errors (err set)
+int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err);
+void sym_release(SymPlan& plan);
+// ipt: targets per lane (1, 2, 4)
+void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
+                       int ipt, hipStream_t st);
+void launch_merge_sym(const DeviceState& s, const SymPlan& plan, bool close_kick, double half_dt,
+                      hipStream_t st);
 // integrate.hip
 void launch_open_step(const DeviceState& s, double half_dt, double dt, hipStream_t st);
 void launch_kick(const DeviceState& s, double dt, hipStream_t st);

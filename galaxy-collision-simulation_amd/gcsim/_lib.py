@@ -23,6 +23,7 @@ GC_ERR_STATE = 6
 GC_SOLVER_DIRECT, GC_SOLVER_FMM = 0, 1
 GC_PRECISION_FP64, GC_PRECISION_MIXED = 0, 1
 GC_SOURCES_AUTO, GC_SOURCES_LDS, GC_SOURCES_SCALAR = 0, 1, 2
+GC_PAIRS_AUTO, GC_PAIRS_TARGETS, GC_PAIRS_SYMMETRIC = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 
@@ -119,6 +120,8 @@ SIGNATURES = {
     "gc_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_set_sources": (C.c_int, [C.c_void_p, C.c_int]),
     "gc_get_sources": (C.c_int, [C.c_void_p]),
+    "gc_set_pairs": (C.c_int, [C.c_void_p, C.c_int]),
+    "gc_get_pairs": (C.c_int, [C.c_void_p]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),
     "gc_generate_disks": (C.c_size_t, [C.POINTER(DiskGalaxy), C.c_int, C.c_double, C.c_double,
                                        C.c_uint64, _dp, _dp, _dp]),

@@ -23,6 +23,7 @@ _dp = C.POINTER(C.c_double)
 SOLVERS = {"hip-direct": L.GC_SOLVER_DIRECT, "hip": L.GC_SOLVER_DIRECT,
            "rocm-direct": L.GC_SOLVER_DIRECT, "hip-fmm": L.GC_SOLVER_FMM,
            "rocm-fmm": L.GC_SOLVER_FMM}
+PAIRS = {"auto": L.GC_PAIRS_AUTO, "targets": L.GC_PAIRS_TARGETS, "symmetric": L.GC_PAIRS_SYMMETRIC}
 SOURCES = {"auto": L.GC_SOURCES_AUTO, "lds": L.GC_SOURCES_LDS, "scalar": L.GC_SOURCES_SCALAR}
 PRECISIONS = {"fp64": L.GC_PRECISION_FP64, "double": L.GC_PRECISION_FP64,
               "mixed": L.GC_PRECISION_MIXED, "fp32": L.GC_PRECISION_MIXED}
@@ -140,6 +141,15 @@ class Engine:
     def sources(self) -> str:
         return {L.GC_SOURCES_LDS: "lds",
                 L.GC_SOURCES_SCALAR: "scalar"}[self._lib.gc_get_sources(self._ctx)]
+
+    def set_pairs(self, mode="auto"):
+        """Mixed full-range pair coverage: "auto", "targets" (every target over
+        all sources) or "symmetric" (each unordered pair once)."""
+        self._check(self._lib.gc_set_pairs(self._ctx, PAIRS[mode]), "set pairs")
+
+    def pairs(self) -> str:
+        return {L.GC_PAIRS_TARGETS: "targets",
+                L.GC_PAIRS_SYMMETRIC: "symmetric"}[self._lib.gc_get_pairs(self._ctx)]
 
     # -- state
     @property

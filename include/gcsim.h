@@ -236,6 +236,22 @@ enum { GC_SOURCES_AUTO = 0, GC_SOURCES_LDS = 1, GC_SOURCES_SCALAR = 2 };
 gc_status gc_set_sources(gc_ctx* ctx, int mode);
 int gc_get_sources(const gc_ctx* ctx);
 
+/* Tuning: how the mixed path covers the pairs of a full-range force.
+ * TARGETS: every target sums over all sources (the targets form,
+ * cpp/direct/direct_solver.cpp:59-83), 2 rsq per unordered pair.
+ * SYMMETRIC: each unordered pair once, its force added to one particle and
+ * subtracted from the other (the reference's symmetric form,
+ * cpp/direct/direct_solver.cpp:37-57), 1 rsq per unordered pair; used up to
+ * 262,144 particles, target ranges always run TARGETS.  AUTO (the
+ * default) is SYMMETRIC where it applies.  Both meet the same error bound;
+ * sums differ by rounding.  gc_set_launch_shape's targets_per_thread also
+ * sets the SYMMETRIC kernel's targets per lane (default 2).  gc_get_pairs
+ * returns the form a full-range force of the resident particles uses (1 or
+ * 2), -1 for a null context. */
+enum { GC_PAIRS_AUTO = 0, GC_PAIRS_TARGETS = 1, GC_PAIRS_SYMMETRIC = 2 };
+gc_status gc_set_pairs(gc_ctx* ctx, int mode);
+int gc_get_pairs(const gc_ctx* ctx);
+
 
 /* ---- synthetic initial conditions (host side) -------------------------
  * No reference counterpart for these workloads: the reference's generator

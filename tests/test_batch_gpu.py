@@ -24,6 +24,9 @@ def _cat(parts):
 def _alone(gpu, pos, vel, m, chunk, steps, dt=1e-3):
     ntiles = (len(m) + 255) // 256
     with gpu.Engine("hip-direct", "mixed", softening=0.05) as eng:
+        # the batch kernel runs the per-target form with LDS source tiles
+        eng.set_pairs("targets")
+        eng.set_sources("lds")
         eng.set_launch_shape(1, (ntiles + chunk - 1) // chunk)
         eng.upload(pos, vel, m)
         eng.compute_accelerations()

@@ -12,7 +12,7 @@ import pytest
 
 import oracle as O
 from conftest import load_diagnostics, load_snapshot
-from helpers import rel_err, smoke_cube
+from helpers import pair_magnitude_sums, rel_err, smoke_cube
 
 pytestmark = pytest.mark.gpu
 
@@ -23,13 +23,15 @@ def _engine(gpu, precision, eps, **kw):
     return gpu.Engine("hip-direct", precision, softening=eps, **kw)
 
 
-def _accel(gpu, precision, pos, vel, mass, eps, G=1.0, shape=None, sources=None):
+def _accel(gpu, precision, pos, vel, mass, eps, G=1.0, shape=None, sources=None, pairs=None):
     with _engine(gpu, precision, eps, G=G) as eng:
         if shape:
             eng.set_launch_shape(*shape)
         if sources:
             eng.set_sources(sources)
             assert eng.sources() == sources
+        if pairs:
+            eng.set_pairs(pairs)
         eng.upload(pos, vel, mass)
         eng.compute_accelerations()
         return eng.download()[2]
@@ -157,6 +159,88 @@ def test_mixed_sources_edge_cases(gpu, sources):
     assert not _accel(gpu, "mixed", pos[:1], None, m[:1], 0.02, sources=sources).any()
 
 
+@pytest.mark.parametrize("ipt", [1, 2, 4])
+@pytest.mark.parametrize("n", [1, 2, 255, 256, 257, 700, 1000, 3001])
+def test_mixed_symmetric_pairs_random(gpu, n, ipt):
+    # each unordered pair once (the reference's symmetric form,
+    # cpp/direct/direct_solver.cpp:37-57), against the FP64 targets form
+    rng = np.random.default_rng(100 + n)
+    pos, m = rng.normal(size=(n, 3)), rng.uniform(0.1, 1.0, n)
+    ref = O.direct_targets(pos, m, 1.5, 0.02)
+    with _engine(gpu, "mixed", 0.02, G=1.5) as eng:
+        eng.set_pairs("symmetric")
+        eng.set_launch_shape(ipt, 0)
+        eng.upload(pos, None, m)
+        assert eng.pairs() == "symmetric"
+        eng.compute_accelerations()
+        got = eng.download()[2]
+    # a random cloud at eps = 0.02 has particles whose force cancels to a
+    # small fraction of its terms: the bound is on the FP32 rounding of the
+    # terms (64 ulp of sum |term|), and 1e-6 relative in median
+    err = np.linalg.norm(got - ref, axis=1)
+    assert (err <= 64 * 2.0**-24 * pair_magnitude_sums(pos, m, 1.5, 0.02) + 1e-300).all()
+    assert np.median(rel_err(got, ref)) < 1e-6
+    # momentum: the pair forces cancel, sum m a = 0 up to FP32 rounding
+    scale = np.abs(m[:, None] * ref).sum(axis=0).max()
+    assert np.abs((m[:, None] * got).sum(axis=0)).max() <= 1e-6 * max(scale, 1e-300)
+
+
+def test_mixed_symmetric_pairs_two_disks(gpu):
+    pos, vel, m = gpu.generators.config1(n=6000)
+    ref = O.direct_targets(pos, m, 1.0, 0.02, threads=0)
+    sym = _accel(gpu, "mixed", pos, vel, m, 0.02, pairs="symmetric")
+    tgt = _accel(gpu, "mixed", pos, vel, m, 0.02, pairs="targets")
+    for got in (sym, tgt):
+        err = rel_err(got, ref)
+        assert err.max() <= MIXED_TOL and np.median(err) < 1e-6
+    # deterministic: a second evaluation gives the same bits
+    assert np.array_equal(sym, _accel(gpu, "mixed", pos, vel, m, 0.02, pairs="symmetric"))
+
+
+def test_mixed_symmetric_edge_cases(gpu):
+    rng = np.random.default_rng(23)
+    pos, m = rng.normal(size=(700, 3)), rng.uniform(0.1, 1.0, 700)
+    # target ranges run the per-target form and leave the others untouched
+    with _engine(gpu, "mixed", 0.02) as eng:
+        eng.set_pairs("symmetric")
+        eng.upload(pos, None, m)
+        eng.compute_accelerations(100, 600)
+        acc = eng.download()[2]
+    ref = O.direct_targets(pos, m, 1.0, 0.02)
+    assert not acc[:100].any() and not acc[600:].any()
+    assert rel_err(acc[100:600], ref[100:600]).max() <= MIXED_TOL
+    # coincident particles in different tiles without softening
+    pos2, m2 = np.concatenate([pos[:300], pos[:1]]), m[:301]
+    got = _accel(gpu, "mixed", pos2, None, m2, 0.0, pairs="symmetric")
+    assert np.isfinite(got).all()
+    assert rel_err(got, O.direct_targets(pos2, m2, 1.0, 0.0)).max() <= MIXED_TOL
+    # re-upload with fewer particles reuses the context: stale state past n
+    # must not leak into the sums
+    with _engine(gpu, "mixed", 0.02) as eng:
+        eng.set_pairs("symmetric")
+        eng.upload(pos, None, m)
+        eng.compute_accelerations()
+        eng.upload(pos[:333], None, m[:333])
+        eng.compute_accelerations()
+        got = eng.download()[2]
+    assert rel_err(got, O.direct_targets(pos[:333], m[:333], 1.0, 0.02)).max() <= MIXED_TOL
+
+
+def test_mixed_symmetric_kdk_tracks_fp64(gpu):
+    pos, vel, m = gpu.generators.config0()
+    with _engine(gpu, "mixed", 0.05) as mx, _engine(gpu, "fp64", 0.05) as fp:
+        mx.set_pairs("symmetric")
+        for eng in (mx, fp):
+            eng.upload(pos, vel, m)
+            eng.compute_accelerations()
+            eng.step(1e-3, 50)
+        pm = mx.download()[0]
+        pf = fp.download()[0]
+        e_m, e_f = mx.diagnostics(), fp.diagnostics()
+    assert np.abs(pm - pf).max() < 1e-6
+    assert abs(e_m["total_energy"] - e_f["total_energy"]) < 1e-7 * abs(e_f["total_energy"])
+
+
 def test_mixed_full_size_against_fp64_gpu(gpu):
     # N = 65,536 (the precision workload): the FP64 GPU path is the oracle
     # here (bit-identical to the CPU FP64 form above); spot-check it on a
@@ -166,9 +250,10 @@ def test_mixed_full_size_against_fp64_gpu(gpu):
     sample = np.arange(0, 65536, 257)
     cpu = O.direct_targets(pos, m, 1.0, 0.02, threads=0)
     assert np.array_equal(a64[sample], cpu[sample])
-    err = rel_err(_accel(gpu, "mixed", pos, vel, m, 0.02), a64)
-    assert err.max() <= MIXED_TOL
-    assert np.median(err) < 1e-6
+    for pairs in ("targets", "symmetric"):
+        err = rel_err(_accel(gpu, "mixed", pos, vel, m, 0.02, pairs=pairs), a64)
+        assert err.max() <= MIXED_TOL, pairs
+        assert np.median(err) < 1e-6, pairs
 
 
 def test_mixed_kdk_tracks_fp64(gpu):
