@@ -64,8 +64,8 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
     // Source tile J in LDS, each 64-source chunk stored twice in a row so
     // that lane l at step s reads entry l + s of its chunk without wrapping.
     __shared__ float4 src2[kChunks][2 * kChunk];
-    // per wave: its lanes' sums of -G m_i r3 d for each source of the tile
-    __shared__ float4 red[kWaves][kTile];
+    // waves 1.. hand their lanes' chunk sums of -G m_i r3 d to wave 0
+    __shared__ float red[kWaves > 1 ? kWaves - 1 : 1][3][kChunk];
     const int4 item = items[blockIdx.x];
     const int I = item.x;
     const int t = threadIdx.x;
@@ -130,23 +130,31 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
                     ay = wave_rol1(ay);
                     az = wave_rol1(az);
                 }
-                // after 64 steps lane l holds source c*64 + l
-                red[wave][c * kChunk + lane] = make_float4(ax, ay, az, 0.f);
-            }
-            __syncthreads();
-            float* out = jpart + pair_slot(I, J) * 3 * kTile;
-            for (int e = t; e < kTile; e += kThreads) {
-                float sx = 0.f, sy = 0.f, sz = 0.f;
+                // after 64 steps lane l holds source c*64 + l; wave 0 adds
+                // the other waves' sums in wave order and stores the chunk
+                float* out = jpart + pair_slot(I, J) * 3 * kTile + c * kChunk + lane;
+                if (kWaves > 1) {
+                    if (wave > 0) {
+                        red[wave - 1][0][lane] = ax;
+                        red[wave - 1][1][lane] = ay;
+                        red[wave - 1][2][lane] = az;
+                    }
+                    __syncthreads();
+                    if (wave == 0) {
 #pragma unroll
-                for (int w = 0; w < kWaves; ++w) {
-                    const float4 v = red[w][e];
-                    sx += v.x;
-                    sy += v.y;
-                    sz += v.z;
+                        for (int w = 0; w < kWaves - 1; ++w) {
+                            ax += red[w][0][lane];
+                            ay += red[w][1][lane];
+                            az += red[w][2][lane];
+                        }
+                    }
+                    __syncthreads();
                 }
-                out[e] = sx;
-                out[kTile + e] = sy;
-                out[2 * kTile + e] = sz;
+                if (wave == 0) {
+                    out[0] = ax;
+                    out[kTile] = ay;
+                    out[2 * kTile] = az;
+                }
             }
         }
         fold_tile<IPT>(px, py, pz, hx, hy, hz, lx, ly, lz);

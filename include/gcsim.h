@@ -221,7 +221,9 @@ gc_status gc_fmm_last_stats(gc_ctx* ctx, gc_fmm_stats* out);
 gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms_per_copy);
 
 /* Tuning: mixed-kernel launch shape (targets per thread 1/2/4, j-splits).
- * 0 = automatic. */
+ * 0 = automatic.  gc_get_launch_shape reports the shape a full-range force
+ * of the resident particles uses: for the symmetric form its targets per
+ * lane and jsplit 0 (it has no source splits). */
 gc_status gc_set_launch_shape(gc_ctx* ctx, int targets_per_thread, int jsplit);
 gc_status gc_get_launch_shape(gc_ctx* ctx, int* targets_per_thread, int* jsplit);
 
