@@ -274,6 +274,8 @@ def main() -> int:
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 GPU comparison")
+    ap.add_argument("--pairs", default="auto", choices=["auto", "targets", "symmetric"],
+                    help="mixed force form: per target, or each unordered pair once")
     ap.add_argument("--workload", default="direct",
                     choices=["direct", "fmm", "batch", "adjoint", "drift", "copy"],
                     help="direct = the headline line; the others print their own records")
@@ -301,6 +303,8 @@ def main() -> int:
     pairs = float(n) * float(n - 1)
 
     eng = gcsim.Engine("hip-direct", args.precision, softening=eps)
+    if args.precision == "mixed":
+        eng.set_pairs(args.pairs)
     eng.upload(pos, vel, mass)
     eng.compute_accelerations()
     _, _, a_step0 = eng.download()
