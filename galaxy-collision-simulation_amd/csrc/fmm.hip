@@ -39,7 +39,6 @@ namespace {
 constexpr int kLevels = 21;       // 3 key bits per level, 63 bits
 constexpr int kMom = 35;          // raw Cartesian moments, total degree <= 4
 constexpr int kN32 = 36;          // FP32 normalised moments per node (+pad)
-constexpr int kStack = 8 * (kLevels + 2);
 constexpr int kEvalLanes = 64;
 
 __device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -352,8 +351,20 @@ __device__ __forceinline__ bool well_separated(const double* __restrict__ geom, 
     return w / dist < theta;
 }
 
-// FILL == false: count far cells, near leaves and near source particles
-// per target leaf; FILL == true: write the lists at the scanned offsets.
+// Interaction lists, one wave per target leaf.  The wave walks the tree
+// breadth first: each round tests up to 64 frontier nodes at once with the
+// reference's rules (fmm_solver.cpp:243-277) — a node is skipped if
+// massless, a far cell if it is not the target and
+// 2 (h_t + h_s) / |c_s - c_t| < theta, a near leaf if it has no children,
+// and otherwise opened, its children joining the next frontier in child
+// order.  Far cells and near leaves are appended in lane order (ballot
+// prefix sums), so the lists are deterministic; they hold the reference's
+// cells, in breadth-first rather than depth-first order.  The grid is
+// persistent: wave w takes leaves w, w + G, ...; its two frontier buffers of
+// `fcap` nodes live in `scratch`.  A frontier that would outgrow `fcap` sets
+// `overflow` and the host repeats the pass with larger buffers.
+// FILL == false: count far cells, near leaves and near source particles per
+// leaf; FILL == true: write the lists at the scanned offsets.
 template <bool FILL>
 __global__ __launch_bounds__(64) void lists_kernel(
     int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
@@ -361,36 +372,84 @@ __global__ __launch_bounds__(64) void lists_kernel(
     const int* __restrict__ nchild, const int* __restrict__ nb, const int* __restrict__ ne,
     double theta, int* __restrict__ nfar, int* __restrict__ nnear, int* __restrict__ nsrc,
     const long long* __restrict__ far_off, const long long* __restrict__ near_off,
-    int* __restrict__ far, int* __restrict__ near) {
-    const int li = blockIdx.x * 64 + threadIdx.x;
-    if (li >= nleaves) return;
-    const int t = leaves[li];
-    int stack[kStack];
-    int sp = 0;
-    stack[sp++] = 0;
-    int cf = 0, cn = 0, cs = 0;
-    long long fo = FILL ? far_off[li] : 0, no = FILL ? near_off[li] : 0;
-    while (sp > 0) {
-        const int s = stack[--sp];
-        if (mcom[4 * s] <= 0.0) continue;
-        if (s != t && well_separated(geom, t, s, theta)) {
-            if (FILL) far[fo + cf] = s;
-            ++cf;
+    int* __restrict__ far, int* __restrict__ near, int* __restrict__ scratch, int fcap,
+    int* __restrict__ overflow) {
+    const int lane = threadIdx.x;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int* bufa = scratch + static_cast<size_t>(blockIdx.x) * 2 * fcap;
+    int* bufb = bufa + fcap;
+    for (int li = blockIdx.x; li < nleaves; li += gridDim.x) {
+        const int t = leaves[li];
+        int* cur = bufa;
+        int* nxt = bufb;
+        if (lane == 0) cur[0] = 0;
+        __syncthreads();  // one wave: orders the frontier writes before the reads
+        int ncur = 1;
+        long long cf = 0, cn = 0, cs = 0;
+        const long long fo = FILL ? far_off[li] : 0, no = FILL ? near_off[li] : 0;
+        bool ovf = false;
+        while (ncur > 0 && !ovf) {
+            int nnext = 0;
+            for (int base = 0; base < ncur; base += 64) {
+                const int j = base + lane;
+                int node = -1, kind = 3, nc = 0;
+                if (j < ncur) {
+                    node = cur[j];
+                    if (mcom[4 * node] > 0.0) {
+                        if (node != t && well_separated(geom, t, node, theta)) {
+                            kind = 0;
+                        } else {
+                            nc = nchild[node];
+                            kind = nc == 0 ? 1 : 2;
+                        }
+                    }
+                }
+                const unsigned long long mfar = __ballot(kind == 0);
+                const unsigned long long mnear = __ballot(kind == 1);
+                if (FILL) {
+                    if (kind == 0) far[fo + cf + __popcll(mfar & below)] = node;
+                    if (kind == 1) near[no + cn + __popcll(mnear & below)] = node;
+                }
+                cf += __popcll(mfar);
+                cn += __popcll(mnear);
+                if (!FILL) {
+                    int src = kind == 1 ? ne[node] - nb[node] : 0;
+                    for (int o = 32; o > 0; o >>= 1) src += __shfl_xor(src, o, 64);
+                    cs += src;
+                }
+                // children of opened nodes join the next frontier in lane order
+                int open = kind == 2 ? nc : 0, excl = open;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int v = __shfl_up(excl, o, 64);
+                    if (lane >= o) excl += v;
+                }
+                const int total = __shfl(excl, 63, 64);
+                excl -= open;
+                if (nnext + total > fcap) {
+                    ovf = true;
+                    break;
+                }
+                if (kind == 2) {
+                    const int fc = first_child[node];
+                    for (int c = 0; c < nc; ++c) nxt[nnext + excl + c] = fc + c;
+                }
+                nnext += total;
+            }
+            __syncthreads();
+            int* tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+            ncur = nnext;
+        }
+        if (ovf) {
+            if (lane == 0) atomicOr(overflow, 1);
             continue;
         }
-        const int nc = nchild[s];
-        if (nc == 0) {
-            if (FILL) near[no + cn] = s;
-            ++cn;
-            cs += ne[s] - nb[s];
-            continue;
+        if (!FILL && lane == 0) {
+            nfar[li] = static_cast<int>(cf);
+            nnear[li] = static_cast<int>(cn);
+            nsrc[li] = static_cast<int>(cs);
         }
-        for (int c = first_child[s] + nc - 1; c >= first_child[s]; --c) stack[sp++] = c;
-    }
-    if (!FILL) {
-        nfar[li] = cf;
-        nnear[li] = cn;
-        nsrc[li] = cs;
     }
 }
 
@@ -717,6 +776,10 @@ struct FmmEngine::Impl {
     int *flag = nullptr, *flag_pos = nullptr, *leaves = nullptr;
     int *nfar = nullptr, *nnear = nullptr, *nsrc = nullptr;
     long long *far_off = nullptr, *near_off = nullptr;
+    // list traversal frontiers: 2 * fcap nodes per persistent wave
+    int* frontier = nullptr;
+    size_t frontier_ints = 0;
+    int fcap = 4096;
     size_t farcap = 0, nearcap = 0;
     int *far = nullptr, *near = nullptr;
     unsigned long long* stats_dev = nullptr;
@@ -729,9 +792,9 @@ struct FmmEngine::Impl {
 
     ~Impl() {
         void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sort_tmp, bbox_part, geom, mcom, mom,
-                        n32, nb, ne, first_child, nchild, bounds, count, offset, scan_tmp,
-                        overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off, near_off,
-                        far, near, stats_dev};
+                        n32, nb, ne, first_child, nchild, bounds, count, offset,
+                        scan_tmp, overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
+                        near_off, frontier, far, near, stats_dev};
         for (void* p : ptrs) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -913,14 +976,34 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         return GC_OK;
     }
 
-    // 6: lists (count, scan, fill)
-    const unsigned lblk = static_cast<unsigned>((nleaves + 63) / 64);
-    hipLaunchKernelGGL((lists_kernel<false>), dim3(lblk), dim3(64), 0, st, nleaves, leaves, geom,
-                       mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
-                       near_off, far, near);
+    // 6: lists (count, scan, fill); persistent waves, larger frontiers on
+    // overflow
+    const int lwaves = std::min(nleaves, 4096);
+    for (;;) {
+        const size_t need = static_cast<size_t>(lwaves) * 2 * fcap;
+        if (need > frontier_ints) {
+            (void)hipFree(frontier);
+            frontier = nullptr;
+            frontier_ints = 0;
+            FMM_HIP(hipMalloc(&frontier, sizeof(int) * need), "allocate list frontiers");
+            frontier_ints = need;
+        }
+        FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
+        hipLaunchKernelGGL((lists_kernel<false>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves,
+                           geom, mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc,
+                           far_off, near_off, far, near, frontier, fcap, overflow);
+        int lovf = 0;
+        FMM_HIP(hipMemcpyAsync(&lovf, overflow, sizeof(int), hipMemcpyDeviceToHost, st), "read flag");
+        FMM_HIP(hipStreamSynchronize(st), "list counts");
+        if (!lovf) break;
+        if (fcap >= (1 << 24)) {
+            if (err) *err = "interaction-list frontier beyond 16M nodes";
+            return GC_ERR_UNSUPPORTED;
+        }
+        fcap *= 4;
+    }
     // 64-bit offsets: widen the counts with a scan over long long
     {
-        // far_off/near_off hold exclusive sums of nfar/nnear
         auto widen = [&](const int* in, long long* out) -> hipError_t {
             size_t need = 0;
             hipcub::TransformInputIterator<long long, ToI64, const int*> it(in, ToI64());
@@ -956,9 +1039,9 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         nearcap = static_cast<size_t>(total_near) + 1 + static_cast<size_t>(total_near) / 4;
         FMM_HIP(hipMalloc(&near, sizeof(int) * nearcap), "allocate near lists");
     }
-    hipLaunchKernelGGL((lists_kernel<true>), dim3(lblk), dim3(64), 0, st, nleaves, leaves, geom,
+    hipLaunchKernelGGL((lists_kernel<true>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves, geom,
                        mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
-                       near_off, far, near);
+                       near_off, far, near, frontier, fcap, overflow);
 
     // statistics (FmmStats, fmm_solver.hpp:20-25, plus pair counts)
     FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
