@@ -479,187 +479,258 @@ struct Src {
     int id;
 };
 
-template <typename F, int ORDER, bool SAFE>
-__global__ __launch_bounds__(kEvalLanes) void eval_kernel(
-    int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
-    const double* __restrict__ mcom, const float* __restrict__ n32, const double* __restrict__ mom,
-    const int* __restrict__ nb, const int* __restrict__ ne, const int* __restrict__ sidx,
-    const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
-    const long long* __restrict__ far_off, const int* __restrict__ nfar,
-    const int* __restrict__ far, const long long* __restrict__ near_off,
-    const int* __restrict__ nnear, const int* __restrict__ near, double G, double eps, int begin,
-    int end, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, int close_kick,
-    double half_dt) {
-#pragma clang fp contract(off)
-    __shared__ Src<F> tile[kEvalLanes];
-    const int leaf = leaves[blockIdx.x];
-    const int lane = threadIdx.x;
-    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
-    const F eps2 = static_cast<F>(eps * eps);
-    const int tb = nb[leaf], te = ne[leaf];
-    const long long fo = far_off[blockIdx.x], no = near_off[blockIdx.x];
-    const int nf = nfar[blockIdx.x], nn = nnear[blockIdx.x];
-    for (int base = tb; base < te; base += kEvalLanes) {
-        const int s = base + lane;
-        const bool active = s < te;
-        const int me = active ? sidx[s] : -1;
-        const F tx = active ? static_cast<F>(x[me] - cx) : F(0);
-        const F ty = active ? static_cast<F>(y[me] - cy) : F(0);
-        const F tz = active ? static_cast<F>(z[me] - cz) : F(0);
-        double far_a[3] = {0.0, 0.0, 0.0}, near_a[3] = {0.0, 0.0, 0.0};
-        for (int f = 0; f < nf; ++f) {
-            const int cell = far[fo + f];
-            const F yx = static_cast<F>(mcom[4 * cell + 1] - cx) - tx;
-            const F yy = static_cast<F>(mcom[4 * cell + 2] - cy) - ty;
-            const F yz = static_cast<F>(mcom[4 * cell + 3] - cz) - tz;
-            F N[kMom];
-            load_moments<F>(cell, n32, mom, mcom, N);
-            F fx = 0, fy = 0, fz = 0;
-            if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-            else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-            else m2p_order4<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-            far_a[0] += static_cast<double>(fx);
-            far_a[1] += static_cast<double>(fy);
-            far_a[2] += static_cast<double>(fz);
-        }
-        for (int q = 0; q < nn; ++q) {
-            const int src_leaf = near[no + q];
-            const int sb = nb[src_leaf], se = ne[src_leaf];
-            const bool self_leaf = src_leaf == leaf;
-            for (int cb = sb; cb < se; cb += kEvalLanes) {
-                __syncthreads();
-                const int j = cb + lane;
-                if (j < se) {
-                    const int id = sidx[j];
-                    Src<F> t;
-                    t.x = static_cast<F>(x[id] - cx);
-                    t.y = static_cast<F>(y[id] - cy);
-                    t.z = static_cast<F>(z[id] - cz);
-                    t.gm = static_cast<F>(G * m[id]);
-                    t.id = id;
-                    tile[lane] = t;
-                }
-                __syncthreads();
-                const int cnt = min(kEvalLanes, se - cb);
-                F px = 0, py = 0, pz = 0;
-                for (int jj = 0; jj < cnt; ++jj) {
-                    const Src<F> sj = tile[jj];
-                    if (self_leaf && sj.id == me) continue;
-                    const F dx = sj.x - tx, dy = sj.y - ty, dz = sj.z - tz;
-                    const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
-                    F r;
-                    if (SAFE) {
-                        r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
-                    } else {
-                        r = gc_rsq(s2);
-                    }
-                    const F k = (sj.gm * r) * (r * r);
-                    px = gc_fma(dx, k, px);
-                    py = gc_fma(dy, k, py);
-                    pz = gc_fma(dz, k, pz);
-                }
-                near_a[0] += static_cast<double>(px);
-                near_a[1] += static_cast<double>(py);
-                near_a[2] += static_cast<double>(pz);
-            }
-        }
-        if (!active || me < begin || me >= end) continue;
-        const double a0 = G * far_a[0] + near_a[0];
-        const double a1 = G * far_a[1] + near_a[1];
-        const double a2 = G * far_a[2] + near_a[2];
-        ax[me] = a0;
-        ay[me] = a1;
-        az[me] = a2;
-        if (close_kick) {
-            vx[me] = vx[me] + a0 * half_dt;
-            vy[me] = vy[me] + a1 * half_dt;
-            vz[me] = vz[me] + a2 * half_dt;
-        }
-    }
-}
+// Far cells staged per chunk: COM offset from the target leaf's centre and
+// the normalised moments.
+constexpr int kFarChunk = 64;
+constexpr int kNearPiece = 64;
 
-// Grouped evaluation: a leaf chunk of c <= 64 targets is served by
-// G = floor(64 / c) lane groups; group g takes far cells f = g, g + G, ...
-// and staged near sources jj = g, g + G, ...; the per-lane partials are then
-// summed over groups in group order through LDS (deterministic).  Keeps
-// ~c*G/64 of the lanes busy instead of c/64.
-template <typename F, int ORDER, bool SAFE>
-__global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
-    int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
+template <typename F>
+struct FarStage {
+    F c[kFarChunk][4];
+    F N[kFarChunk][kN32];
+};
+
+// Evaluation, in two kernels per target leaf (one 64-lane wave each),
+// particles in key order (sorted FP64 copies sx..sm).  A leaf chunk of
+// c <= 64 targets is served by G = floor(64 / c) lane groups; per lane the
+// FP32 (or FP64) terms of a chunk are added into FP64 sums, and the groups'
+// sums are added in group order through LDS (deterministic).
+//  far  (eval_far_kernel): chunks of 64 far cells are staged in LDS by one
+//       cooperative load (cell indices, COM offsets, moments); group g
+//       evaluates cells g, g + G, ... of the chunk by M2P; writes G * sum.
+//  near (eval_near_kernel): the near leaves' particles, 64 leaves at a time
+//       laid end to end, are staged kNearStage sources per round, the next
+//       round's sources loaded into registers while the current one is
+//       evaluated (each lane finds its sources' leaves by binary search over
+//       the leaves' prefix sizes); group g takes sources g, g + G, ...; the
+//       self pair is skipped (fmm_solver.cpp:295-310); adds its sum to the
+//       far kernel's and applies the closing kick.
+// Splitting keeps the register-heavy M2P out of the P2P kernel, which then
+// runs at full occupancy.
+template <typename F, int ORDER>
+__global__ __launch_bounds__(kEvalLanes) void eval_far_kernel(
+    const int* __restrict__ leaves, const double* __restrict__ geom,
     const double* __restrict__ mcom, const float* __restrict__ n32, const double* __restrict__ mom,
     const int* __restrict__ nb, const int* __restrict__ ne, const int* __restrict__ sidx,
-    const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const long long* __restrict__ far_off, const int* __restrict__ nfar,
-    const int* __restrict__ far, const long long* __restrict__ near_off,
-    const int* __restrict__ nnear, const int* __restrict__ near, double G, double eps, int begin,
-    int end, double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, int close_kick,
-    double half_dt) {
+    const int* __restrict__ far, double G, double eps, int begin, int end,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
 #pragma clang fp contract(off)
-    __shared__ Src<F> tile[kEvalLanes];
+    __shared__ FarStage<F> fs;
     __shared__ double red[3][kEvalLanes];
     const int leaf = leaves[blockIdx.x];
     const int lane = threadIdx.x;
     const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
     const F eps2 = static_cast<F>(eps * eps);
     const int tb = nb[leaf], te = ne[leaf];
-    const long long fo = far_off[blockIdx.x], no = near_off[blockIdx.x];
-    const int nf = nfar[blockIdx.x], nn = nnear[blockIdx.x];
+    const long long fo = far_off[blockIdx.x];
+    const int nf = nfar[blockIdx.x];
     for (int base = tb; base < te; base += kEvalLanes) {
         const int c = min(kEvalLanes, te - base);
         const int groups = kEvalLanes / c;
         const int g = lane / c;
         const int tl = lane - g * c;
         const bool active = g < groups;
-        const int me = sidx[base + tl];
-        const F tx = static_cast<F>(x[me] - cx);
-        const F ty = static_cast<F>(y[me] - cy);
-        const F tz = static_cast<F>(z[me] - cz);
-        double far_a[3] = {0.0, 0.0, 0.0}, near_a[3] = {0.0, 0.0, 0.0};
-        if (active) {
-            for (int f = g; f < nf; f += groups) {
-                const int cell = far[fo + f];
-                const F yx = static_cast<F>(mcom[4 * cell + 1] - cx) - tx;
-                const F yy = static_cast<F>(mcom[4 * cell + 2] - cy) - ty;
-                const F yz = static_cast<F>(mcom[4 * cell + 3] - cz) - tz;
+        const int mine = base + tl;  // sorted position of this lane's target
+        const F tx = static_cast<F>(sx[mine] - cx);
+        const F ty = static_cast<F>(sy[mine] - cy);
+        const F tz = static_cast<F>(sz[mine] - cz);
+        double far_a[3] = {0.0, 0.0, 0.0};
+        for (int f0 = 0; f0 < nf; f0 += kFarChunk) {
+            const int k = min(kFarChunk, nf - f0);
+            __syncthreads();
+            if (lane < k) {
+                const int cell = far[fo + f0 + lane];
+                fs.c[lane][0] = static_cast<F>(mcom[4 * cell + 1] - cx);
+                fs.c[lane][1] = static_cast<F>(mcom[4 * cell + 2] - cy);
+                fs.c[lane][2] = static_cast<F>(mcom[4 * cell + 3] - cz);
                 F N[kMom];
                 load_moments<F>(cell, n32, mom, mcom, N);
+#pragma unroll
+                for (int a = 0; a < kMom; ++a) fs.N[lane][a] = N[a];
+            }
+            __syncthreads();
+            if (!active) continue;
+            F sfx = 0, sfy = 0, sfz = 0;
+            for (int f = g; f < k; f += groups) {
+                const F yx = fs.c[f][0] - tx, yy = fs.c[f][1] - ty, yz = fs.c[f][2] - tz;
+                F N[kMom];
+#pragma unroll
+                for (int a = 0; a < kMom; ++a) N[a] = fs.N[f][a];
                 F fx = 0, fy = 0, fz = 0;
                 if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
                 else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
                 else m2p_order4<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-                far_a[0] += static_cast<double>(fx);
-                far_a[1] += static_cast<double>(fy);
-                far_a[2] += static_cast<double>(fz);
+                sfx += fx;
+                sfy += fy;
+                sfz += fz;
             }
+            far_a[0] += static_cast<double>(sfx);
+            far_a[1] += static_cast<double>(sfy);
+            far_a[2] += static_cast<double>(sfz);
         }
-        for (int q = 0; q < nn; ++q) {
-            const int src_leaf = near[no + q];
-            const int sb = nb[src_leaf], se = ne[src_leaf];
-            const bool self_leaf = src_leaf == leaf;
-            for (int cb = sb; cb < se; cb += kEvalLanes) {
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) red[d][lane] = active ? far_a[d] : 0.0;
+        __syncthreads();
+        if (g != 0) continue;
+        double a[3];
+        for (int d = 0; d < 3; ++d) {
+            double acc = red[d][tl];
+            for (int k = 1; k < groups; ++k) acc += red[d][tl + k * c];
+            a[d] = G * acc;
+        }
+        const int me = sidx[mine];
+        if (me < begin || me >= end) continue;
+        ax[me] = a[0];
+        ay[me] = a[1];
+        az[me] = a[2];
+    }
+}
+
+constexpr int kNearStage = 256;     // sources staged per round
+constexpr int kNearPer = kNearStage / kEvalLanes;
+constexpr int kNearItem = 256;      // near leaves per work item
+
+// Near-field work items: item k of a leaf covers its near leaves
+// [k kNearItem, (k + 1) kNearItem); each item writes FP64 partial sums for
+// the leaf's targets at `slot` (rows of 3).  A leaf's items are consecutive.
+struct NearItem {
+    int leaf;   // index into the leaf list
+    int q0, q1;
+    int slot;   // first partial row (one row per target of the leaf)
+};
+
+__global__ __launch_bounds__(256) void near_item_count_kernel(int nleaves,
+                                                              const int* __restrict__ nnear,
+                                                              int* __restrict__ icount) {
+    const int li = blockIdx.x * 256 + threadIdx.x;
+    if (li >= nleaves) return;
+    icount[li] = max(1, (nnear[li] + kNearItem - 1) / kNearItem);
+}
+
+// Items of each leaf at the leaf's scanned item offset; partial rows at the
+// scanned offset of (items x targets).
+__global__ __launch_bounds__(256) void near_item_fill_kernel(
+    int nleaves, const int* __restrict__ leaves, const int* __restrict__ nb,
+    const int* __restrict__ ne, const int* __restrict__ nnear, const int* __restrict__ icount,
+    const int* __restrict__ ioff, const long long* __restrict__ row_off,
+    NearItem* __restrict__ items) {
+    const int li = blockIdx.x * 256 + threadIdx.x;
+    if (li >= nleaves) return;
+    const int cnt = ne[leaves[li]] - nb[leaves[li]];
+    const int k = icount[li];
+    const long long r0 = row_off[li];
+    for (int j = 0; j < k; ++j) {
+        NearItem it;
+        it.leaf = li;
+        it.q0 = j * kNearItem;
+        it.q1 = min(nnear[li], (j + 1) * kNearItem);
+        it.slot = static_cast<int>(r0 + static_cast<long long>(j) * cnt);
+        items[ioff[li] + j] = it;
+    }
+}
+
+template <typename F, bool SAFE>
+__global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
+    const NearItem* __restrict__ items, const int* __restrict__ leaves,
+    const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const double* __restrict__ sm, const long long* __restrict__ near_off,
+    const int* __restrict__ near, double G, double eps, double* __restrict__ part) {
+#pragma clang fp contract(off)
+    __shared__ Src<F> tile[kNearStage];
+    __shared__ int qs[kNearPiece], qpre[kNearPiece + 1];
+    __shared__ double red[3][kEvalLanes];
+    const NearItem item = items[blockIdx.x];
+    const int leaf = leaves[item.leaf];
+    const int lane = threadIdx.x;
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    const F eps2 = static_cast<F>(eps * eps);
+    const int tb = nb[leaf], te = ne[leaf];
+    const long long no = near_off[item.leaf];
+    // source j of the current piece: its sorted index (-1 past the end)
+    auto locate = [&](int j, int total, int kq) -> int {
+        if (j >= total) return -1;
+        int lo = 0, hi = kq - 1;  // last q with qpre[q] <= j
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (qpre[mid] <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        return qs[lo] + (j - qpre[lo]);
+    };
+    for (int base = tb; base < te; base += kEvalLanes) {
+        const int c = min(kEvalLanes, te - base);
+        const int groups = kEvalLanes / c;
+        const int g = lane / c;
+        const int tl = lane - g * c;
+        const bool active = g < groups;
+        const int mine = base + tl;
+        const F tx = static_cast<F>(sx[mine] - cx);
+        const F ty = static_cast<F>(sy[mine] - cy);
+        const F tz = static_cast<F>(sz[mine] - cz);
+        double near_a[3] = {0.0, 0.0, 0.0};
+        for (int q0 = item.q0; q0 < item.q1; q0 += kNearPiece) {
+            const int kq = min(kNearPiece, item.q1 - q0);
+            __syncthreads();
+            int len = 0;
+            if (lane < kq) {
+                const int sl = near[no + q0 + lane];
+                qs[lane] = nb[sl];
+                len = ne[sl] - nb[sl];
+            }
+            int incl = len;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += v;
+            }
+            qpre[lane + 1] = incl;
+            if (lane == 0) qpre[0] = 0;
+            const int total = __shfl(incl, 63, 64);
+            __syncthreads();
+            // registers hold the next round's sources
+            double nxv[kNearPer][4];
+            int nid[kNearPer];
+#pragma unroll
+            for (int u = 0; u < kNearPer; ++u) {
+                nid[u] = locate(lane + u * kEvalLanes, total, kq);
+                const int sj = nid[u] < 0 ? mine : nid[u];
+                nxv[u][0] = sx[sj];
+                nxv[u][1] = sy[sj];
+                nxv[u][2] = sz[sj];
+                nxv[u][3] = sm[sj];
+            }
+            for (int cb = 0; cb < total; cb += kNearStage) {
                 __syncthreads();
-                const int j = cb + lane;
-                if (j < se) {
-                    const int id = sidx[j];
+#pragma unroll
+                for (int u = 0; u < kNearPer; ++u) {
                     Src<F> t;
-                    t.x = static_cast<F>(x[id] - cx);
-                    t.y = static_cast<F>(y[id] - cy);
-                    t.z = static_cast<F>(z[id] - cz);
-                    t.gm = static_cast<F>(G * m[id]);
-                    t.id = id;
-                    tile[lane] = t;
+                    t.x = static_cast<F>(nxv[u][0] - cx);
+                    t.y = static_cast<F>(nxv[u][1] - cy);
+                    t.z = static_cast<F>(nxv[u][2] - cz);
+                    t.gm = nid[u] < 0 ? F(0) : static_cast<F>(G * nxv[u][3]);
+                    t.id = nid[u];
+                    tile[lane + u * kEvalLanes] = t;
                 }
                 __syncthreads();
+                if (cb + kNearStage < total) {
+#pragma unroll
+                    for (int u = 0; u < kNearPer; ++u) {
+                        nid[u] = locate(cb + kNearStage + lane + u * kEvalLanes, total, kq);
+                        const int sj = nid[u] < 0 ? mine : nid[u];
+                        nxv[u][0] = sx[sj];
+                        nxv[u][1] = sy[sj];
+                        nxv[u][2] = sz[sj];
+                        nxv[u][3] = sm[sj];
+                    }
+                }
                 if (!active) continue;
-                const int cnt = min(kEvalLanes, se - cb);
+                const int cnt = min(kNearStage, total - cb);
                 F px = 0, py = 0, pz = 0;
                 for (int jj = g; jj < cnt; jj += groups) {
                     const Src<F> sj = tile[jj];
-                    if (self_leaf && sj.id == me) continue;
+                    if (sj.id == mine) continue;
                     const F dx = sj.x - tx, dy = sj.y - ty, dz = sj.z - tz;
                     const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
                     F r;
@@ -668,10 +739,10 @@ __global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
                     } else {
                         r = gc_rsq(s2);
                     }
-                    const F k = (sj.gm * r) * (r * r);
-                    px = gc_fma(dx, k, px);
-                    py = gc_fma(dy, k, py);
-                    pz = gc_fma(dz, k, pz);
+                    const F kk = (sj.gm * r) * (r * r);
+                    px = gc_fma(dx, kk, px);
+                    py = gc_fma(dy, kk, py);
+                    pz = gc_fma(dz, kk, pz);
                 }
                 near_a[0] += static_cast<double>(px);
                 near_a[1] += static_cast<double>(py);
@@ -679,16 +750,45 @@ __global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
             }
         }
         __syncthreads();
-        for (int d = 0; d < 3; ++d) red[d][lane] = active ? G * far_a[d] + near_a[d] : 0.0;
+        for (int d = 0; d < 3; ++d) red[d][lane] = active ? near_a[d] : 0.0;
         __syncthreads();
         if (g != 0) continue;
-        double a[3];
+        double* row = part + 3 * (static_cast<size_t>(item.slot) + (base - tb) + tl);
         for (int d = 0; d < 3; ++d) {
-            double s = red[d][tl];
-            for (int k = 1; k < groups; ++k) s += red[d][tl + k * c];
-            a[d] = s;
+            double acc = red[d][tl];
+            for (int k = 1; k < groups; ++k) acc += red[d][tl + k * c];
+            row[d] = acc;
         }
+    }
+}
+
+// a = far (already in ax) + the leaf's near item partials in item order;
+// optional closing half kick.  One thread per target.
+__global__ __launch_bounds__(kEvalLanes) void near_merge_kernel(
+    const int* __restrict__ leaves, const int* __restrict__ nb, const int* __restrict__ ne,
+    const int* __restrict__ sidx, const int* __restrict__ icount, const int* __restrict__ ioff,
+    const NearItem* __restrict__ items, const double* __restrict__ part, int begin, int end,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, int close_kick,
+    double half_dt) {
+#pragma clang fp contract(off)
+    const int li = blockIdx.x;
+    const int leaf = leaves[li];
+    const int tb = nb[leaf], te = ne[leaf];
+    const int k = icount[li], i0 = ioff[li];
+    for (int t = tb + static_cast<int>(threadIdx.x); t < te; t += kEvalLanes) {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < k; ++j) {
+            const double* row = part + 3 * (static_cast<size_t>(items[i0 + j].slot) + (t - tb));
+            a[0] += row[0];
+            a[1] += row[1];
+            a[2] += row[2];
+        }
+        const int me = sidx[t];
         if (me < begin || me >= end) continue;
+        a[0] = ax[me] + a[0];
+        a[1] = ay[me] + a[1];
+        a[2] = az[me] + a[2];
         ax[me] = a[0];
         ay[me] = a[1];
         az[me] = a[2];
@@ -699,6 +799,21 @@ __global__ __launch_bounds__(kEvalLanes) void eval_grouped_kernel(
         }
     }
 }
+
+// Particle state in key order, for coalesced source and target reads.
+__global__ __launch_bounds__(256) void gather_sorted_kernel(
+    int n, const int* __restrict__ sidx, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, double* __restrict__ sx,
+    double* __restrict__ sy, double* __restrict__ sz, double* __restrict__ sm) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int i = sidx[j];
+    sx[j] = x[i];
+    sy[j] = y[i];
+    sz[j] = z[i];
+    sm[j] = m[i];
+}
+
 
 // zero accelerations of [begin, end) (particles in massless leaves keep 0,
 // as in fmm_solver.cpp:31 + :210-226)
@@ -729,6 +844,15 @@ struct ToI64 {
     __host__ __device__ long long operator()(int v) const { return static_cast<long long>(v); }
 };
 
+// Partial rows a leaf's near items need: items x targets.
+struct RowsPerLeaf {
+    const int *leaves, *nb, *ne, *icount;
+    __host__ __device__ long long operator()(int li) const {
+        const int l = leaves[li];
+        return static_cast<long long>(icount[li]) * (ne[l] - nb[l]);
+    }
+};
+
 template <typename T>
 hipError_t grow(T** p, size_t* cap, size_t need, size_t keep, hipStream_t st) {
     if (need <= *cap) return hipSuccess;
@@ -755,6 +879,7 @@ struct FmmEngine::Impl {
     size_t pcap = 0;
     unsigned long long *keys = nullptr, *keys_sorted = nullptr;
     int *idx = nullptr, *idx_sorted = nullptr;
+    double *sxs = nullptr, *sys = nullptr, *szs = nullptr, *sms = nullptr;  // key order
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     double* bbox_part = nullptr;
@@ -782,6 +907,13 @@ struct FmmEngine::Impl {
     int fcap = 4096;
     size_t farcap = 0, nearcap = 0;
     int *far = nullptr, *near = nullptr;
+    // near-field work items and their partial sums
+    int *icount = nullptr, *ioff = nullptr;
+    long long* row_off = nullptr;
+    size_t icap = 0, itemcap = 0, partcap = 0;
+    NearItem* items = nullptr;
+    double* npart = nullptr;
+    int nitems = 0;
     unsigned long long* stats_dev = nullptr;
     // host-side results of the last build
     std::vector<int> level_begin;
@@ -791,10 +923,10 @@ struct FmmEngine::Impl {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
     ~Impl() {
-        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sort_tmp, bbox_part, geom, mcom, mom,
+        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
                         n32, nb, ne, first_child, nchild, bounds, count, offset,
                         scan_tmp, overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
-                        near_off, frontier, far, near, stats_dev};
+                        near_off, frontier, far, near, icount, ioff, row_off, items, npart, stats_dev};
         for (void* p : ptrs) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -841,9 +973,13 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
 
     // particle buffers
     if (static_cast<size_t>(n) > pcap) {
-        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sort_tmp, bbox_part};
+        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part};
         for (void* q : ptrs) (void)hipFree(q);
         pcap = static_cast<size_t>(n);
+        FMM_HIP(hipMalloc(&sxs, sizeof(double) * pcap), "allocate sorted state");
+        FMM_HIP(hipMalloc(&sys, sizeof(double) * pcap), "allocate sorted state");
+        FMM_HIP(hipMalloc(&szs, sizeof(double) * pcap), "allocate sorted state");
+        FMM_HIP(hipMalloc(&sms, sizeof(double) * pcap), "allocate sorted state");
         FMM_HIP(hipMalloc(&keys, sizeof(unsigned long long) * pcap), "allocate keys");
         FMM_HIP(hipMalloc(&keys_sorted, sizeof(unsigned long long) * pcap), "allocate keys");
         FMM_HIP(hipMalloc(&idx, sizeof(int) * pcap), "allocate indices");
@@ -871,6 +1007,8 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
                                                idx_sorted, n, 0, 3 * kLevels, st),
             "radix sort keys");
     FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, idx_sorted,
+                       s.x, s.y, s.z, s.m, sxs, sys, szs, sms);
 
     // 4: levels
     level_begin.assign(1, 0);
@@ -1043,6 +1181,64 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
                        mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
                        near_off, far, near, frontier, fcap, overflow);
 
+    // near-field work items: leaf li gets max(1, ceil(nnear / kNearItem))
+    // items, each with one partial row per target of the leaf
+    if (static_cast<size_t>(nleaves) > icap) {
+        (void)hipFree(icount);
+        (void)hipFree(ioff);
+        (void)hipFree(row_off);
+        icap = static_cast<size_t>(nleaves) * 2;
+        FMM_HIP(hipMalloc(&icount, sizeof(int) * icap), "allocate near items");
+        FMM_HIP(hipMalloc(&ioff, sizeof(int) * icap), "allocate near items");
+        FMM_HIP(hipMalloc(&row_off, sizeof(long long) * icap), "allocate near items");
+    }
+    hipLaunchKernelGGL(near_item_count_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st,
+                       nleaves, nnear, icount);
+    {
+        size_t need = 0;
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, icount, ioff, nleaves, st), "size scan");
+        RowsPerLeaf rows_fn{leaves, nb, ne, icount};
+        hipcub::CountingInputIterator<int> ci(0);
+        hipcub::TransformInputIterator<long long, RowsPerLeaf, hipcub::CountingInputIterator<int>> rit(ci, rows_fn);
+        size_t need2 = 0;
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, rit, row_off, nleaves, st), "size scan");
+        need = std::max(need, need2);
+        if (need > scan_tmp_bytes) {
+            (void)hipFree(scan_tmp);
+            scan_tmp_bytes = need;
+            FMM_HIP(hipMalloc(&scan_tmp, scan_tmp_bytes), "allocate scan storage");
+        }
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, icount, ioff, nleaves, st),
+                "scan near items");
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, rit, row_off, nleaves, st),
+                "scan near rows");
+    }
+    int ilast[2] = {0, 0};
+    long long rlast = 0;
+    int leaf_last = 0, nb_last = 0, ne_last = 0;
+    FMM_HIP(hipMemcpyAsync(&ilast[0], ioff + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "item count");
+    FMM_HIP(hipMemcpyAsync(&ilast[1], icount + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "item count");
+    FMM_HIP(hipMemcpyAsync(&rlast, row_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "row count");
+    FMM_HIP(hipMemcpyAsync(&leaf_last, leaves + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "row count");
+    FMM_HIP(hipStreamSynchronize(st), "item count");
+    FMM_HIP(hipMemcpyAsync(&nb_last, nb + leaf_last, sizeof(int), hipMemcpyDeviceToHost, st), "row count");
+    FMM_HIP(hipMemcpyAsync(&ne_last, ne + leaf_last, sizeof(int), hipMemcpyDeviceToHost, st), "row count");
+    FMM_HIP(hipStreamSynchronize(st), "row count");
+    nitems = ilast[0] + ilast[1];
+    const size_t rows = static_cast<size_t>(rlast) + static_cast<size_t>(ilast[1]) * (ne_last - nb_last);
+    if (static_cast<size_t>(nitems) > itemcap) {
+        (void)hipFree(items);
+        itemcap = static_cast<size_t>(nitems) + static_cast<size_t>(nitems) / 4;
+        FMM_HIP(hipMalloc(&items, sizeof(NearItem) * itemcap), "allocate near items");
+    }
+    if (3 * rows > partcap) {
+        (void)hipFree(npart);
+        partcap = 3 * rows + 3 * rows / 4;
+        FMM_HIP(hipMalloc(&npart, sizeof(double) * partcap), "allocate near partials");
+    }
+    hipLaunchKernelGGL(near_item_fill_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves,
+                       leaves, nb, ne, nnear, icount, ioff, row_off, items);
+
     // statistics (FmmStats, fmm_solver.hpp:20-25, plus pair counts)
     FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
     hipLaunchKernelGGL(stats_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves, leaves,
@@ -1072,35 +1268,34 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const bool mixed = p.precision == GC_PRECISION_MIXED;
     const bool safe = !(static_cast<float>(p.softening * p.softening) >= 1e-30f);
     const dim3 grid(static_cast<unsigned>(nleaves)), block(kEvalLanes);
-    // lane-per-target evaluation only when asked for (comparison runs)
-    const char* mode = std::getenv("GCSIM_FMM_EVAL");
-    const bool grouped = !(mode && std::strcmp(mode, "simple") == 0);
-#define GC_EVAL_ARGS                                                                             \
-    grid, block, 0, st, nleaves, leaves, geom, mcom, n32, mom, nb, ne, idx_sorted, s.x, s.y, s.z, s.m, \
-        far_off, nfar, far, near_off, nnear, near, p.gravitational_constant, p.softening,        \
-        static_cast<int>(begin), static_cast<int>(end), s.ax, s.ay, s.az, s.vx, s.vy, s.vz,      \
-        close_kick ? 1 : 0, half_dt
-#define GC_EVAL(F, O, S)                                                   \
-    do {                                                                   \
-        if (grouped) hipLaunchKernelGGL((eval_grouped_kernel<F, O, S>), GC_EVAL_ARGS); \
-        else hipLaunchKernelGGL((eval_kernel<F, O, S>), GC_EVAL_ARGS);     \
-    } while (0)
-#define GC_EVAL_ORDER(F, S)                         \
-    do {                                            \
-        if (order == 0) GC_EVAL(F, 0, S);           \
-        else if (order == 2) GC_EVAL(F, 2, S);      \
-        else GC_EVAL(F, 4, S);                      \
-    } while (0)
+    const int b = static_cast<int>(begin), e = static_cast<int>(end);
+    const double G = p.gravitational_constant, eps = p.softening;
+#define GC_FAR(F, O)                                                                             \
+    hipLaunchKernelGGL((eval_far_kernel<F, O>), grid, block, 0, st, leaves, geom, mcom, n32, mom, \
+                       nb, ne, idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, s.ax,  \
+                       s.ay, s.az)
+#define GC_NEAR(F, S)                                                                            \
+    hipLaunchKernelGGL((eval_near_kernel<F, S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
+                       items, leaves, geom, nb, ne, sxs, sys, szs, sms, near_off, near, G, eps,   \
+                       npart)
     if (mixed) {
-        if (safe) GC_EVAL_ORDER(float, true);
-        else GC_EVAL_ORDER(float, false);
+        if (order == 0) GC_FAR(float, 0);
+        else if (order == 2) GC_FAR(float, 2);
+        else GC_FAR(float, 4);
+        if (safe) GC_NEAR(float, true);
+        else GC_NEAR(float, false);
     } else {
-        if (safe) GC_EVAL_ORDER(double, true);
-        else GC_EVAL_ORDER(double, false);
+        if (order == 0) GC_FAR(double, 0);
+        else if (order == 2) GC_FAR(double, 2);
+        else GC_FAR(double, 4);
+        if (safe) GC_NEAR(double, true);
+        else GC_NEAR(double, false);
     }
-#undef GC_EVAL_ORDER
-#undef GC_EVAL
-#undef GC_EVAL_ARGS
+#undef GC_FAR
+#undef GC_NEAR
+    hipLaunchKernelGGL(near_merge_kernel, grid, block, 0, st, leaves, nb, ne, idx_sorted, icount, ioff,
+                       items, npart, b, e, s.ax, s.ay, s.az, s.vx, s.vy, s.vz, close_kick ? 1 : 0,
+                       half_dt);
     FMM_HIP(hipGetLastError(), "launch FMM evaluation");
     return GC_OK;
 }
