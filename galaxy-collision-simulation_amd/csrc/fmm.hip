@@ -54,60 +54,26 @@ __device__ __forceinline__ double gc_rsq(double s) { return 1.0 / __builtin_sqrt
 #undef GC_RSQ
 
 // ---- multipole tables (graded exponent order of multipole.cpp:17-26) ----
+// 1 / alpha! for the 35 moments in the graded order of multipole.cpp:17-26
+// (N_alpha = M_alpha / alpha! is what M2P contracts with).
 struct Tables {
-    int e[kMom][3];
-    double inv_fact[kMom];  // 1 / alpha!
-    // M2M pairs (alpha, beta <= alpha, 2 <= |alpha|), coefficient
-    // prod_i C(alpha_i, beta_i), shift exponent alpha - beta
-    int npair;
-    int pa[512], pb[512], sx[512], sy[512], sz[512];
-    double coef[512];
+    double inv_fact[kMom];
 };
 
 constexpr Tables make_tables() {
     Tables t{};
     int k = 0;
+    const int fact[5] = {1, 1, 2, 6, 24};
     for (int d = 0; d <= 4; ++d)
         for (int a = d; a >= 0; --a)
             for (int b = d - a; b >= 0; --b) {
-                t.e[k][0] = a;
-                t.e[k][1] = b;
-                t.e[k][2] = d - a - b;
+                t.inv_fact[k] = 1.0 / static_cast<double>(fact[a] * fact[b] * fact[d - a - b]);
                 ++k;
             }
-    int fact[5] = {1, 1, 2, 6, 24};
-    for (int i = 0; i < kMom; ++i)
-        t.inv_fact[i] = 1.0 / static_cast<double>(fact[t.e[i][0]] * fact[t.e[i][1]] * fact[t.e[i][2]]);
-    int binom[5][5] = {{1, 0, 0, 0, 0}, {1, 1, 0, 0, 0}, {1, 2, 1, 0, 0}, {1, 3, 3, 1, 0}, {1, 4, 6, 4, 1}};
-    int n = 0;
-    for (int a = 0; a < kMom; ++a) {
-        const int da = t.e[a][0] + t.e[a][1] + t.e[a][2];
-        if (da < 2) continue;
-        for (int b = 0; b < kMom; ++b) {
-            if (t.e[b][0] > t.e[a][0] || t.e[b][1] > t.e[a][1] || t.e[b][2] > t.e[a][2]) continue;
-            const int db = t.e[b][0] + t.e[b][1] + t.e[b][2];
-            if (db == 1) continue;  // dipole about the COM vanishes
-            t.pa[n] = a;
-            t.pb[n] = b;
-            t.sx[n] = t.e[a][0] - t.e[b][0];
-            t.sy[n] = t.e[a][1] - t.e[b][1];
-            t.sz[n] = t.e[a][2] - t.e[b][2];
-            t.coef[n] = static_cast<double>(binom[t.e[a][0]][t.e[b][0]] * binom[t.e[a][1]][t.e[b][1]] *
-                                            binom[t.e[a][2]][t.e[b][2]]);
-            ++n;
-        }
-    }
-    t.npair = n;
     return t;
 }
 
 __constant__ Tables c_tables = make_tables();
-
-__device__ __forceinline__ double ipow(double v, int k) {
-    double r = 1.0;
-    for (int i = 0; i < k; ++i) r *= v;
-    return r;
-}
 
 // ---- 1. bounding box -----------------------------------------------------
 __global__ __launch_bounds__(256) void bbox_partial_kernel(int n, const double* __restrict__ x,
@@ -259,28 +225,39 @@ __global__ __launch_bounds__(256) void split_emit_kernel(
 }
 
 // ---- 5. upward pass -------------------------------------------------------
+#define GC_UP_INLINE __device__ __forceinline__
+#include "upward_generated.inc"
+#undef GC_UP_INLINE
+
+// One thread per node of a level: mass and centre of mass, then the raw
+// moments about the COM — P2M over the leaf's particles (key order, the
+// sorted copies) or M2M over the children (multipole.cpp:226-261,
+// fmm_solver.cpp:157-208).  Moments stay in registers (generated code).
 __global__ __launch_bounds__(64) void upward_kernel(
     int lb, int le, const int* __restrict__ nb, const int* __restrict__ ne,
     const int* __restrict__ first_child, const int* __restrict__ nchild,
-    const double* __restrict__ geom, const int* __restrict__ sidx, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
-    double* __restrict__ mcom, double* __restrict__ mom, float* __restrict__ n32) {
+    const double* __restrict__ geom, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, const double* __restrict__ sm, double* __restrict__ mcom,
+    double* __restrict__ mom, float* __restrict__ n32) {
 #pragma clang fp contract(off)
     const int k = lb + blockIdx.x * 64 + threadIdx.x;
     if (k >= le) return;
     double mass = 0.0, w[3] = {0.0, 0.0, 0.0}, M[kMom];
+#pragma unroll
     for (int a = 0; a < kMom; ++a) M[a] = 0.0;
     const int nc = nchild[k];
+    const int b = nb[k], e = ne[k];
+    const int fc = first_child[k];
     if (nc == 0) {
-        for (int s = nb[k]; s < ne[k]; ++s) {
-            const int i = sidx[s];
-            mass += m[i];
-            w[0] += x[i] * m[i];
-            w[1] += y[i] * m[i];
-            w[2] += z[i] * m[i];
+        for (int j = b; j < e; ++j) {
+            const double mj = sm[j];
+            mass += mj;
+            w[0] += sx[j] * mj;
+            w[1] += sy[j] * mj;
+            w[2] += sz[j] * mj;
         }
     } else {
-        for (int c = first_child[k]; c < first_child[k] + nc; ++c) {
+        for (int c = fc; c < fc + nc; ++c) {
             const double cm = mcom[4 * c];
             mass += cm;
             for (int d = 0; d < 3; ++d) w[d] += mcom[4 * c + 1 + d] * cm;
@@ -289,32 +266,23 @@ __global__ __launch_bounds__(64) void upward_kernel(
     double com[3];
     for (int d = 0; d < 3; ++d) com[d] = mass > 0.0 ? w[d] / mass : geom[4 * k + d];
     if (nc == 0) {
-        for (int s = nb[k]; s < ne[k]; ++s) {
-            const int i = sidx[s];
-            const double o[3] = {x[i] - com[0], y[i] - com[1], z[i] - com[2]};
-            for (int a = 4; a < kMom; ++a)
-                M[a] += m[i] * (ipow(o[0], c_tables.e[a][0]) * ipow(o[1], c_tables.e[a][1]) *
-                                ipow(o[2], c_tables.e[a][2]));
-        }
+        for (int j = b; j < e; ++j)
+            p2m_add(sx[j] - com[0], sy[j] - com[1], sz[j] - com[2], sm[j], M);
     } else {
-        for (int c = first_child[k]; c < first_child[k] + nc; ++c) {
-            const double cm = mcom[4 * c];
-            const double sh[3] = {mcom[4 * c + 1] - com[0], mcom[4 * c + 2] - com[1],
-                                  mcom[4 * c + 3] - com[2]};
+        for (int c = fc; c < fc + nc; ++c) {
+            double C[kMom];
             const double* cmom = mom + static_cast<size_t>(kMom) * c;
-            for (int t = 0; t < c_tables.npair; ++t) {
-                const int b = c_tables.pb[t];
-                const double operand = b == 0 ? cm : cmom[b];
-                M[c_tables.pa[t]] += c_tables.coef[t] * ipow(sh[0], c_tables.sx[t]) *
-                                     ipow(sh[1], c_tables.sy[t]) * ipow(sh[2], c_tables.sz[t]) *
-                                     operand;
-            }
+#pragma unroll
+            for (int a = 4; a < kMom; ++a) C[a] = cmom[a];
+            m2m_add(mcom[4 * c + 1] - com[0], mcom[4 * c + 2] - com[1], mcom[4 * c + 3] - com[2],
+                    mcom[4 * c], C, M);
         }
     }
     mcom[4 * k] = mass;
     for (int d = 0; d < 3; ++d) mcom[4 * k + 1 + d] = com[d];
     double* out = mom + static_cast<size_t>(kMom) * k;
     float* nf = n32 + static_cast<size_t>(kN32) * k;
+#pragma unroll
     for (int a = 0; a < kMom; ++a) {
         out[a] = M[a];
         nf[a] = static_cast<float>(a == 0 ? mass : M[a] * c_tables.inv_fact[a]);
@@ -1069,7 +1037,7 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         const int lb = level_begin[level], le = level_begin[level + 1];
         if (le <= lb) continue;
         hipLaunchKernelGGL(upward_kernel, dim3((le - lb + 63) / 64), dim3(64), 0, st, lb, le, nb, ne,
-                           first_child, nchild, geom, idx_sorted, s.x, s.y, s.z, s.m, mcom, mom, n32);
+                           first_child, nchild, geom, sxs, sys, szs, sms, mcom, mom, n32);
     }
 
     // leaves (mass > 0)
