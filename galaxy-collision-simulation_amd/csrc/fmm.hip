@@ -523,9 +523,7 @@ __global__ __launch_bounds__(kEvalLanes) void eval_far_kernel(
             F sfx = 0, sfy = 0, sfz = 0;
             for (int f = g; f < k; f += groups) {
                 const F yx = fs.c[f][0] - tx, yy = fs.c[f][1] - ty, yz = fs.c[f][2] - tz;
-                F N[kMom];
-#pragma unroll
-                for (int a = 0; a < kMom; ++a) N[a] = fs.N[f][a];
+                const F* N = fs.N[f];  // read from LDS as the contraction needs them
                 F fx = 0, fy = 0, fz = 0;
                 if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
                 else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);

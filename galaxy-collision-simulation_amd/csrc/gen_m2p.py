@@ -18,8 +18,9 @@ derivatives factor per axis like Hermite polynomials:
                             * y^(beta - 2 gamma) * T_(|beta| - |gamma|)
     T_m = (-1)^m (2m - 1)!! s^(-m - 1/2),   s = |y|^2 + eps^2.
 
-The emitted function computes the needed q_m = s^(-m-1/2) from one rsqrt,
-the monomials y^beta, the derivative tensors, then the contraction.
+The emitted function computes the needed q_m = s^(-m-1/2) from one rsqrt and
+the monomials y^beta, then forms each derivative tensor and contracts it at
+once into the three sums.
 Run: python3 gen_m2p.py > m2p_generated.inc
 """
 from __future__ import annotations
@@ -124,6 +125,10 @@ def emit(p):
             w(f"    const F {name(mono)} = {axis[c]};")
         else:
             w(f"    const F {name(mono)} = {name(prev)} * {axis[c]};")
+    # each derivative tensor is contracted as soon as it is formed, so only
+    # the three sums stay live (fewer registers than forming all of them)
+    alpha_set = set(alphas)
+    w("    F sx = F(0), sy = F(0), sz = F(0);")
     for b in betas:
         expr = None
         for coef, mono, m in deriv_terms(b):
@@ -134,15 +139,20 @@ def emit(p):
             else:
                 expr = (f"({factor}) * {name(mono)}" if expr is None else
                         f"GC_FMA({factor}, {name(mono)}, {expr})")
-        w(f"    const F D{b[0]}{b[1]}{b[2]} = {expr};")
-    for c, out in enumerate(("ax", "ay", "az")):
-        expr = None
-        for a in alphas:
-            b = list(a)
-            b[c] += 1
-            d = f"D{b[0]}{b[1]}{b[2]}"
-            expr = f"N[{INDEX[a]}] * {d}" if expr is None else f"GC_FMA(N[{INDEX[a]}], {d}, {expr})"
-        w(f"    {out} -= {expr};")
+        d = f"D{b[0]}{b[1]}{b[2]}"
+        w(f"    {{ const F {d} = {expr};")
+        for c, acc in enumerate(("sx", "sy", "sz")):
+            if b[c] == 0:
+                continue
+            a = list(b)
+            a[c] -= 1
+            a = tuple(a)
+            if a in alpha_set:
+                w(f"      {acc} = GC_FMA(N[{INDEX[a]}], {d}, {acc});")
+        w("    }")
+    w("    ax -= sx;")
+    w("    ay -= sy;")
+    w("    az -= sz;")
     w("}")
     return "\n".join(lines)
 
