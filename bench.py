@@ -200,6 +200,29 @@ def workload_adjoint(args):
     out["mixed_vs_fp64_grad_rel"] = {
         "v": ((gv32 - gv64).norm() / gv64.norm()).item(),
         "m": ((gm32 - gm64).norm() / gm64.norm()).item()}
+    # VJP kernel throughput on the two-disk N = 65,536 state (47 flop per
+    # directed pair), mixed and FP64
+    p2, v2, m2 = gcsim.generators.config1()
+    n2 = len(m2)
+    lam = torch.randn((3, n2), dtype=torch.float64, device=dev, generator=g)
+    xbar = torch.empty((3, n2), dtype=torch.float64, device=dev)
+    mbar = torch.empty((n2,), dtype=torch.float64, device=dev)
+    for prec, reps in (("mixed", 10), ("fp64", 2)):
+        eng = gcsim.Engine("hip-direct", prec, softening=0.02)
+        eng.upload(p2, v2, m2)
+        torch.cuda.synchronize()
+        eng.force_vjp(lam.data_ptr(), xbar.data_ptr(), mbar.data_ptr())
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eng.force_vjp(lam.data_ptr(), xbar.data_ptr(), mbar.data_ptr())
+        eng.synchronize()
+        sec = (time.perf_counter() - t0) / reps
+        eng.close()
+        rate = n2 * (n2 - 1) / sec
+        out["vjp_n65536_" + prec] = {"ms": sec * 1e3, "Gpair_per_s": rate / 1e9,
+                                     "tflops_47": 47.0 * rate / 1e12,
+                                     "frac_fp32_peak": 47.0 * rate / 157.3e12}
     return out
 
 

@@ -5,10 +5,14 @@
 //                  L = sum m (x cross v); grid-stride lanes, wavefront
 //                  (DPP/shuffle) reductions, one partial row per block.
 //   potential    : PE = -sum_{i<j} G m_i m_j / sqrt(|x_j - x_i|^2 + eps^2),
-//                  one target per lane over sources j > i staged in LDS,
-//                  Neumaier-compensated per lane, one partial per block.
-//   final        : one block folds the partials in fixed order with
-//                  compensation and writes the gc_diagnostics record.
+//                  grid (target tile I, source chunk c): one target per
+//                  lane over the sources j > i of kPotChunk tiles from
+//                  tile I on, staged in LDS, Neumaier-compensated per lane,
+//                  one partial per block (zero for chunks left of I); a
+//                  row pass folds each target tile's chunks in order;
+//   final        : one block folds the row sums and the linear partials in
+//                  fixed order with compensation and writes the
+//                  gc_diagnostics record.
 // Deterministic (no atomics).  Per-pair arithmetic follows the reference
 // ((G*m_i)*m_j / d, d = sqrt(((dx*dx + dy*dy) + dz*dz) + eps^2)); only the
 // summation order differs.
@@ -80,6 +84,8 @@ struct alignas(32) Src64 {
     double x, y, z, m;
 };
 
+constexpr int kPotChunk = 4;  // source tiles per block
+
 __global__ __launch_bounds__(256) void diag_potential_kernel(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ m, double G, double eps2,
@@ -88,12 +94,20 @@ __global__ __launch_bounds__(256) void diag_potential_kernel(
     __shared__ Src64 tile[256];
     __shared__ double red[4];
     const int t = threadIdx.x;
-    const int i = blockIdx.x * 256 + t;
+    const int I = blockIdx.x, c = blockIdx.y;
+    const int i = I * 256 + t;
+    const int jt0 = max(I, c * kPotChunk), jt1 = min((c + 1) * kPotChunk, static_cast<int>(gridDim.x));
+    double* out = partial + static_cast<size_t>(I) * gridDim.y + c;
+    if (jt0 >= jt1) {  // chunk left of the target tile: no pairs
+        if (t == 0) *out = 0.0;
+        return;
+    }
     const bool active = i < n;
     const double xi = active ? x[i] : 0.0, yi = active ? y[i] : 0.0, zi = active ? z[i] : 0.0;
     const double gmi = active ? G * m[i] : 0.0;
-    double s = 0.0, c = 0.0;
-    for (int j0 = blockIdx.x * 256; j0 < n; j0 += 256) {
+    double s = 0.0, cc = 0.0;
+    for (int jt = jt0; jt < jt1; ++jt) {
+        const int j0 = jt * 256;
         __syncthreads();
         if (j0 + t < n) {
             Src64 q;
@@ -111,14 +125,26 @@ __global__ __launch_bounds__(256) void diag_potential_kernel(
             const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
             const double d = __builtin_sqrt((dx * dx + dy * dy + dz * dz) + eps2);
             if (d == 0.0) continue;
-            neumaier(s, c, gmi * q.m / d);
+            neumaier(s, cc, gmi * q.m / d);
         }
     }
-    double v = active ? (s + c) : 0.0;
+    double v = active ? (s + cc) : 0.0;
     v = wave_sum(v);
     if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();
-    if (t == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+    if (t == 0) *out = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// One thread per target tile: its chunk partials in chunk order.
+__global__ __launch_bounds__(256) void diag_rows_kernel(int tiles, int chunks,
+                                                        const double* __restrict__ partial,
+                                                        double* __restrict__ rows) {
+#pragma clang fp contract(off)
+    const int I = blockIdx.x * 256 + threadIdx.x;
+    if (I >= tiles) return;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < chunks; ++k) neumaier(s, c, partial[static_cast<size_t>(I) * chunks + k]);
+    rows[I] = s + c;
 }
 
 // Writes the gc_diagnostics field order: KE, PE, E, M, P[3], COM[3], L[3].
@@ -160,26 +186,30 @@ int linear_blocks(size_t n) {
     return static_cast<int>(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, kLinBlocksMax)));
 }
 
-int potential_blocks(size_t n) { return static_cast<int>(std::max<size_t>(1, (n + 255) / 256)); }
+int potential_tiles(size_t n) { return static_cast<int>(std::max<size_t>(1, (n + 255) / 256)); }
+int potential_chunks(size_t n) { return (potential_tiles(n) + kPotChunk - 1) / kPotChunk; }
 
 }  // namespace
 
 size_t diagnostics_scratch_doubles(size_t n) {
-    return static_cast<size_t>(linear_blocks(n)) * kLin + potential_blocks(n);
+    const size_t T = static_cast<size_t>(potential_tiles(n));
+    return static_cast<size_t>(linear_blocks(n)) * kLin + T * potential_chunks(n) + T;
 }
 
 void launch_diagnostics(const DeviceState& s, double G, double eps, double* scratch,
                         size_t scratch_doubles, double* out_dev, hipStream_t st) {
-    const int nl = linear_blocks(s.n), np = potential_blocks(s.n);
-    if (scratch_doubles < static_cast<size_t>(nl) * kLin + np) return;
+    const int nl = linear_blocks(s.n), T = potential_tiles(s.n), C = potential_chunks(s.n);
+    if (scratch_doubles < diagnostics_scratch_doubles(s.n)) return;
     double* lin = scratch;
     double* pot = scratch + static_cast<size_t>(nl) * kLin;
+    double* rows = pot + static_cast<size_t>(T) * C;
     const int n = static_cast<int>(s.n);
     hipLaunchKernelGGL(diag_linear_kernel, dim3(nl), dim3(256), 0, st, n, s.x, s.y, s.z, s.vx,
                        s.vy, s.vz, s.m, lin);
-    hipLaunchKernelGGL(diag_potential_kernel, dim3(np), dim3(256), 0, st, n, s.x, s.y, s.z, s.m, G,
-                       eps * eps, pot);
-    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(64), 0, st, nl, lin, np, pot, out_dev);
+    hipLaunchKernelGGL(diag_potential_kernel, dim3(T, C), dim3(256), 0, st, n, s.x, s.y, s.z, s.m,
+                       G, eps * eps, pot);
+    hipLaunchKernelGGL(diag_rows_kernel, dim3((T + 255) / 256), dim3(256), 0, st, T, C, pot, rows);
+    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(64), 0, st, nl, lin, T, rows, out_dev);
 }
 
 }  // namespace gc
