@@ -216,7 +216,7 @@ gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double hal
         if (st != GC_OK) return st;
         launch_pack_mixed(c->s, c->mixed, p.gravitational_constant, c->stream);
         launch_direct_sym(c->s, c->mixed, c->sym, p.softening, sym_ipt(c), c->stream);
-        launch_merge_sym(c->s, c->sym, close_kick, half_dt, c->stream);
+        launch_merge_sym(c->s, c->sym, nullptr, close_kick, half_dt, c->stream);
         return launch_status(c, "launch symmetric mixed kernels");
     }
     int ipt = 1, js = 1;
@@ -535,7 +535,7 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
     if (inter) *inter = static_cast<double>(n) * static_cast<double>(n - 1);
     if (sym) {
         // restore accelerations consistent with the state
-        launch_merge_sym(c->s, c->sym, false, 0.0, c->stream);
+        launch_merge_sym(c->s, c->sym, nullptr, false, 0.0, c->stream);
     } else if (c->params.precision == GC_PRECISION_MIXED) {
         launch_merge_mixed(c->s, c->mixed, js, 0, n, false, 0.0, c->stream);
     }

@@ -55,7 +55,7 @@ constexpr int kChunks = kTile / kChunk;   // 4
 // a step serve 2 * IPT interactions.
 template <int IPT, bool SAFE>
 __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
-    const int4* __restrict__ items, int n, int ntiles, const double* __restrict__ x,
+    const SymItem* __restrict__ items, int nitems, int n, int ntiles, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ jp,
     const double* __restrict__ centre, float eps2, float* __restrict__ ipart,
     float* __restrict__ jpart) {
@@ -66,109 +66,112 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
     __shared__ float4 src2[kChunks][2 * kChunk];
     // waves 1.. hand their lanes' chunk sums of -G m_i r3 d to wave 0
     __shared__ float red[kWaves > 1 ? kWaves - 1 : 1][3][kChunk];
-    const int4 item = items[blockIdx.x];
-    const int I = item.x;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    double xi[IPT], yi[IPT], zi[IPT];
-    float mi[IPT];
-    float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const SymItem item = items[it];
+        const int I = item.I;
+        double xi[IPT], yi[IPT], zi[IPT];
+        float mi[IPT];
+        float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        const int i = I * kTile + k * kThreads + t;
-        const bool active = i < n;
-        xi[k] = active ? x[i] : 0.0;
-        yi[k] = active ? y[i] : 0.0;
-        zi[k] = active ? z[i] : 0.0;
-        mi[k] = jp[i].w;  // G m_i; zero past n
-        hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
-    }
-    for (int J = item.y; J < item.z; ++J) {
-        float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
-        tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
-#pragma unroll
-        for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
-        const float4* src = jp + static_cast<size_t>(J) * kTile;
-        if (J == I) {
-            tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, px, py, pz);
-        } else {
-            for (int e = t; e < kTile; e += kThreads) {
-                const float4 v = src[e];
-                src2[e / kChunk][e % kChunk] = v;
-                src2[e / kChunk][kChunk + e % kChunk] = v;
-            }
-            __syncthreads();
-            for (int c = 0; c < kChunks; ++c) {
-                // lane l holds the running sum of source c*64 + (l + s) % 64
-                // at step s; wave_rol1 after each step keeps that true.
-                float ax = 0.f, ay = 0.f, az = 0.f;
-                const float4* row = &src2[c][lane];
-#pragma unroll 8
-                for (int s = 0; s < kChunk; ++s) {
-                    const float4 q = row[s];
-#pragma unroll
-                    for (int k = 0; k < IPT; ++k) {
-                        const float dx = q.x - tx[k], dy = q.y - ty[k], dz = q.z - tz[k];
-                        const float s2 = __builtin_fmaf(
-                            dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-                        float r;
-                        if (SAFE) {
-                            r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
-                        } else {
-                            r = __builtin_amdgcn_rsqf(s2);
-                        }
-                        const float r3 = (r * r) * r;
-                        const float kj = q.w * r3, ki = mi[k] * r3;
-                        px[k] = __builtin_fmaf(dx, kj, px[k]);
-                        py[k] = __builtin_fmaf(dy, kj, py[k]);
-                        pz[k] = __builtin_fmaf(dz, kj, pz[k]);
-                        ax = __builtin_fmaf(-dx, ki, ax);
-                        ay = __builtin_fmaf(-dy, ki, ay);
-                        az = __builtin_fmaf(-dz, ki, az);
-                    }
-                    ax = wave_rol1(ax);
-                    ay = wave_rol1(ay);
-                    az = wave_rol1(az);
-                }
-                // after 64 steps lane l holds source c*64 + l; wave 0 adds
-                // the other waves' sums in wave order and stores the chunk
-                float* out = jpart + pair_slot(I, J) * 3 * kTile + c * kChunk + lane;
-                if (kWaves > 1) {
-                    if (wave > 0) {
-                        red[wave - 1][0][lane] = ax;
-                        red[wave - 1][1][lane] = ay;
-                        red[wave - 1][2][lane] = az;
-                    }
-                    __syncthreads();
-                    if (wave == 0) {
-#pragma unroll
-                        for (int w = 0; w < kWaves - 1; ++w) {
-                            ax += red[w][0][lane];
-                            ay += red[w][1][lane];
-                            az += red[w][2][lane];
-                        }
-                    }
-                    __syncthreads();
-                }
-                if (wave == 0) {
-                    out[0] = ax;
-                    out[kTile] = ay;
-                    out[2 * kTile] = az;
-                }
-            }
+        for (int k = 0; k < IPT; ++k) {
+            const int i = I * kTile + k * kThreads + t;
+            const bool active = i < n;
+            xi[k] = active ? x[i] : 0.0;
+            yi[k] = active ? y[i] : 0.0;
+            zi[k] = active ? z[i] : 0.0;
+            mi[k] = jp[i].w;  // G m_i; zero past n
+            hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
         }
-        fold_tile<IPT>(px, py, pz, hx, hy, hz, lx, ly, lz);
-    }
-    float* base = ipart + static_cast<size_t>(item.w) * 6 * kTile;
+        for (int J = item.jb; J < item.je; ++J) {
+            float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
+            tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        const int e = k * kThreads + t;
-        base[e] = hx[k];
-        base[kTile + e] = hy[k];
-        base[2 * kTile + e] = hz[k];
-        base[3 * kTile + e] = lx[k];
-        base[4 * kTile + e] = ly[k];
-        base[5 * kTile + e] = lz[k];
+            for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
+            const float4* src = jp + static_cast<size_t>(J) * kTile;
+            if (J == I) {
+                tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, px, py, pz);
+            } else {
+                for (int e = t; e < kTile; e += kThreads) {
+                    const float4 v = src[e];
+                    src2[e / kChunk][e % kChunk] = v;
+                    src2[e / kChunk][kChunk + e % kChunk] = v;
+                }
+                __syncthreads();
+                for (int c = 0; c < kChunks; ++c) {
+                    // lane l holds the running sum of source c*64 + (l + s) % 64
+                    // at step s; wave_rol1 after each step keeps that true.
+                    float ax = 0.f, ay = 0.f, az = 0.f;
+                    const float4* row = &src2[c][lane];
+#pragma unroll 8
+                    for (int s = 0; s < kChunk; ++s) {
+                        const float4 q = row[s];
+#pragma unroll
+                        for (int k = 0; k < IPT; ++k) {
+                            const float dx = q.x - tx[k], dy = q.y - ty[k], dz = q.z - tz[k];
+                            const float s2 = __builtin_fmaf(
+                                dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+                            float r;
+                            if (SAFE) {
+                                r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
+                            } else {
+                                r = __builtin_amdgcn_rsqf(s2);
+                            }
+                            const float r3 = (r * r) * r;
+                            const float kj = q.w * r3, ki = mi[k] * r3;
+                            px[k] = __builtin_fmaf(dx, kj, px[k]);
+                            py[k] = __builtin_fmaf(dy, kj, py[k]);
+                            pz[k] = __builtin_fmaf(dz, kj, pz[k]);
+                            ax = __builtin_fmaf(-dx, ki, ax);
+                            ay = __builtin_fmaf(-dy, ki, ay);
+                            az = __builtin_fmaf(-dz, ki, az);
+                        }
+                        ax = wave_rol1(ax);
+                        ay = wave_rol1(ay);
+                        az = wave_rol1(az);
+                    }
+                    // after 64 steps lane l holds source c*64 + l; wave 0 adds
+                    // the other waves' sums in wave order and stores the chunk
+                    float* out = jpart + (item.pbase + pair_slot(I - item.t0, J - item.t0)) * 3 * kTile + c * kChunk + lane;
+                    if (kWaves > 1) {
+                        if (wave > 0) {
+                            red[wave - 1][0][lane] = ax;
+                            red[wave - 1][1][lane] = ay;
+                            red[wave - 1][2][lane] = az;
+                        }
+                        __syncthreads();
+                        if (wave == 0) {
+#pragma unroll
+                            for (int w = 0; w < kWaves - 1; ++w) {
+                                ax += red[w][0][lane];
+                                ay += red[w][1][lane];
+                                az += red[w][2][lane];
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (wave == 0) {
+                        out[0] = ax;
+                        out[kTile] = ay;
+                        out[2 * kTile] = az;
+                    }
+                }
+            }
+            fold_tile<IPT>(px, py, pz, hx, hy, hz, lx, ly, lz);
+        }
+        float* base = ipart + static_cast<size_t>(item.islot) * 6 * kTile;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int e = k * kThreads + t;
+            base[e] = hx[k];
+            base[kTile + e] = hy[k];
+            base[2 * kTile + e] = hz[k];
+            base[3 * kTile + e] = lx[k];
+            base[4 * kTile + e] = ly[k];
+            base[5 * kTile + e] = lz[k];
+        }
+        __syncthreads();  // LDS reuse by the next item
     }
 }
 
@@ -181,10 +184,11 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
 constexpr int kMergeGroups = 4;
 
 __global__ __launch_bounds__(kTile * kMergeGroups) void merge_sym_kernel(
-    int n, int ntiles, const int* __restrict__ row_start, const float* __restrict__ ipart,
-    const float* __restrict__ jpart, double* __restrict__ ax, double* __restrict__ ay,
-    double* __restrict__ az, double* __restrict__ vx, double* __restrict__ vy,
-    double* __restrict__ vz, int close_kick, double half_dt) {
+    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    const long long* __restrict__ tile_pbase, const int* __restrict__ tile_valid,
+    const float* __restrict__ ipart, const float* __restrict__ jpart, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
+    double* __restrict__ vy, double* __restrict__ vz, int close_kick, double half_dt) {
 #pragma clang fp contract(off)
     __shared__ double part[kMergeGroups - 1][3][kTile];
     const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
@@ -200,8 +204,10 @@ __global__ __launch_bounds__(kTile * kMergeGroups) void merge_sym_kernel(
                         static_cast<double>(b[(c + 3) * kTile + l]);
         }
     }
-    const float* col = jpart + pair_slot(0, T) * 3 * kTile;
-    const int i0 = g * T / kMergeGroups, i1 = (g + 1) * T / kMergeGroups;
+    // column: pair partials (I, T) for I in [t0, T) of T's run
+    const int t0 = tile_t0[T], cols = T - t0;
+    const float* col = jpart + (tile_pbase[T] + pair_slot(0, cols)) * 3 * kTile;
+    const int i0 = g * cols / kMergeGroups, i1 = (g + 1) * cols / kMergeGroups;
     for (int I = i0; I < i1; ++I) {
         const float* b = col + static_cast<size_t>(I) * 3 * kTile;
 #pragma unroll
@@ -218,8 +224,9 @@ __global__ __launch_bounds__(kTile * kMergeGroups) void merge_sym_kernel(
 #pragma unroll
         for (int c = 0; c < 3; ++c) a[c] += part[k][c][l];
     }
+    const int valid = tile_valid ? tile_valid[T] : n - T * kTile;
+    if (l >= valid) return;
     const int i = T * kTile + l;
-    if (i >= n) return;
     ax[i] = a[0];
     ay[i] = a[1];
     az[i] = a[2];
@@ -232,40 +239,75 @@ __global__ __launch_bounds__(kTile * kMergeGroups) void merge_sym_kernel(
 
 }  // namespace
 
-int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err) {
-    if (ntiles == plan.ntiles && plan.items) return 0;
+int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns,
+                     hipStream_t st, std::string* err) {
+    int ntiles = 0;
+    for (int r = 0; r < nruns; ++r) ntiles = std::max(ntiles, run_t0[r] + run_nt[r]);
     if (plan.items) (void)hipStreamSynchronize(st);
     sym_release(plan);
-    if (ntiles < 1 || ntiles > kSymMaxTiles) {
-        *err = "symmetric form: tile count out of range";
+    if (ntiles < 1) {
+        *err = "symmetric form: no tiles";
         return 1;
     }
-    // items of `len` consecutive J tiles per row, ~8192 items in all
-    const size_t tri = static_cast<size_t>(ntiles) * (ntiles + 1) / 2;
-    const int len = static_cast<int>(std::max<size_t>(1, tri / 8192));
-    std::vector<int4> items;
-    std::vector<int> row_start(ntiles + 1, 0);
-    for (int I = 0; I < ntiles; ++I) {
-        row_start[I] = static_cast<int>(items.size());
-        for (int J = I; J < ntiles; J += len)
-            items.push_back(make_int4(I, J, std::min(J + len, ntiles), static_cast<int>(items.size())));
+    // per run: items of `len` consecutive J tiles per row, ~8192 items per
+    // run of 128+ tiles; row slots in (run, row, J) order; pair partials per
+    // run from that run's base
+    std::vector<SymItem> items;
+    std::vector<int> row_start(ntiles + 1, 0), tile_t0(ntiles, 0);
+    std::vector<long long> tile_pbase(ntiles, 0);
+    long long pbase = 0;
+    int next_tile = 0;
+    for (int r = 0; r < nruns; ++r) {
+        const int t0 = run_t0[r], nt = run_nt[r];
+        if (nt > kSymMaxTiles) {
+            *err = "symmetric form: run of more than 1024 tiles";
+            return 1;
+        }
+        for (int T = next_tile; T < t0; ++T) row_start[T] = static_cast<int>(items.size());
+        const size_t tri = static_cast<size_t>(nt) * (nt + 1) / 2;
+        const int len = static_cast<int>(std::max<size_t>(1, tri / 8192));
+        for (int I = t0; I < t0 + nt; ++I) {
+            row_start[I] = static_cast<int>(items.size());
+            tile_t0[I] = t0;
+            tile_pbase[I] = pbase;
+            for (int J = I; J < t0 + nt; J += len) {
+                SymItem it{};
+                it.I = I;
+                it.jb = J;
+                it.je = std::min(J + len, t0 + nt);
+                it.islot = static_cast<int>(items.size());
+                it.pbase = pbase;
+                it.t0 = t0;
+                items.push_back(it);
+            }
+        }
+        pbase += static_cast<long long>(pair_slot(0, nt));
+        next_tile = t0 + nt;
     }
-    row_start[ntiles] = static_cast<int>(items.size());
-    // longest items first; ties keep row-major order
+    for (int T = next_tile; T <= ntiles; ++T) row_start[T] = static_cast<int>(items.size());
+    // longest items first; ties keep (run, row) order
     std::stable_sort(items.begin(), items.end(),
-                     [](const int4& a, const int4& b) { return (a.z - a.y) > (b.z - b.y); });
+                     [](const SymItem& a, const SymItem& b) { return (a.je - a.jb) > (b.je - b.jb); });
     plan.nitems = static_cast<int>(items.size());
     plan.ipart_floats = static_cast<size_t>(plan.nitems) * 6 * kTile;
-    plan.jpart_floats = std::max<size_t>(1, pair_slot(0, ntiles)) * 3 * kTile;
-    hipError_t e = hipMalloc(&plan.items, sizeof(int4) * items.size());
+    plan.jpart_floats = std::max<size_t>(1, static_cast<size_t>(pbase)) * 3 * kTile;
+    hipError_t e = hipMalloc(&plan.items, sizeof(SymItem) * items.size());
     if (e == hipSuccess) e = hipMalloc(&plan.row_start, sizeof(int) * row_start.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.tile_t0, sizeof(int) * tile_t0.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.tile_pbase, sizeof(long long) * tile_pbase.size());
     if (e == hipSuccess) e = hipMalloc(&plan.ipart, sizeof(float) * plan.ipart_floats);
     if (e == hipSuccess) e = hipMalloc(&plan.jpart, sizeof(float) * plan.jpart_floats);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(plan.items, items.data(), sizeof(int4) * items.size(),
+        e = hipMemcpyAsync(plan.items, items.data(), sizeof(SymItem) * items.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(plan.row_start, row_start.data(), sizeof(int) * row_start.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan.tile_t0, tile_t0.data(), sizeof(int) * tile_t0.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan.tile_pbase, tile_pbase.data(), sizeof(long long) * tile_pbase.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);  // host vectors end here
     if (e != hipSuccess) {
@@ -277,45 +319,65 @@ int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err) {
     return 0;
 }
 
+int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err) {
+    if (ntiles == plan.ntiles && plan.items) return 0;
+    if (ntiles < 1 || ntiles > kSymMaxTiles) {
+        *err = "symmetric form: tile count out of range";
+        return 1;
+    }
+    const int t0 = 0;
+    return sym_prepare_runs(plan, &t0, &ntiles, 1, st, err);
+}
+
 void sym_release(SymPlan& plan) {
     (void)hipFree(plan.items);
     (void)hipFree(plan.row_start);
+    (void)hipFree(plan.tile_t0);
+    (void)hipFree(plan.tile_pbase);
     (void)hipFree(plan.ipart);
     (void)hipFree(plan.jpart);
     plan = SymPlan{};
 }
 
 template <int IPT>
-void launch_sym_ipt(bool safe, const DeviceState& s, const MixedLayout& L, const SymPlan& plan,
-                    float eps2, hipStream_t st) {
-    const int n = static_cast<int>(s.n), ntiles = static_cast<int>(L.ntiles);
+void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const double* centre,
+                    int ntiles, const SymPlan& plan, float eps2, int blocks, hipStream_t st) {
+    const int n = static_cast<int>(s.n);
+    const dim3 grid(static_cast<unsigned>(blocks > 0 ? std::min(blocks, plan.nitems) : plan.nitems));
     if (safe) {
-        hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), dim3(plan.nitems), dim3(kTile / IPT), 0,
-                           st, plan.items, n, ntiles, s.x, s.y, s.z, L.jp, L.centre, eps2,
+        hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), grid, dim3(kTile / IPT), 0, st,
+                           plan.items, plan.nitems, n, ntiles, s.x, s.y, s.z, jp, centre, eps2,
                            plan.ipart, plan.jpart);
     } else {
-        hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), dim3(plan.nitems), dim3(kTile / IPT), 0,
-                           st, plan.items, n, ntiles, s.x, s.y, s.z, L.jp, L.centre, eps2,
+        hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), grid, dim3(kTile / IPT), 0, st,
+                           plan.items, plan.nitems, n, ntiles, s.x, s.y, s.z, jp, centre, eps2,
                            plan.ipart, plan.jpart);
+    }
+}
+
+void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const double* centre,
+                             int ntiles, const SymPlan& plan, double eps, int ipt, int blocks,
+                             hipStream_t st) {
+    const float eps2 = static_cast<float>(eps * eps);
+    const bool safe = !(eps2 >= kTinySeparation2);
+    switch (ipt) {
+    case 4: launch_sym_ipt<4>(safe, s, jp, centre, ntiles, plan, eps2, blocks, st); break;
+    case 2: launch_sym_ipt<2>(safe, s, jp, centre, ntiles, plan, eps2, blocks, st); break;
+    default: launch_sym_ipt<1>(safe, s, jp, centre, ntiles, plan, eps2, blocks, st); break;
     }
 }
 
 void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
                        int ipt, hipStream_t st) {
-    const float eps2 = static_cast<float>(eps * eps);
-    const bool safe = !(eps2 >= kTinySeparation2);
-    switch (ipt) {
-    case 4: launch_sym_ipt<4>(safe, s, L, plan, eps2, st); break;
-    case 2: launch_sym_ipt<2>(safe, s, L, plan, eps2, st); break;
-    default: launch_sym_ipt<1>(safe, s, L, plan, eps2, st); break;
-    }
+    launch_direct_sym_tiles(s, L.jp, L.centre, static_cast<int>(L.ntiles), plan, eps, ipt, 0, st);
 }
 
-void launch_merge_sym(const DeviceState& s, const SymPlan& plan, bool close_kick, double half_dt,
-                      hipStream_t st) {
+void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
+                      bool close_kick, double half_dt, hipStream_t st) {
     hipLaunchKernelGGL(merge_sym_kernel, dim3(plan.ntiles), dim3(kTile * kMergeGroups), 0, st,
-                       static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.ipart, plan.jpart,
-                       s.ax, s.ay, s.az, s.vx, s.vy, s.vz, close_kick ? 1 : 0, half_dt);
+                       static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
+                       plan.tile_pbase, tile_valid, plan.ipart, plan.jpart, s.ax, s.ay, s.az, s.vx,
+                       s.vy, s.vz, close_kick ? 1 : 0, half_dt);
 }
 
 }  // namespace gc

@@ -43,13 +43,22 @@ struct MixedLayout {
 // Symmetric (each unordered pair once) mixed form: the work list over the
 // upper triangle of source-tile pairs and its partial sums.
 constexpr int kSymMaxTiles = 1024;  // N <= 262,144: pair partials <= 1.6 GB
+// One work item: row tile I, source tiles [jb, je), its row-partial slot;
+// pair partials of its run start at pbase, the run's first tile is t0.
+struct SymItem {
+    int I, jb, je, islot;
+    long long pbase;
+    int t0, pad;
+};
 struct SymPlan {
     int ntiles = 0;             // plan built for this tile count
     int nitems = 0;
-    int4* items = nullptr;      // (row I, J begin, J end, slot), longest first
+    SymItem* items = nullptr;   // longest first
     int* row_start = nullptr;   // ntiles + 1 slot offsets; a row's slots in J order
+    int* tile_t0 = nullptr;     // first tile of each tile's run
+    long long* tile_pbase = nullptr;  // pair-partial base of each tile's run
     float* ipart = nullptr;     // nitems * 6 * kTile (hi x,y,z | lo x,y,z)
-    float* jpart = nullptr;     // ntiles (ntiles - 1) / 2 pairs * 3 * kTile
+    float* jpart = nullptr;     // per run nt (nt - 1) / 2 pairs * 3 * kTile
     size_t ipart_floats = 0, jpart_floats = 0;
 };
 
@@ -73,12 +82,21 @@ void launch_merge_mixed(const DeviceState& s, const MixedLayout& L, int jsplit, 
                         size_t end, bool close_kick, double half_dt, hipStream_t st);
 // direct_sym.hip: 0 on success, 2 out of memory, 1/3 other errors (err set)
 int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err);
+// independent runs of tiles [run_t0[r], run_t0[r] + run_nt[r]) (pairs only
+// within a run)
+int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns,
+                     hipStream_t st, std::string* err);
 void sym_release(SymPlan& plan);
 // ipt: targets per lane (1, 2, 4)
 void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
                        int ipt, hipStream_t st);
-void launch_merge_sym(const DeviceState& s, const SymPlan& plan, bool close_kick, double half_dt,
-                      hipStream_t st);
+// blocks > 0: persistent grid of that many blocks over the items
+void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const double* centre,
+                             int ntiles, const SymPlan& plan, double eps, int ipt, int blocks,
+                             hipStream_t st);
+// tile_valid: valid targets per tile, or null for the first s.n slots
+void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
+                      bool close_kick, double half_dt, hipStream_t st);
 // integrate.hip
 void launch_open_step(const DeviceState& s, double half_dt, double dt, hipStream_t st);
 void launch_kick(const DeviceState& s, double dt, hipStream_t st);
