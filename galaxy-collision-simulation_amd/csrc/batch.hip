@@ -162,9 +162,15 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(
     }
 }
 
-// process-wide autotune cache: (runs, tiles, items at chunk 1) -> (chunk, blocks)
+// process-wide autotune cache: (runs, tiles, items at chunk 1, requested
+// pair form) -> tuned schedule
+struct Tuned {
+    int form;                  // GC_PAIRS_TARGETS or GC_PAIRS_SYMMETRIC
+    int chunk, blocks;         // per-target form
+    int ipt, sym_blocks;       // symmetric form
+};
 std::mutex g_tune_mutex;
-std::map<std::tuple<int, int, long long>, std::pair<int, int>> g_tune_cache;
+std::map<std::tuple<int, int, long long, int>, Tuned> g_tune_cache;
 
 }  // namespace
 }  // namespace gc
@@ -190,6 +196,11 @@ struct gc_batch {
     float* part = nullptr;
     size_t part_floats = 0;
     int chunk = 4, blocks = 1024, graph_steps = 10;
+    // each unordered pair once within a run (direct_sym.hip): pair form,
+    // targets per lane (0 = automatic), persistent blocks (0 = one per item)
+    int pairs = GC_PAIRS_AUTO, sym_ipt = 0, sym_blocks = 0;
+    int tuned_form = 0;  // AUTO after autotune: GC_PAIRS_TARGETS or _SYMMETRIC
+    SymPlan sym;
     hipGraphExec_t graph = nullptr;
     double graph_dt = 0.0;
     double* stage = nullptr;
@@ -218,6 +229,35 @@ gc_status bfail(gc_batch* b, gc_status code, const std::string& msg) {
 void drop_graph(gc_batch* b) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
+}
+
+// The symmetric form unless the per-target form is asked for (or chosen by
+// the autotuner) or a run has more than kSymMaxTiles tiles.
+bool batch_symmetric(const gc_batch* b) {
+    if (b->pairs == GC_PAIRS_TARGETS) return false;
+    if (b->pairs == GC_PAIRS_AUTO && b->tuned_form == GC_PAIRS_TARGETS) return false;
+    for (int nt : b->run_ntiles)
+        if (nt > kSymMaxTiles) return false;
+    return true;
+}
+
+gc_status ensure_batch_sym(gc_batch* b) {
+    if (b->sym.items) return GC_OK;
+    std::string msg;
+    const int rc = sym_prepare_runs(b->sym, b->run_tile0.data(), b->run_ntiles.data(), b->runs,
+                                    b->stream, &msg);
+    if (rc == 0) return GC_OK;
+    return bfail(b, rc == 2 ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP, msg);
+}
+
+// Targets per lane: as set, else the largest of 4, 2, 1 that still gives
+// >= 8192 waves.
+int batch_sym_ipt(const gc_batch* b) {
+    if (b->sym_ipt == 1 || b->sym_ipt == 2 || b->sym_ipt == 4) return b->sym_ipt;
+    const long waves4 = b->sym.nitems;
+    if (waves4 >= 8192) return 4;
+    if (2 * waves4 >= 8192) return 2;
+    return 1;
 }
 
 // Builds the work list for `chunk` source tiles per item (slots in
@@ -270,6 +310,11 @@ void launch_force(gc_batch* b) {
     hipLaunchKernelGGL(batch_pack_kernel, dim3(b->ntiles), dim3(kTile), 0, b->stream, b->tile_valid,
                        b->s.x, b->s.y, b->s.z, b->s.m, b->params.gravitational_constant, b->ntiles,
                        b->jp, b->centre);
+    if (batch_symmetric(b)) {
+        launch_direct_sym_tiles(b->s, b->jp, b->centre, b->tile_valid, b->ntiles, b->sym,
+                                b->params.softening, batch_sym_ipt(b), b->sym_blocks, b->stream);
+        return;
+    }
     const dim3 grid(static_cast<unsigned>(std::max(1, std::min(b->blocks, b->nitems))));
     if (safe) {
         hipLaunchKernelGGL(batch_force_kernel<true>, grid, dim3(kTile), 0, b->stream, b->nitems,
@@ -283,6 +328,10 @@ void launch_force(gc_batch* b) {
 }
 
 void launch_merge(gc_batch* b, bool kick, double half_dt) {
+    if (batch_symmetric(b)) {
+        launch_merge_sym(b->s, b->sym, b->tile_valid, kick, half_dt, b->stream);
+        return;
+    }
     hipLaunchKernelGGL(batch_merge_kernel, dim3((b->total + 255) / 256), dim3(256), 0, b->stream,
                        b->total, b->tile_valid, b->tile_slot0, b->tile_nchunk, b->part, b->s.ax,
                        b->s.ay, b->s.az, b->s.vx, b->s.vy, b->s.vz, kick ? 1 : 0, half_dt);
@@ -406,6 +455,7 @@ gc_status gc_batch_create(gc_batch** out, int device, const gc_params* p, int ru
     if (e == hipSuccess) e = hipMemset(b->s.base, 0, sizeof(double) * 10 * T);
     if (e != hipSuccess) return cleanup_fail("initialise batch", e);
     gc_status st = build_schedule(b, 4);
+    if (st == GC_OK && batch_symmetric(b)) st = ensure_batch_sym(b);
     if (st != GC_OK) {
         std::string msg = b->err;
         gc_batch_destroy(b);
@@ -420,6 +470,7 @@ void gc_batch_destroy(gc_batch* b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
+    sym_release(b->sym);
     void* ptrs[] = {b->s.base, b->jp, b->centre, b->tile_valid, b->tile_slot0, b->tile_nchunk,
                     b->items, b->part, b->stage};
     for (void* q : ptrs) (void)hipFree(q);
@@ -510,23 +561,26 @@ gc_status gc_batch_synchronize(gc_batch* b) {
 
 gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps) {
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
-    if (chunk_tiles < 1 || chunk_tiles > 4096 || blocks < 1 || blocks > 1 << 20 || graph_steps < 0 || graph_steps > 1000)
+    const bool sym = batch_symmetric(b);
+    if (chunk_tiles < 1 || chunk_tiles > 4096 || blocks < (sym ? 0 : 1) || blocks > 1 << 20 ||
+        graph_steps < 0 || graph_steps > 1000)
         return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_set_schedule: out of range");
     GB_HIP(b, hipSetDevice(b->device), "select device");
     if (chunk_tiles != b->chunk) {
         gc_status st = build_schedule(b, chunk_tiles);
         if (st != GC_OK) return st;
     }
-    if (blocks != b->blocks || graph_steps != b->graph_steps) drop_graph(b);
-    b->blocks = blocks;
+    drop_graph(b);
+    (sym ? b->sym_blocks : b->blocks) = blocks;
     b->graph_steps = graph_steps;
     return GC_OK;
 }
 
 gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps) {
     if (!b || !chunk_tiles || !blocks || !graph_steps) return bfail(b, GC_ERR_INVALID_ARGUMENT, "null argument");
-    *chunk_tiles = b->chunk;
-    *blocks = b->blocks;
+    const bool sym = batch_symmetric(b);
+    *chunk_tiles = sym ? 0 : b->chunk;
+    *blocks = sym ? b->sym_blocks : b->blocks;
     *graph_steps = b->graph_steps;
     return GC_OK;
 }
@@ -540,9 +594,14 @@ gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms, double* 
                        b->s.x, b->s.y, b->s.z, b->s.m, b->params.gravitational_constant, b->ntiles,
                        b->jp, b->centre);
     const dim3 grid(static_cast<unsigned>(std::max(1, std::min(b->blocks, b->nitems))));
+    const bool sym = batch_symmetric(b);
     GB_HIP(b, hipEventRecord(b->ev0, b->stream), "record");
     for (int r = 0; r < reps; ++r) {
-        if (safe) {
+        if (sym) {
+            launch_direct_sym_tiles(b->s, b->jp, b->centre, b->tile_valid, b->ntiles, b->sym,
+                                    b->params.softening, batch_sym_ipt(b), b->sym_blocks,
+                                    b->stream);
+        } else if (safe) {
             hipLaunchKernelGGL(batch_force_kernel<true>, grid, dim3(kTile), 0, b->stream, b->nitems,
                                b->items, b->tile_valid, b->s.x, b->s.y, b->s.z, b->jp, b->centre,
                                b->ntiles, eps2, b->part);
@@ -573,51 +632,114 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
     GB_HIP(b, hipSetDevice(b->device), "select device");
     long long items1 = 0;
     for (int r = 0; r < b->runs; ++r) items1 += static_cast<long long>(b->run_ntiles[r]) * b->run_ntiles[r];
-    const auto key = std::make_tuple(b->runs, b->ntiles, items1);
-    int best_c = b->chunk, best_b = b->blocks;
+    const auto key = std::make_tuple(b->runs, b->ntiles, items1, b->pairs);
+    Tuned best_t{batch_symmetric(b) ? GC_PAIRS_SYMMETRIC : GC_PAIRS_TARGETS, b->chunk, b->blocks,
+                 b->sym_ipt, b->sym_blocks};
     bool cached = false;
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            best_c = it->second.first;
-            best_b = it->second.second;
+            best_t = it->second;
             cached = true;
         }
     }
+    const int requested = b->pairs;
     if (!cached) {
         hipDeviceProp_t prop;
         GB_HIP(b, hipGetDeviceProperties(&prop, b->device), "device properties");
         const int cus = std::max(1, prop.multiProcessorCount);
         double best = 1e30;
-        const int chunks[] = {1, 2, 4, 8, 16, 32};
-        const int per_cu[] = {2, 4, 8, 16};
-        for (int c : chunks) {
-            gc_status st = build_schedule(b, c);
+        auto time_it = [&](double* ms) -> gc_status {
+            gc_status st = gc_batch_time_force_kernel(b, 1, ms, nullptr);  // warm
             if (st != GC_OK) return st;
-            for (int pc : per_cu) {
-                b->blocks = cus * pc;
-                double ms = 0.0;
-                st = gc_batch_time_force_kernel(b, 1, &ms, nullptr);  // warm
-                if (st != GC_OK) return st;
-                st = gc_batch_time_force_kernel(b, 3, &ms, nullptr);
-                if (st != GC_OK) return st;
-                if (ms < best) {
-                    best = ms;
-                    best_c = c;
-                    best_b = b->blocks;
+            return gc_batch_time_force_kernel(b, 3, ms, nullptr);
+        };
+        if (requested != GC_PAIRS_SYMMETRIC) {
+            b->pairs = GC_PAIRS_TARGETS;
+            const int chunks[] = {1, 2, 4, 8, 16, 32};
+            const int per_cu[] = {2, 4, 8, 16};
+            for (int c : chunks) {
+                gc_status st = build_schedule(b, c);
+                if (st != GC_OK) return b->pairs = requested, st;
+                for (int pc : per_cu) {
+                    b->blocks = cus * pc;
+                    double ms = 0.0;
+                    st = time_it(&ms);
+                    if (st != GC_OK) return b->pairs = requested, st;
+                    if (ms < best) {
+                        best = ms;
+                        best_t = Tuned{GC_PAIRS_TARGETS, c, b->blocks, 0, 0};
+                    }
                 }
             }
         }
+        b->pairs = requested;
+        if (requested != GC_PAIRS_TARGETS) {
+            b->pairs = GC_PAIRS_SYMMETRIC;
+            if (batch_symmetric(b)) {
+                gc_status st = ensure_batch_sym(b);
+                if (st != GC_OK) return b->pairs = requested, st;
+                const int ipts[] = {1, 2, 4};
+                const int per_cu[] = {0, 8, 16, 32};
+                for (int ipt : ipts) {
+                    for (int pc : per_cu) {
+                        b->sym_ipt = ipt;
+                        b->sym_blocks = cus * pc;
+                        double ms = 0.0;
+                        st = time_it(&ms);
+                        if (st != GC_OK) return b->pairs = requested, st;
+                        if (ms < best) {
+                            best = ms;
+                            best_t = Tuned{GC_PAIRS_SYMMETRIC, b->chunk, b->blocks, ipt,
+                                           b->sym_blocks};
+                        }
+                    }
+                }
+            }
+            b->pairs = requested;
+        }
         std::lock_guard<std::mutex> lock(g_tune_mutex);
-        g_tune_cache[key] = std::make_pair(best_c, best_b);
+        g_tune_cache[key] = best_t;
     }
-    gc_status st = build_schedule(b, best_c);
+    b->tuned_form = best_t.form;
+    gc_status st = build_schedule(b, best_t.chunk);
     if (st != GC_OK) return st;
-    b->blocks = best_b;
+    b->blocks = best_t.blocks;
+    b->sym_ipt = best_t.ipt;
+    b->sym_blocks = best_t.sym_blocks;
+    if (batch_symmetric(b) && (st = ensure_batch_sym(b)) != GC_OK) return st;
     drop_graph(b);
-    if (chunk_out) *chunk_out = best_c;
-    if (blocks_out) *blocks_out = best_b;
+    const bool sym = batch_symmetric(b);
+    if (chunk_out) *chunk_out = sym ? 0 : best_t.chunk;
+    if (blocks_out) *blocks_out = sym ? best_t.sym_blocks : best_t.blocks;
+    return GC_OK;
+}
+
+gc_status gc_batch_set_pairs(gc_batch* b, int mode, int targets_per_lane) {
+    if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
+    if (mode != GC_PAIRS_AUTO && mode != GC_PAIRS_TARGETS && mode != GC_PAIRS_SYMMETRIC)
+        return bfail(b, GC_ERR_INVALID_ARGUMENT, "pair mode must be 0, 1 or 2");
+    if (targets_per_lane != 0 && targets_per_lane != 1 && targets_per_lane != 2 &&
+        targets_per_lane != 4)
+        return bfail(b, GC_ERR_INVALID_ARGUMENT, "targets per lane must be 0, 1, 2 or 4");
+    GB_HIP(b, hipSetDevice(b->device), "select device");
+    b->pairs = mode;
+    b->tuned_form = 0;
+    b->sym_ipt = targets_per_lane;
+    if (batch_symmetric(b)) {
+        gc_status st = ensure_batch_sym(b);
+        if (st != GC_OK) return st;
+    }
+    drop_graph(b);
+    return GC_OK;
+}
+
+gc_status gc_batch_get_pairs(gc_batch* b, int* mode, int* targets_per_lane) {
+    if (!b || !mode || !targets_per_lane) return bfail(b, GC_ERR_INVALID_ARGUMENT, "null argument");
+    const bool sym = batch_symmetric(b);
+    *mode = sym ? GC_PAIRS_SYMMETRIC : GC_PAIRS_TARGETS;
+    *targets_per_lane = sym ? batch_sym_ipt(b) : 1;
     return GC_OK;
 }
 

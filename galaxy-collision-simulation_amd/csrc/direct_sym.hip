@@ -55,7 +55,8 @@ constexpr int kChunks = kTile / kChunk;   // 4
 // a step serve 2 * IPT interactions.
 template <int IPT, bool SAFE>
 __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
-    const SymItem* __restrict__ items, int nitems, int n, int ntiles, const double* __restrict__ x,
+    const SymItem* __restrict__ items, int nitems, int n, const int* __restrict__ tile_valid,
+    int ntiles, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ jp,
     const double* __restrict__ centre, float eps2, float* __restrict__ ipart,
     float* __restrict__ jpart) {
@@ -74,10 +75,11 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
         double xi[IPT], yi[IPT], zi[IPT];
         float mi[IPT];
         float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
+        const int valid = tile_valid ? tile_valid[I] : n - I * kTile;
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             const int i = I * kTile + k * kThreads + t;
-            const bool active = i < n;
+            const bool active = k * kThreads + t < valid;
             xi[k] = active ? x[i] : 0.0;
             yi[k] = active ? y[i] : 0.0;
             zi[k] = active ? z[i] : 0.0;
@@ -341,35 +343,37 @@ void sym_release(SymPlan& plan) {
 
 template <int IPT>
 void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const double* centre,
-                    int ntiles, const SymPlan& plan, float eps2, int blocks, hipStream_t st) {
+                    const int* tile_valid, int ntiles, const SymPlan& plan, float eps2, int blocks,
+                    hipStream_t st) {
     const int n = static_cast<int>(s.n);
     const dim3 grid(static_cast<unsigned>(blocks > 0 ? std::min(blocks, plan.nitems) : plan.nitems));
     if (safe) {
         hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), grid, dim3(kTile / IPT), 0, st,
-                           plan.items, plan.nitems, n, ntiles, s.x, s.y, s.z, jp, centre, eps2,
-                           plan.ipart, plan.jpart);
+                           plan.items, plan.nitems, n, tile_valid, ntiles, s.x, s.y, s.z, jp, centre,
+                           eps2, plan.ipart, plan.jpart);
     } else {
         hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), grid, dim3(kTile / IPT), 0, st,
-                           plan.items, plan.nitems, n, ntiles, s.x, s.y, s.z, jp, centre, eps2,
-                           plan.ipart, plan.jpart);
+                           plan.items, plan.nitems, n, tile_valid, ntiles, s.x, s.y, s.z, jp, centre,
+                           eps2, plan.ipart, plan.jpart);
     }
 }
 
 void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const double* centre,
-                             int ntiles, const SymPlan& plan, double eps, int ipt, int blocks,
-                             hipStream_t st) {
+                             const int* tile_valid, int ntiles, const SymPlan& plan, double eps,
+                             int ipt, int blocks, hipStream_t st) {
     const float eps2 = static_cast<float>(eps * eps);
     const bool safe = !(eps2 >= kTinySeparation2);
     switch (ipt) {
-    case 4: launch_sym_ipt<4>(safe, s, jp, centre, ntiles, plan, eps2, blocks, st); break;
-    case 2: launch_sym_ipt<2>(safe, s, jp, centre, ntiles, plan, eps2, blocks, st); break;
-    default: launch_sym_ipt<1>(safe, s, jp, centre, ntiles, plan, eps2, blocks, st); break;
+    case 4: launch_sym_ipt<4>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, st); break;
+    case 2: launch_sym_ipt<2>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, st); break;
+    default: launch_sym_ipt<1>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, st); break;
     }
 }
 
 void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
                        int ipt, hipStream_t st) {
-    launch_direct_sym_tiles(s, L.jp, L.centre, static_cast<int>(L.ntiles), plan, eps, ipt, 0, st);
+    launch_direct_sym_tiles(s, L.jp, L.centre, nullptr, static_cast<int>(L.ntiles), plan, eps, ipt,
+                            0, st);
 }
 
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,

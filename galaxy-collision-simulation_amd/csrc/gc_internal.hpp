@@ -90,10 +90,11 @@ void sym_release(SymPlan& plan);
 // ipt: targets per lane (1, 2, 4)
 void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
                        int ipt, hipStream_t st);
+// tile_valid: valid targets per tile, or null for the first s.n slots;
 // blocks > 0: persistent grid of that many blocks over the items
 void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const double* centre,
-                             int ntiles, const SymPlan& plan, double eps, int ipt, int blocks,
-                             hipStream_t st);
+                             const int* tile_valid, int ntiles, const SymPlan& plan, double eps,
+                             int ipt, int blocks, hipStream_t st);
 // tile_valid: valid targets per tile, or null for the first s.n slots
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
                       bool close_kick, double half_dt, hipStream_t st);

@@ -122,6 +122,8 @@ SIGNATURES = {
     "gc_get_sources": (C.c_int, [C.c_void_p]),
     "gc_set_pairs": (C.c_int, [C.c_void_p, C.c_int]),
     "gc_get_pairs": (C.c_int, [C.c_void_p]),
+    "gc_batch_set_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "gc_batch_get_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),
     "gc_generate_disks": (C.c_size_t, [C.POINTER(DiskGalaxy), C.c_int, C.c_double, C.c_double,
                                        C.c_uint64, _dp, _dp, _dp]),

@@ -2,7 +2,7 @@
 
     b = Batch(counts, softening=0.05)
     b.upload(pos, vel, mass)        # runs concatenated in order
-    b.autotune()                    # (chunk tiles, persistent blocks)
+    b.autotune()                    # pair form and its schedule
     b.compute_accelerations()
     b.step(1e-3, 500)               # hipGraph chunks of graph_steps steps
     pos, vel, acc = b.download()
@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
-from .engine import make_params
+from .engine import PAIRS, make_params
 
 _dp = C.POINTER(C.c_double)
 
@@ -90,6 +90,18 @@ class Batch:
         self._check(self._lib.gc_batch_get_schedule(self._b, C.byref(a), C.byref(b), C.byref(c)),
                     "batch schedule")
         return a.value, b.value, c.value
+
+    def set_pairs(self, mode="auto", targets_per_lane=0):
+        """Pair coverage within each run: "auto", "targets" (scheduled
+        per-target kernel) or "symmetric" (each unordered pair once)."""
+        self._check(self._lib.gc_batch_set_pairs(self._b, PAIRS[mode], targets_per_lane),
+                    "batch pairs")
+
+    def pairs(self):
+        """(form in use, targets per lane)."""
+        m, t = C.c_int(), C.c_int()
+        self._check(self._lib.gc_batch_get_pairs(self._b, C.byref(m), C.byref(t)), "batch pairs")
+        return {L.GC_PAIRS_TARGETS: "targets", L.GC_PAIRS_SYMMETRIC: "symmetric"}[m.value], t.value
 
     def autotune(self):
         a, b = C.c_int(), C.c_int()

@@ -188,12 +188,16 @@ gc_status gc_batch_download(gc_batch* b, double* pos, double* vel, double* acc);
 gc_status gc_batch_compute_accelerations(gc_batch* b);
 gc_status gc_batch_step(gc_batch* b, double dt, int nsteps);
 gc_status gc_batch_synchronize(gc_batch* b);
-/* chunk_tiles: source tiles (of 256 particles) per work item; blocks:
- * persistent workgroups; graph_steps: steps per captured graph (0/1: none). */
+/* chunk_tiles: source tiles (of 256 particles) per work item of the
+ * per-target form; blocks: persistent workgroups of the pair form in use
+ * (gc_batch_set_pairs; for SYMMETRIC 0 means one per work item);
+ * graph_steps: steps per captured graph (0/1: none).  get reports
+ * chunk_tiles 0 while the SYMMETRIC form is in use. */
 gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps);
 gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps);
-/* Times (chunk_tiles, blocks) candidates on this batch's force kernel and
- * keeps the fastest; results are cached per process by batch shape. */
+/* Times schedule candidates on this batch's force kernel and keeps the
+ * fastest; results are cached per process by batch shape and requested pair
+ * form (gc_batch_set_pairs). */
 gc_status gc_batch_autotune(gc_batch* b, int* chunk_tiles, int* blocks);
 gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms_per_launch,
                                      double* interactions_per_launch);
@@ -247,12 +251,24 @@ int gc_get_sources(const gc_ctx* ctx);
  * 262,144 particles, target ranges always run TARGETS.  AUTO (the
  * default) is SYMMETRIC where it applies.  Both meet the same error bound;
  * sums differ by rounding.  gc_set_launch_shape's targets_per_thread also
- * sets the SYMMETRIC kernel's targets per lane (default 2).  gc_get_pairs
- * returns the form a full-range force of the resident particles uses (1 or
- * 2), -1 for a null context. */
+ * sets the SYMMETRIC kernel's targets per lane (default 4, or 2 or 1 when
+ * that leaves fewer than 8192 waves).  gc_get_pairs returns the form a
+ * full-range force of the resident particles uses (1 or 2), -1 for a null
+ * context. */
 enum { GC_PAIRS_AUTO = 0, GC_PAIRS_TARGETS = 1, GC_PAIRS_SYMMETRIC = 2 };
 gc_status gc_set_pairs(gc_ctx* ctx, int mode);
 int gc_get_pairs(const gc_ctx* ctx);
+
+/* The same choice for a batch: SYMMETRIC covers each unordered pair of a
+ * run once (runs of up to 262,144 particles; AUTO is SYMMETRIC when every
+ * run fits), TARGETS is the scheduled per-target kernel above.
+ * targets_per_lane: 1, 2, 4 or 0 (automatic, as for a context).
+ * gc_batch_get_pairs reports the form in use and its targets per lane.
+ * gc_batch_autotune times both forms unless one was set explicitly; its
+ * chunk_tiles output is 0 when SYMMETRIC wins and blocks is then the
+ * symmetric kernel's persistent blocks (0: one per work item). */
+gc_status gc_batch_set_pairs(gc_batch* b, int mode, int targets_per_lane);
+gc_status gc_batch_get_pairs(gc_batch* b, int* mode, int* targets_per_lane);
 
 
 /* ---- synthetic initial conditions (host side) -------------------------

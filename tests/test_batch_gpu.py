@@ -1,7 +1,7 @@
 """Batched stepping: each run of a batch is bit-identical to the same run
-stepped alone through hip-direct (mixed) with the matching source split, the
-graph-replayed steps equal eager steps, and the autotuner's choice keeps
-results within the mixed-precision bound."""
+stepped alone through hip-direct (mixed) with the matching pair form and
+launch shape, the graph-replayed steps equal eager steps, and the
+autotuner's choice keeps results within the mixed-precision bound."""
 import numpy as np
 import pytest
 
@@ -21,13 +21,17 @@ def _cat(parts):
     return tuple(np.concatenate([p[i] for p in parts]) for i in range(3))
 
 
-def _alone(gpu, pos, vel, m, chunk, steps, dt=1e-3):
+def _alone(gpu, pos, vel, m, chunk, steps, dt=1e-3, ipt=None):
     ntiles = (len(m) + 255) // 256
     with gpu.Engine("hip-direct", "mixed", softening=0.05) as eng:
-        # the batch kernel runs the per-target form with LDS source tiles
-        eng.set_pairs("targets")
-        eng.set_sources("lds")
-        eng.set_launch_shape(1, (ntiles + chunk - 1) // chunk)
+        if ipt is None:
+            # the scheduled batch kernel: per-target form, LDS source tiles
+            eng.set_pairs("targets")
+            eng.set_sources("lds")
+            eng.set_launch_shape(1, (ntiles + chunk - 1) // chunk)
+        else:
+            eng.set_pairs("symmetric")
+            eng.set_launch_shape(ipt, 1)
         eng.upload(pos, vel, m)
         eng.compute_accelerations()
         eng.step(dt, steps)
@@ -40,6 +44,7 @@ def test_batched_runs_bit_identical_to_single_runs(gpu, chunk):
 
     specs, parts = _runs(gpu, 6)
     with Batch([len(p[2]) for p in parts], softening=0.05) as b:
+        b.set_pairs("targets")
         b.set_schedule(chunk, 97, graph_steps=0)
         b.upload(*_cat(parts))
         b.compute_accelerations()
@@ -51,7 +56,32 @@ def test_batched_runs_bit_identical_to_single_runs(gpu, chunk):
             assert np.array_equal(b.split(g)[r], w), (r, chunk)
 
 
-def test_graph_replay_equals_eager(gpu):
+@pytest.mark.parametrize("ipt,blocks", [(1, 0), (2, 97), (4, 0)])
+def test_symmetric_batch_bit_identical_to_single_runs(gpu, ipt, blocks):
+    from gcsim.batch import Batch
+
+    # runs of 2-12 tiles plus one of ~190 tiles, whose rows split into
+    # multi-tile work items
+    specs, parts = _runs(gpu, 5, seed=11)
+    big = gpu.generators.collisions(1, seed=12, n_min=48000, n_max=49000)[0]
+    parts.insert(2, gpu.generators.collision_particles(big, softening=0.05))
+    with Batch([len(p[2]) for p in parts], softening=0.05) as b:
+        b.set_pairs("symmetric", ipt)
+        b.set_schedule(4, blocks, graph_steps=0)
+        assert b.pairs() == ("symmetric", ipt)
+        assert b.schedule() == (0, blocks, 0)
+        b.upload(*_cat(parts))
+        b.compute_accelerations()
+        b.step(1e-3, 5)
+        got = b.download()
+    for r, p in enumerate(parts):
+        want = _alone(gpu, *p, 0, 5, ipt=ipt)
+        for g, w in zip(got, want):
+            assert np.array_equal(b.split(g)[r], w), (r, ipt)
+
+
+@pytest.mark.parametrize("pairs", ["targets", "symmetric"])
+def test_graph_replay_equals_eager(gpu, pairs):
     from gcsim.batch import Batch
 
     specs, parts = _runs(gpu, 5, seed=7)
@@ -59,6 +89,7 @@ def test_graph_replay_equals_eager(gpu):
     out = []
     for graph_steps in (0, 10, 4):
         with Batch([len(p[2]) for p in parts], softening=0.05) as b:
+            b.set_pairs(pairs)
             b.set_schedule(4, 512, graph_steps=graph_steps)
             b.upload(*data)
             b.compute_accelerations()
@@ -70,16 +101,23 @@ def test_graph_replay_equals_eager(gpu):
             assert np.array_equal(g, w)
 
 
-def test_autotune_and_forces_against_oracle(gpu):
+@pytest.mark.parametrize("pairs", ["auto", "targets"])
+def test_autotune_and_forces_against_oracle(gpu, pairs):
     from gcsim.batch import Batch
 
     specs, parts = _runs(gpu, 12, seed=3)
     with Batch([len(p[2]) for p in parts], softening=0.05) as b:
+        b.set_pairs(pairs)
         b.upload(*_cat(parts))
         chunk, blocks = b.autotune()
-        assert chunk >= 1 and blocks >= 1
+        form, ipt = b.pairs()
+        if form == "targets":
+            assert chunk >= 1 and blocks >= 1 and ipt == 1
+        else:
+            assert pairs == "auto" and chunk == 0 and blocks >= 0 and ipt in (1, 2, 4)
         assert b.schedule()[:2] == (chunk, blocks)
         assert b.autotune() == (chunk, blocks)  # cached
+        assert b.pairs() == (form, ipt)
         b.compute_accelerations()
         acc = b.download()[2]
         ms, pairs = b.time_force_kernel(3)
@@ -89,16 +127,23 @@ def test_autotune_and_forces_against_oracle(gpu):
         assert rel_err(b.split(acc)[r], ref).max() <= 1e-5
 
 
-def test_single_particle_runs_and_bad_input(gpu):
+@pytest.mark.parametrize("pairs", ["targets", "symmetric"])
+def test_single_particle_runs_and_bad_input(gpu, pairs):
     from gcsim.batch import Batch
 
     with Batch([1, 255, 256, 257], softening=0.05) as b:
+        b.set_pairs(pairs)
         rng = np.random.default_rng(1)
         n = b.n
-        b.upload(rng.normal(size=(n, 3)), None, rng.uniform(0.5, 1, n))
+        pos, mass = rng.normal(size=(n, 3)), rng.uniform(0.5, 1, n)
+        b.upload(pos, None, mass)
         b.compute_accelerations()
         acc = b.download()[2]
         assert not acc[0].any()  # a lone particle feels nothing
         assert np.isfinite(acc).all()
+        for p, m, a in zip(b.split(pos)[1:], b.split(mass)[1:], b.split(acc)[1:]):
+            assert rel_err(a, O.direct_targets(p, m, 1.0, 0.05)).max() <= 1e-5
+        with pytest.raises(gpu.GcError):
+            b.set_pairs("symmetric", 3)
     with pytest.raises(gpu.GcError):
         Batch([0, 5])
