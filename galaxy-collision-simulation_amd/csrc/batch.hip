@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void batch_merge_kernel(
 struct Tuned {
     int form;                  // GC_PAIRS_TARGETS or GC_PAIRS_SYMMETRIC
     int chunk, blocks;         // per-target form
-    int ipt, sym_blocks;       // symmetric form
+    int ipt, sym_blocks, sym_len;  // symmetric form
 };
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, long long, int>, Tuned> g_tune_cache;
@@ -199,6 +199,7 @@ struct gc_batch {
     // each unordered pair once within a run (direct_sym.hip): pair form,
     // targets per lane (0 = automatic), persistent blocks (0 = one per item)
     int pairs = GC_PAIRS_AUTO, sym_ipt = 0, sym_blocks = 0;
+    int sym_len = 0, sym_built_len = -1;  // J tiles per item (0: default), plan's
     int tuned_form = 0;  // AUTO after autotune: GC_PAIRS_TARGETS or _SYMMETRIC
     SymPlan sym;
     hipGraphExec_t graph = nullptr;
@@ -241,13 +242,18 @@ bool batch_symmetric(const gc_batch* b) {
     return true;
 }
 
+// The work list for the current J tiles per item; a new list drops the
+// captured graph.
 gc_status ensure_batch_sym(gc_batch* b) {
-    if (b->sym.items) return GC_OK;
+    if (b->sym.items && b->sym_built_len == b->sym_len) return GC_OK;
+    drop_graph(b);
+    b->sym_built_len = -1;
     std::string msg;
     const int rc = sym_prepare_runs(b->sym, b->run_tile0.data(), b->run_ntiles.data(), b->runs,
-                                    b->stream, &msg);
-    if (rc == 0) return GC_OK;
-    return bfail(b, rc == 2 ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP, msg);
+                                    b->sym_len, b->stream, &msg);
+    if (rc != 0) return bfail(b, rc == 2 ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP, msg);
+    b->sym_built_len = b->sym_len;
+    return GC_OK;
 }
 
 // Targets per lane: as set, else the largest of 4, 2, 1 that still gives
@@ -562,16 +568,23 @@ gc_status gc_batch_synchronize(gc_batch* b) {
 gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps) {
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     const bool sym = batch_symmetric(b);
-    if (chunk_tiles < 1 || chunk_tiles > 4096 || blocks < (sym ? 0 : 1) || blocks > 1 << 20 ||
-        graph_steps < 0 || graph_steps > 1000)
+    if (chunk_tiles < (sym ? 0 : 1) || chunk_tiles > 4096 || blocks < (sym ? 0 : 1) ||
+        blocks > 1 << 20 || graph_steps < 0 || graph_steps > 1000)
         return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_set_schedule: out of range");
     GB_HIP(b, hipSetDevice(b->device), "select device");
-    if (chunk_tiles != b->chunk) {
-        gc_status st = build_schedule(b, chunk_tiles);
-        if (st != GC_OK) return st;
-    }
     drop_graph(b);
-    (sym ? b->sym_blocks : b->blocks) = blocks;
+    if (sym) {
+        b->sym_len = chunk_tiles;
+        b->sym_blocks = blocks;
+        gc_status st = ensure_batch_sym(b);
+        if (st != GC_OK) return st;
+    } else {
+        if (chunk_tiles != b->chunk) {
+            gc_status st = build_schedule(b, chunk_tiles);
+            if (st != GC_OK) return st;
+        }
+        b->blocks = blocks;
+    }
     b->graph_steps = graph_steps;
     return GC_OK;
 }
@@ -579,7 +592,7 @@ gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int gr
 gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps) {
     if (!b || !chunk_tiles || !blocks || !graph_steps) return bfail(b, GC_ERR_INVALID_ARGUMENT, "null argument");
     const bool sym = batch_symmetric(b);
-    *chunk_tiles = sym ? 0 : b->chunk;
+    *chunk_tiles = sym ? b->sym_len : b->chunk;
     *blocks = sym ? b->sym_blocks : b->blocks;
     *graph_steps = b->graph_steps;
     return GC_OK;
@@ -634,7 +647,7 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
     for (int r = 0; r < b->runs; ++r) items1 += static_cast<long long>(b->run_ntiles[r]) * b->run_ntiles[r];
     const auto key = std::make_tuple(b->runs, b->ntiles, items1, b->pairs);
     Tuned best_t{batch_symmetric(b) ? GC_PAIRS_SYMMETRIC : GC_PAIRS_TARGETS, b->chunk, b->blocks,
-                 b->sym_ipt, b->sym_blocks};
+                 b->sym_ipt, b->sym_blocks, b->sym_len};
     bool cached = false;
     {
         std::lock_guard<std::mutex> lock(g_tune_mutex);
@@ -669,7 +682,7 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
                     if (st != GC_OK) return b->pairs = requested, st;
                     if (ms < best) {
                         best = ms;
-                        best_t = Tuned{GC_PAIRS_TARGETS, c, b->blocks, 0, 0};
+                        best_t = Tuned{GC_PAIRS_TARGETS, c, b->blocks, 0, 0, 0};
                     }
                 }
             }
@@ -678,21 +691,26 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
         if (requested != GC_PAIRS_TARGETS) {
             b->pairs = GC_PAIRS_SYMMETRIC;
             if (batch_symmetric(b)) {
-                gc_status st = ensure_batch_sym(b);
-                if (st != GC_OK) return b->pairs = requested, st;
+                gc_status st = GC_OK;
                 const int ipts[] = {1, 2, 4};
-                const int per_cu[] = {0, 8, 16, 32};
-                for (int ipt : ipts) {
-                    for (int pc : per_cu) {
-                        b->sym_ipt = ipt;
-                        b->sym_blocks = cus * pc;
-                        double ms = 0.0;
-                        st = time_it(&ms);
-                        if (st != GC_OK) return b->pairs = requested, st;
-                        if (ms < best) {
-                            best = ms;
-                            best_t = Tuned{GC_PAIRS_SYMMETRIC, b->chunk, b->blocks, ipt,
-                                           b->sym_blocks};
+                const int lens[] = {0, 1, 2, 4};
+                const int per_cu[] = {0, 16};
+                for (int len : lens) {
+                    b->sym_len = len;
+                    st = ensure_batch_sym(b);
+                    if (st != GC_OK) return b->pairs = requested, st;
+                    for (int ipt : ipts) {
+                        for (int pc : per_cu) {
+                            b->sym_ipt = ipt;
+                            b->sym_blocks = cus * pc;
+                            double ms = 0.0;
+                            st = time_it(&ms);
+                            if (st != GC_OK) return b->pairs = requested, st;
+                            if (ms < best) {
+                                best = ms;
+                                best_t = Tuned{GC_PAIRS_SYMMETRIC, b->chunk, b->blocks, ipt,
+                                               b->sym_blocks, len};
+                            }
                         }
                     }
                 }
@@ -708,10 +726,11 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
     b->blocks = best_t.blocks;
     b->sym_ipt = best_t.ipt;
     b->sym_blocks = best_t.sym_blocks;
+    b->sym_len = best_t.sym_len;
     if (batch_symmetric(b) && (st = ensure_batch_sym(b)) != GC_OK) return st;
     drop_graph(b);
     const bool sym = batch_symmetric(b);
-    if (chunk_out) *chunk_out = sym ? 0 : best_t.chunk;
+    if (chunk_out) *chunk_out = sym ? best_t.sym_len : best_t.chunk;
     if (blocks_out) *blocks_out = sym ? best_t.sym_blocks : best_t.blocks;
     return GC_OK;
 }

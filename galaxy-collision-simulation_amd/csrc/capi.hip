@@ -160,8 +160,8 @@ bool symmetric_pairs(const gc_ctx* c, size_t begin, size_t end) {
 
 // Targets per lane of the symmetric kernel unless set: the largest of 4, 2,
 // 1 that still launches >= 8192 waves (8 per SIMD).  Measured at N = 65,536
-// (8,320 items): 0.886 ms per launch with 4, 0.950 ms with 2, 1.067 ms
-// with 1.
+// (16,512 items of 2 source tiles): 0.853 ms per launch with 4 per lane,
+// 0.952 ms with 2; items of 4 tiles took 0.886 ms with 4.
 int sym_ipt(const gc_ctx* c) {
     const int ipt = c->shape.ipt;
     if (ipt == 1 || ipt == 2 || ipt == 4) return ipt;

@@ -178,146 +178,118 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
 }
 
 // a_i = sum over the items of row T (J order) of (hi + lo) + sum over I < T
-// of the (I, T) pair partials, in FP64; optional closing half kick.  Four
-// thread groups per target split the column: group g adds the pair partials
-// of I in [g T / 4, (g + 1) T / 4) (group 0 also the row items) and the four
-// group sums are added in group order, so the result is deterministic.
-// Blocks run the longest columns (largest T) first.
-constexpr int kMergeGroups = 4;
+// of the (I, T) pair partials, in FP64; optional closing half kick.  A
+// block of 16 groups of 64 lanes merges tile T, 4 targets per lane (float4
+// loads); group g adds the g-th of 16 consecutive ranges of the row's
+// items and of the column's pair partials, and the 16 group sums are added
+// in group order, so the result is deterministic.  With items of 2 source
+// tiles a row's item partials and a column's pair partials are about the
+// same bytes, ~3 KB per source tile.  Blocks run the longest columns
+// (largest T) first.
+constexpr int kMergeGroups = 16;
+constexpr int kMergeLanes = kTile / 4;  // 64
 
-__global__ __launch_bounds__(kTile * kMergeGroups) void merge_sym_kernel(
+__global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
     int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
     const long long* __restrict__ tile_pbase, const int* __restrict__ tile_valid,
     const float* __restrict__ ipart, const float* __restrict__ jpart, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
     double* __restrict__ vy, double* __restrict__ vz, int close_kick, double half_dt) {
 #pragma clang fp contract(off)
-    __shared__ double part[kMergeGroups - 1][3][kTile];
+    __shared__ double part[kMergeGroups][3][kTile];
     const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
-    const int l = threadIdx.x % kTile;
-    const int g = threadIdx.x / kTile;
-    double a[3] = {0.0, 0.0, 0.0};
-    if (g == 0) {
-        for (int s = row_start[T]; s < row_start[T + 1]; ++s) {
-            const float* b = ipart + static_cast<size_t>(s) * 6 * kTile;
+    const int lane = threadIdx.x % kMergeLanes;
+    const int g = threadIdx.x / kMergeLanes;
+    double a[3][4];
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                a[c] += static_cast<double>(b[c * kTile + l]) +
-                        static_cast<double>(b[(c + 3) * kTile + l]);
+    for (int c = 0; c < 3; ++c) a[c][0] = a[c][1] = a[c][2] = a[c][3] = 0.0;
+    // row: item partials (hi, lo) of row T
+    const int r0 = row_start[T], nr = row_start[T + 1] - r0;
+    const int s1 = r0 + (g + 1) * nr / kMergeGroups;
+    for (int s = r0 + g * nr / kMergeGroups; s < s1; ++s) {
+        const float* b = ipart + static_cast<size_t>(s) * 6 * kTile + 4 * lane;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float4 h = *reinterpret_cast<const float4*>(b + c * kTile);
+            const float4 lo = *reinterpret_cast<const float4*>(b + (c + 3) * kTile);
+            a[c][0] += static_cast<double>(h.x) + static_cast<double>(lo.x);
+            a[c][1] += static_cast<double>(h.y) + static_cast<double>(lo.y);
+            a[c][2] += static_cast<double>(h.z) + static_cast<double>(lo.z);
+            a[c][3] += static_cast<double>(h.w) + static_cast<double>(lo.w);
         }
     }
     // column: pair partials (I, T) for I in [t0, T) of T's run
     const int t0 = tile_t0[T], cols = T - t0;
-    const float* col = jpart + (tile_pbase[T] + pair_slot(0, cols)) * 3 * kTile;
-    const int i0 = g * cols / kMergeGroups, i1 = (g + 1) * cols / kMergeGroups;
-    for (int I = i0; I < i1; ++I) {
+    const float* col = jpart + (tile_pbase[T] + pair_slot(0, cols)) * 3 * kTile + 4 * lane;
+    const int i1 = (g + 1) * cols / kMergeGroups;
+    for (int I = g * cols / kMergeGroups; I < i1; ++I) {
         const float* b = col + static_cast<size_t>(I) * 3 * kTile;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a[c] += static_cast<double>(b[c * kTile + l]);
+        for (int c = 0; c < 3; ++c) {
+            const float4 v = *reinterpret_cast<const float4*>(b + c * kTile);
+            a[c][0] += static_cast<double>(v.x);
+            a[c][1] += static_cast<double>(v.y);
+            a[c][2] += static_cast<double>(v.z);
+            a[c][3] += static_cast<double>(v.w);
+        }
     }
-    if (g > 0) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) part[g - 1][c][l] = a[c];
-    }
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[g][c][4 * lane + k] = a[c][k];
     __syncthreads();
-    if (g > 0) return;
+    // thread (c, e) of the first 3 * kTile adds target e's component c
+    const int t = threadIdx.x;
+    if (t >= 3 * kTile) return;
+    const int c = t / kTile, e = t % kTile;
+    double sum = 0.0;
 #pragma unroll
-    for (int k = 0; k < kMergeGroups - 1; ++k) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) a[c] += part[k][c][l];
-    }
+    for (int k = 0; k < kMergeGroups; ++k) sum += part[k][c][e];
     const int valid = tile_valid ? tile_valid[T] : n - T * kTile;
-    if (l >= valid) return;
-    const int i = T * kTile + l;
-    ax[i] = a[0];
-    ay[i] = a[1];
-    az[i] = a[2];
-    if (close_kick) {
-        vx[i] = vx[i] + a[0] * half_dt;
-        vy[i] = vy[i] + a[1] * half_dt;
-        vz[i] = vz[i] + a[2] * half_dt;
-    }
+    if (e >= valid) return;
+    const int i = T * kTile + e;
+    double* acc = c == 0 ? ax : (c == 1 ? ay : az);
+    double* vel = c == 0 ? vx : (c == 1 ? vy : vz);
+    acc[i] = sum;
+    if (close_kick) vel[i] = vel[i] + sum * half_dt;
 }
 
 }  // namespace
 
-int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns,
+int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns, int len,
                      hipStream_t st, std::string* err) {
-    int ntiles = 0;
-    for (int r = 0; r < nruns; ++r) ntiles = std::max(ntiles, run_t0[r] + run_nt[r]);
     if (plan.items) (void)hipStreamSynchronize(st);
     sym_release(plan);
-    if (ntiles < 1) {
-        *err = "symmetric form: no tiles";
-        return 1;
-    }
-    // per run: items of `len` consecutive J tiles per row, ~8192 items per
-    // run of 128+ tiles; row slots in (run, row, J) order; pair partials per
-    // run from that run's base
-    std::vector<SymItem> items;
-    std::vector<int> row_start(ntiles + 1, 0), tile_t0(ntiles, 0);
-    std::vector<long long> tile_pbase(ntiles, 0);
-    long long pbase = 0;
-    int next_tile = 0;
-    for (int r = 0; r < nruns; ++r) {
-        const int t0 = run_t0[r], nt = run_nt[r];
-        if (nt > kSymMaxTiles) {
-            *err = "symmetric form: run of more than 1024 tiles";
-            return 1;
-        }
-        for (int T = next_tile; T < t0; ++T) row_start[T] = static_cast<int>(items.size());
-        const size_t tri = static_cast<size_t>(nt) * (nt + 1) / 2;
-        const int len = static_cast<int>(std::max<size_t>(1, tri / 8192));
-        for (int I = t0; I < t0 + nt; ++I) {
-            row_start[I] = static_cast<int>(items.size());
-            tile_t0[I] = t0;
-            tile_pbase[I] = pbase;
-            for (int J = I; J < t0 + nt; J += len) {
-                SymItem it{};
-                it.I = I;
-                it.jb = J;
-                it.je = std::min(J + len, t0 + nt);
-                it.islot = static_cast<int>(items.size());
-                it.pbase = pbase;
-                it.t0 = t0;
-                items.push_back(it);
-            }
-        }
-        pbase += static_cast<long long>(pair_slot(0, nt));
-        next_tile = t0 + nt;
-    }
-    for (int T = next_tile; T <= ntiles; ++T) row_start[T] = static_cast<int>(items.size());
-    // longest items first; ties keep (run, row) order
-    std::stable_sort(items.begin(), items.end(),
-                     [](const SymItem& a, const SymItem& b) { return (a.je - a.jb) > (b.je - b.jb); });
-    plan.nitems = static_cast<int>(items.size());
+    SymHostPlan h;
+    if (!build_sym_plan(run_t0, run_nt, nruns, len, kSymMaxTiles, h, err)) return 1;
+    plan.nitems = static_cast<int>(h.items.size());
     plan.ipart_floats = static_cast<size_t>(plan.nitems) * 6 * kTile;
-    plan.jpart_floats = std::max<size_t>(1, static_cast<size_t>(pbase)) * 3 * kTile;
-    hipError_t e = hipMalloc(&plan.items, sizeof(SymItem) * items.size());
-    if (e == hipSuccess) e = hipMalloc(&plan.row_start, sizeof(int) * row_start.size());
-    if (e == hipSuccess) e = hipMalloc(&plan.tile_t0, sizeof(int) * tile_t0.size());
-    if (e == hipSuccess) e = hipMalloc(&plan.tile_pbase, sizeof(long long) * tile_pbase.size());
+    plan.jpart_floats = std::max<size_t>(1, static_cast<size_t>(h.pairs)) * 3 * kTile;
+    hipError_t e = hipMalloc(&plan.items, sizeof(SymItem) * h.items.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.row_start, sizeof(int) * h.row_start.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.tile_t0, sizeof(int) * h.tile_t0.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.tile_pbase, sizeof(long long) * h.tile_pbase.size());
     if (e == hipSuccess) e = hipMalloc(&plan.ipart, sizeof(float) * plan.ipart_floats);
     if (e == hipSuccess) e = hipMalloc(&plan.jpart, sizeof(float) * plan.jpart_floats);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(plan.items, items.data(), sizeof(SymItem) * items.size(),
+        e = hipMemcpyAsync(plan.items, h.items.data(), sizeof(SymItem) * h.items.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(plan.row_start, row_start.data(), sizeof(int) * row_start.size(),
+        e = hipMemcpyAsync(plan.row_start, h.row_start.data(), sizeof(int) * h.row_start.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(plan.tile_t0, tile_t0.data(), sizeof(int) * tile_t0.size(),
+        e = hipMemcpyAsync(plan.tile_t0, h.tile_t0.data(), sizeof(int) * h.tile_t0.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(plan.tile_pbase, tile_pbase.data(), sizeof(long long) * tile_pbase.size(),
-                           hipMemcpyHostToDevice, st);
+        e = hipMemcpyAsync(plan.tile_pbase, h.tile_pbase.data(),
+                           sizeof(long long) * h.tile_pbase.size(), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);  // host vectors end here
     if (e != hipSuccess) {
         *err = std::string("symmetric form buffers: ") + hipGetErrorString(e);
         sym_release(plan);
         return e == hipErrorOutOfMemory ? 2 : 3;
     }
-    plan.ntiles = ntiles;
+    plan.ntiles = h.ntiles;
     return 0;
 }
 
@@ -328,7 +300,7 @@ int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err) {
         return 1;
     }
     const int t0 = 0;
-    return sym_prepare_runs(plan, &t0, &ntiles, 1, st, err);
+    return sym_prepare_runs(plan, &t0, &ntiles, 1, 0, st, err);
 }
 
 void sym_release(SymPlan& plan) {
@@ -378,7 +350,7 @@ void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan
 
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
                       bool close_kick, double half_dt, hipStream_t st) {
-    hipLaunchKernelGGL(merge_sym_kernel, dim3(plan.ntiles), dim3(kTile * kMergeGroups), 0, st,
+    hipLaunchKernelGGL(merge_sym_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups), 0, st,
                        static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
                        plan.tile_pbase, tile_valid, plan.ipart, plan.jpart, s.ax, s.ay, s.az, s.vx,
                        s.vy, s.vz, close_kick ? 1 : 0, half_dt);

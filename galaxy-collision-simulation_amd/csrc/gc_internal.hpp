@@ -8,6 +8,7 @@
 #include <string>
 
 #include "gcsim.h"
+#include "sym_plan.hpp"
 
 namespace gc {
 
@@ -43,13 +44,6 @@ struct MixedLayout {
 // Symmetric (each unordered pair once) mixed form: the work list over the
 // upper triangle of source-tile pairs and its partial sums.
 constexpr int kSymMaxTiles = 1024;  // N <= 262,144: pair partials <= 1.6 GB
-// One work item: row tile I, source tiles [jb, je), its row-partial slot;
-// pair partials of its run start at pbase, the run's first tile is t0.
-struct SymItem {
-    int I, jb, je, islot;
-    long long pbase;
-    int t0, pad;
-};
 struct SymPlan {
     int ntiles = 0;             // plan built for this tile count
     int nitems = 0;
@@ -84,7 +78,8 @@ void launch_merge_mixed(const DeviceState& s, const MixedLayout& L, int jsplit, 
 int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err);
 // independent runs of tiles [run_t0[r], run_t0[r] + run_nt[r]) (pairs only
 // within a run)
-int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns,
+// len: J tiles per item (0: the default of sym_plan.hpp)
+int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns, int len,
                      hipStream_t st, std::string* err);
 void sym_release(SymPlan& plan);
 // ipt: targets per lane (1, 2, 4)

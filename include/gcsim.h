@@ -188,11 +188,12 @@ gc_status gc_batch_download(gc_batch* b, double* pos, double* vel, double* acc);
 gc_status gc_batch_compute_accelerations(gc_batch* b);
 gc_status gc_batch_step(gc_batch* b, double dt, int nsteps);
 gc_status gc_batch_synchronize(gc_batch* b);
-/* chunk_tiles: source tiles (of 256 particles) per work item of the
- * per-target form; blocks: persistent workgroups of the pair form in use
- * (gc_batch_set_pairs; for SYMMETRIC 0 means one per work item);
- * graph_steps: steps per captured graph (0/1: none).  get reports
- * chunk_tiles 0 while the SYMMETRIC form is in use. */
+/* Schedule of the pair form in use (gc_batch_set_pairs).  Per target:
+ * chunk_tiles = source tiles (of 256 particles) per work item (>= 1),
+ * blocks = persistent workgroups (>= 1).  Symmetric: chunk_tiles = source
+ * tiles per work item of a row (0: about 16,384 items per run), blocks =
+ * persistent workgroups (0: one per work item).  graph_steps: steps per
+ * captured graph (0/1: none). */
 gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps);
 gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps);
 /* Times schedule candidates on this batch's force kernel and keeps the
@@ -264,9 +265,8 @@ int gc_get_pairs(const gc_ctx* ctx);
  * run fits), TARGETS is the scheduled per-target kernel above.
  * targets_per_lane: 1, 2, 4 or 0 (automatic, as for a context).
  * gc_batch_get_pairs reports the form in use and its targets per lane.
- * gc_batch_autotune times both forms unless one was set explicitly; its
- * chunk_tiles output is 0 when SYMMETRIC wins and blocks is then the
- * symmetric kernel's persistent blocks (0: one per work item). */
+ * gc_batch_autotune times both forms unless one was set explicitly and
+ * reports the winner's schedule as gc_batch_get_schedule does. */
 gc_status gc_batch_set_pairs(gc_batch* b, int mode, int targets_per_lane);
 gc_status gc_batch_get_pairs(gc_batch* b, int* mode, int* targets_per_lane);
 

@@ -1,7 +1,8 @@
 """Batched stepping: each run of a batch is bit-identical to the same run
 stepped alone through hip-direct (mixed) with the matching pair form and
-launch shape, the graph-replayed steps equal eager steps, and the
-autotuner's choice keeps results within the mixed-precision bound."""
+launch shape; every schedule is deterministic and within the
+mixed-precision bound of the oracle; graph-replayed steps equal eager
+steps, and the autotuner's choice keeps results within the bound."""
 import numpy as np
 import pytest
 
@@ -15,6 +16,18 @@ def _runs(gpu, k, seed=2026):
     specs = gpu.generators.collisions(k, seed=seed, n_min=512, n_max=3000)
     parts = [gpu.generators.collision_particles(s, softening=0.05) for s in specs]
     return specs, parts
+
+
+_ORACLE = {}
+
+
+def _oracle_acc(p):
+    """FP64 oracle forces of one run (cached: the ~48K-particle run takes
+    seconds on all host cores)."""
+    key = (len(p[2]), p[0][:2].tobytes())
+    if key not in _ORACLE:
+        _ORACLE[key] = O.direct_targets(p[0], p[2], 1.0, 0.05, threads=0)
+    return _ORACLE[key]
 
 
 def _cat(parts):
@@ -67,7 +80,7 @@ def test_symmetric_batch_bit_identical_to_single_runs(gpu, ipt, blocks):
     parts.insert(2, gpu.generators.collision_particles(big, softening=0.05))
     with Batch([len(p[2]) for p in parts], softening=0.05) as b:
         b.set_pairs("symmetric", ipt)
-        b.set_schedule(4, blocks, graph_steps=0)
+        b.set_schedule(0, blocks, graph_steps=0)
         assert b.pairs() == ("symmetric", ipt)
         assert b.schedule() == (0, blocks, 0)
         b.upload(*_cat(parts))
@@ -78,6 +91,30 @@ def test_symmetric_batch_bit_identical_to_single_runs(gpu, ipt, blocks):
         want = _alone(gpu, *p, 0, 5, ipt=ipt)
         for g, w in zip(got, want):
             assert np.array_equal(b.split(g)[r], w), (r, ipt)
+
+
+@pytest.mark.parametrize("ipt,tiles,blocks", [(1, 0, 0), (2, 1, 97), (4, 0, 0), (4, 3, 5000)])
+def test_symmetric_batch_against_oracle_and_deterministic(gpu, ipt, tiles, blocks):
+    from gcsim.batch import Batch
+
+    # runs of 2-12 tiles plus one of ~190 tiles; items of the default, 1 or
+    # 3 source tiles; one block per item or persistent grids of 97 and 5000
+    specs, parts = _runs(gpu, 5, seed=11)
+    big = gpu.generators.collisions(1, seed=12, n_min=48000, n_max=49000)[0]
+    parts.insert(2, gpu.generators.collision_particles(big, softening=0.05))
+    out = []
+    for _ in range(2):
+        with Batch([len(p[2]) for p in parts], softening=0.05) as b:
+            b.set_pairs("symmetric", ipt)
+            b.set_schedule(tiles, blocks, graph_steps=0)
+            assert b.schedule() == (tiles, blocks, 0)
+            b.upload(*_cat(parts))
+            b.compute_accelerations()
+            out.append(b.download()[2])
+    assert np.array_equal(out[0], out[1])
+    for r, p in enumerate(parts):
+        ref = _oracle_acc(p)
+        assert rel_err(b.split(out[0])[r], ref).max() <= 1e-5, (r, ipt, blocks)
 
 
 @pytest.mark.parametrize("pairs", ["targets", "symmetric"])
@@ -114,7 +151,7 @@ def test_autotune_and_forces_against_oracle(gpu, pairs):
         if form == "targets":
             assert chunk >= 1 and blocks >= 1 and ipt == 1
         else:
-            assert pairs == "auto" and chunk == 0 and blocks >= 0 and ipt in (1, 2, 4)
+            assert pairs == "auto" and chunk >= 0 and blocks >= 0 and ipt in (1, 2, 4)
         assert b.schedule()[:2] == (chunk, blocks)
         assert b.autotune() == (chunk, blocks)  # cached
         assert b.pairs() == (form, ipt)
