@@ -28,6 +28,7 @@ sys.path.insert(0, str(ROOT / "galaxy-collision-simulation_amd"))
 
 FLOP_PER_PAIR = 20.0          # GPU Gems 3 ch. 31 convention
 FP32_PEAK_TFLOPS = 157.3      # MI355X FP32 vector
+FP64_PEAK_TFLOPS = 78.6       # MI355X FP64 vector
 HBM_PEAK_GBS = 8000.0
 METRIC = "Gpair-interactions/s + %FP32 peak, mixed vs FP64 GPU @<=1e-5 rel force err"
 
@@ -349,7 +350,9 @@ def main() -> int:
     diag = eng.diagnostics()
     eng.close()
 
+    peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     if args.precision == "fp64":
+        ipt, jsplit = 1, 0
         kernel_name = "direct_fp64_kernel"
     elif pair_form == "symmetric":
         kernel_name = "direct_sym_kernel"
@@ -379,9 +382,9 @@ def main() -> int:
             "bound": "valu",
             "kernel": kernel_name,
             "achieved": round(achieved, 3),
-            "peak": FP32_PEAK_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+            "frac": round(achieved / peak, 4),
             "traffic": _traffic(kernel_name),
             "kernel_ms": round(k_ms, 4),
             "flop_per_pair": FLOP_PER_PAIR,

@@ -2,7 +2,9 @@
 // path").
 //
 // One target per lane, sources streamed through LDS in kTile-particle tiles
-// read with coalesced SoA loads.  Each lane sums its sources in ascending j
+// read with coalesced SoA loads.  At N = 65,536 that is one wave per SIMD,
+// so the instruction-level parallelism comes from evaluating 8 sources side
+// by side before adding their terms in order.  Each lane sums its sources in ascending j
 // with the arithmetic of compute_direct_accelerations_for_targets
 // (cpp/direct/direct_solver.cpp:8-23,59-83): r = x_j - x_i,
 // s2 = ((rx*rx + ry*ry) + rz*rz) + eps^2, skip j == i and s2 == 0,
@@ -19,6 +21,10 @@ namespace {
 struct alignas(32) Src64 {
     double x, y, z, gm;
 };
+
+// Sources per step of a lane's sweep (kTile is a multiple).
+constexpr int kUnroll = 8;
+static_assert(kTile % kUnroll == 0, "tile holds whole steps");
 
 __global__ __launch_bounds__(kTile) void direct_fp64_kernel(
     int n, int begin, int end, const double* __restrict__ x, const double* __restrict__ y,
@@ -49,20 +55,34 @@ __global__ __launch_bounds__(kTile) void direct_fp64_kernel(
         }
         __syncthreads();
         const int cnt = min(kTile, n - j0);
-        for (int jj = 0; jj < cnt; ++jj) {
-            const Src64 s = tile[jj];
-            if (j0 + jj == i) continue;
-            const double rx = s.x - xi;
-            const double ry = s.y - yi;
-            const double rz = s.z - zi;
-            const double s2 = (rx * rx + ry * ry + rz * rz) + eps2;
-            if (s2 == 0.0) continue;
-            const double inv_r = 1.0 / __builtin_sqrt(s2);
-            const double inv_r3 = inv_r * inv_r * inv_r;
-            const double k = s.gm * inv_r3;
-            sx += rx * k;
-            sy += ry * k;
-            sz += rz * k;
+        // kUnroll independent sources per step: their terms are computed side
+        // by side and then added in ascending j.  A skipped source (j == i,
+        // s2 == 0, or past the tile's end) contributes an exact +0.0, which
+        // leaves the sum's bits unchanged: the sum starts at +0.0 and so is
+        // never -0.0.
+        for (int jj = 0; jj < cnt; jj += kUnroll) {
+            double tx[kUnroll], ty[kUnroll], tz[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const Src64 s = tile[jj + u];
+                const double rx = s.x - xi;
+                const double ry = s.y - yi;
+                const double rz = s.z - zi;
+                const double s2 = (rx * rx + ry * ry + rz * rz) + eps2;
+                const double inv_r = 1.0 / __builtin_sqrt(s2);
+                const double inv_r3 = inv_r * inv_r * inv_r;
+                const double k = s.gm * inv_r3;
+                const bool skip = jj + u >= cnt || j0 + jj + u == i || s2 == 0.0;
+                tx[u] = skip ? 0.0 : rx * k;
+                ty[u] = skip ? 0.0 : ry * k;
+                tz[u] = skip ? 0.0 : rz * k;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                sx += tx[u];
+                sy += ty[u];
+                sz += tz[u];
+            }
         }
     }
     if (!active) return;
