@@ -12,14 +12,15 @@
 //     reference's insertion builds (fmm_solver.cpp:136-155).  Empty children
 //     are not materialised (they have zero mass and never enter a list);
 //  5. upward pass, deepest level first: mass, centre of mass, 35 raw
-//     moments about the COM in FP64 (P2M at leaves, binomial M2M shift),
-//     then FP32 normalised moments N_alpha = M_alpha / alpha!;
+//     moments about the COM in FP64 (P2M at leaves, binomial M2M shift);
+//     then per node the far-field coefficient table of far_m2p.hpp (FP64
+//     and FP32) from N_alpha = M_alpha / alpha!;
 //  6. per target leaf a preorder DFS with the reference's acceptance test
 //     2 (h_t + h_s) / |c_s - c_t| < theta on geometric centres
 //     (fmm_solver.cpp:228-277): far cells and near leaves, in the
 //     reference's list order;
 //  7. evaluation, one 64-lane wave per target leaf: every far cell by M2P
-//     (generated Cartesian derivative tensors, m2p_generated.inc), every
+//     (the radial-kernel form of far_m2p.hpp over the cell's table), every
 //     near leaf by P2P over sources staged in LDS as offsets from the
 //     target leaf's centre; self pairs skipped (fmm_solver.cpp:295-310).
 // The evaluation is instantiated in FP32 (precision "mixed": FP32 pairs and
@@ -38,7 +39,6 @@ namespace {
 
 constexpr int kLevels = 21;       // 3 key bits per level, 63 bits
 constexpr int kMom = 35;          // raw Cartesian moments, total degree <= 4
-constexpr int kN32 = 36;          // FP32 normalised moments per node (+pad)
 constexpr int kEvalLanes = 64;
 
 __device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -49,7 +49,7 @@ __device__ __forceinline__ double gc_rsq(double s) { return 1.0 / __builtin_sqrt
 #define GC_M2P_INLINE __device__ __forceinline__
 #define GC_FMA(a, b, c) gc_fma((a), (b), (c))
 #define GC_RSQ(s) gc_rsq(s)
-#include "m2p_generated.inc"
+#include "far_m2p.hpp"
 #undef GC_FMA
 #undef GC_RSQ
 
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(64) void upward_kernel(
     const int* __restrict__ first_child, const int* __restrict__ nchild,
     const double* __restrict__ geom, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, const double* __restrict__ sm, double* __restrict__ mcom,
-    double* __restrict__ mom, float* __restrict__ n32) {
+    double* __restrict__ mom) {
 #pragma clang fp contract(off)
     const int k = lb + blockIdx.x * 64 + threadIdx.x;
     if (k >= le) return;
@@ -281,13 +281,33 @@ __global__ __launch_bounds__(64) void upward_kernel(
     mcom[4 * k] = mass;
     for (int d = 0; d < 3; ++d) mcom[4 * k + 1 + d] = com[d];
     double* out = mom + static_cast<size_t>(kMom) * k;
-    float* nf = n32 + static_cast<size_t>(kN32) * k;
 #pragma unroll
-    for (int a = 0; a < kMom; ++a) {
-        out[a] = M[a];
-        nf[a] = static_cast<float>(a == 0 ? mass : M[a] * c_tables.inv_fact[a]);
+    for (int a = 0; a < kMom; ++a) out[a] = M[a];
+}
+
+// Per node, the far-field coefficient table of far_m2p.hpp from the FP64
+// moments (N_alpha = M_alpha / alpha!, N_0 = mass), in FP64 and FP32.
+__global__ __launch_bounds__(64) void far_coef_kernel(int nnodes, const double* __restrict__ mcom,
+                                                      const double* __restrict__ mom,
+                                                      double* __restrict__ kc64,
+                                                      float* __restrict__ kc32) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= nnodes) return;
+    double N[kMom], K[kFarCoef];
+    const double* src = mom + static_cast<size_t>(kMom) * k;
+    N[0] = mcom[4 * k];
+    N[1] = N[2] = N[3] = 0.0;
+#pragma unroll
+    for (int a = 4; a < kMom; ++a) N[a] = src[a] * c_tables.inv_fact[a];
+    far_coefficients(N, K);
+    double* o64 = kc64 + static_cast<size_t>(kFarCoef) * k;
+    float* o32 = kc32 + static_cast<size_t>(kFarCoef) * k;
+#pragma unroll
+    for (int a = 0; a < kFarCoef; ++a) {
+        o64[a] = K[a];
+        o32[a] = static_cast<float>(K[a]);
     }
-    nf[kMom] = 0.f;
 }
 
 // leaf flags for the scan of target leaves: non-empty leaves with mass > 0
@@ -422,25 +442,6 @@ __global__ __launch_bounds__(64) void lists_kernel(
 }
 
 // ---- 7. evaluation ----------------------------------------------------------
-// Normalised moments N_alpha = M_alpha / alpha! of a cell: the FP32 copy for
-// the FP32 evaluation, recomputed from the FP64 moments for the FP64 one.
-template <typename F>
-__device__ __forceinline__ void load_moments(int cell, const float* __restrict__ n32,
-                                             const double* __restrict__ mom,
-                                             const double* __restrict__ mcom, F* N) {
-    if constexpr (sizeof(F) == sizeof(double)) {
-        const double* src = mom + static_cast<size_t>(kMom) * cell;
-        N[0] = mcom[4 * cell];
-        N[1] = N[2] = N[3] = 0.0;
-#pragma unroll
-        for (int a = 4; a < kMom; ++a) N[a] = src[a] * c_tables.inv_fact[a];
-    } else {
-        const float* src = n32 + static_cast<size_t>(kN32) * cell;
-#pragma unroll
-        for (int a = 0; a < kMom; ++a) N[a] = src[a];
-    }
-}
-
 template <typename F>
 struct Src {
     F x, y, z, gm;
@@ -448,14 +449,17 @@ struct Src {
 };
 
 // Far cells staged per chunk: COM offset from the target leaf's centre and
-// the normalised moments.
-constexpr int kFarChunk = 64;
+// the cell's far-field coefficient table (far_m2p.hpp).  32 cells per chunk
+// in FP32 (8.7 KB of tables), 16 in FP64.
+template <typename F>
+constexpr int far_chunk() { return sizeof(F) == sizeof(float) ? 32 : 16; }
 constexpr int kNearPiece = 64;
 
 template <typename F>
-struct FarStage {
-    F c[kFarChunk][4];
-    F N[kFarChunk][kN32];
+struct alignas(16) FarStage {
+    F c[far_chunk<F>()][4];
+    F K[far_chunk<F>()][kFarCoef];
+    int id[far_chunk<F>()];
 };
 
 // Evaluation, in two kernels per target leaf (one 64-lane wave each),
@@ -475,20 +479,124 @@ struct FarStage {
 //       far kernel's and applies the closing kick.
 // Splitting keeps the register-heavy M2P out of the P2P kernel, which then
 // runs at full occupancy.
+// Targets per lane for a leaf chunk of c targets: t = 1, 2 or 3, whichever
+// keeps the most of the wave's 64 x t target slots busy when the chunk is
+// spread over floor(64 / ceil(c / t)) lane groups (ties to the smaller t).
+// On a Plummer sphere of 2^20 particles in leaves of <= 64 this serves
+// 93.5% of the slots, against 81% for one target per lane.
+__host__ __device__ inline int far_targets_per_lane(int c) {
+    int best = 1, best_used = 0;
+    for (int t = 1; t <= 3; ++t) {
+        const int h = (c + t - 1) / t;
+        const int used3 = c * (kEvalLanes / h) * (6 / t);  // slots used, scaled by 6/t
+        if (used3 > best_used) {
+            best_used = used3;
+            best = t;
+        }
+    }
+    return best;
+}
+
+// One leaf chunk of c targets at TPL targets per lane: lane (g, l) of group
+// g holds targets l, l + h, .. (h = ceil(c / TPL) lanes per group) and
+// evaluates cells g, g + groups, .. of each staged chunk for all of them,
+// reading each cell's table once.  Group sums are added in group order and
+// written by group 0.
+template <typename F, int ORDER, int TPL>
+__device__ __forceinline__ void far_leaf_chunk(
+    FarStage<F>& fs, double (&red)[3][kEvalLanes], int base, int c, double cx, double cy,
+    double cz, F eps2, const double* __restrict__ mcom, const F* __restrict__ kcoef,
+    const int* __restrict__ sidx, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, long long fo, int nf, const int* __restrict__ far, double G,
+    int begin, int end, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az) {
+#pragma clang fp contract(off)
+    constexpr int kChunk = far_chunk<F>();
+    const int lane = threadIdx.x;
+    const int h = (c + TPL - 1) / TPL;
+    const int groups = kEvalLanes / h;
+    const int g = lane / h;
+    const int tl = lane - g * h;
+    const bool active = g < groups;
+    F tx[TPL], ty[TPL], tz[TPL];
+    double far_a[TPL][3];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int ti = min(tl + k * h, c - 1);  // a spare slot repeats the last target
+        tx[k] = static_cast<F>(sx[base + ti] - cx);
+        ty[k] = static_cast<F>(sy[base + ti] - cy);
+        tz[k] = static_cast<F>(sz[base + ti] - cz);
+        far_a[k][0] = far_a[k][1] = far_a[k][2] = 0.0;
+    }
+    for (int f0 = 0; f0 < nf; f0 += kChunk) {
+        const int k = min(kChunk, nf - f0);
+        __syncthreads();
+        if (lane < k) {
+            const int cell = far[fo + f0 + lane];
+            fs.id[lane] = cell;
+            fs.c[lane][0] = static_cast<F>(mcom[4 * cell + 1] - cx);
+            fs.c[lane][1] = static_cast<F>(mcom[4 * cell + 2] - cy);
+            fs.c[lane][2] = static_cast<F>(mcom[4 * cell + 3] - cz);
+        }
+        __syncthreads();
+        // the chunk's tables, 16-byte units spread over the whole wave
+        constexpr int kUnits = kFarCoef * static_cast<int>(sizeof(F)) / 16;
+        for (int e = lane; e < k * kUnits; e += kEvalLanes) {
+            const int f = e / kUnits, u = e - f * kUnits;
+            const uint4 v = reinterpret_cast<const uint4*>(
+                kcoef + static_cast<size_t>(kFarCoef) * fs.id[f])[u];
+            reinterpret_cast<uint4*>(fs.K[f])[u] = v;
+        }
+        __syncthreads();
+        if (!active) continue;
+        F sf[TPL][3];
+#pragma unroll
+        for (int q = 0; q < TPL; ++q) sf[q][0] = sf[q][1] = sf[q][2] = 0;
+        for (int f = g; f < k; f += groups) {
+#pragma unroll
+            for (int q = 0; q < TPL; ++q) {
+                const F yx = fs.c[f][0] - tx[q], yy = fs.c[f][1] - ty[q], yz = fs.c[f][2] - tz[q];
+                far_m2p<F, ORDER>(yx, yy, yz, eps2, fs.K[f], sf[q][0], sf[q][1], sf[q][2]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TPL; ++q)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) far_a[q][d] += static_cast<double>(sf[q][d]);
+    }
+#pragma unroll
+    for (int q = 0; q < TPL; ++q) {
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) red[d][lane] = active ? far_a[q][d] : 0.0;
+        __syncthreads();
+        const int ti = tl + q * h;
+        if (g != 0 || ti >= c) continue;
+        double a[3];
+        for (int d = 0; d < 3; ++d) {
+            double acc = red[d][tl];
+            for (int j = 1; j < groups; ++j) acc += red[d][tl + j * h];
+            a[d] = G * acc;
+        }
+        const int me = sidx[base + ti];
+        if (me < begin || me >= end) continue;
+        ax[me] = a[0];
+        ay[me] = a[1];
+        az[me] = a[2];
+    }
+}
+
 template <typename F, int ORDER>
 __global__ __launch_bounds__(kEvalLanes) void eval_far_kernel(
     const int* __restrict__ leaves, const double* __restrict__ geom,
-    const double* __restrict__ mcom, const float* __restrict__ n32, const double* __restrict__ mom,
+    const double* __restrict__ mcom, const F* __restrict__ kcoef,
     const int* __restrict__ nb, const int* __restrict__ ne, const int* __restrict__ sidx,
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const long long* __restrict__ far_off, const int* __restrict__ nfar,
     const int* __restrict__ far, double G, double eps, int begin, int end,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
-#pragma clang fp contract(off)
     __shared__ FarStage<F> fs;
     __shared__ double red[3][kEvalLanes];
     const int leaf = leaves[blockIdx.x];
-    const int lane = threadIdx.x;
     const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
     const F eps2 = static_cast<F>(eps * eps);
     const int tb = nb[leaf], te = ne[leaf];
@@ -496,61 +604,15 @@ __global__ __launch_bounds__(kEvalLanes) void eval_far_kernel(
     const int nf = nfar[blockIdx.x];
     for (int base = tb; base < te; base += kEvalLanes) {
         const int c = min(kEvalLanes, te - base);
-        const int groups = kEvalLanes / c;
-        const int g = lane / c;
-        const int tl = lane - g * c;
-        const bool active = g < groups;
-        const int mine = base + tl;  // sorted position of this lane's target
-        const F tx = static_cast<F>(sx[mine] - cx);
-        const F ty = static_cast<F>(sy[mine] - cy);
-        const F tz = static_cast<F>(sz[mine] - cz);
-        double far_a[3] = {0.0, 0.0, 0.0};
-        for (int f0 = 0; f0 < nf; f0 += kFarChunk) {
-            const int k = min(kFarChunk, nf - f0);
-            __syncthreads();
-            if (lane < k) {
-                const int cell = far[fo + f0 + lane];
-                fs.c[lane][0] = static_cast<F>(mcom[4 * cell + 1] - cx);
-                fs.c[lane][1] = static_cast<F>(mcom[4 * cell + 2] - cy);
-                fs.c[lane][2] = static_cast<F>(mcom[4 * cell + 3] - cz);
-                F N[kMom];
-                load_moments<F>(cell, n32, mom, mcom, N);
-#pragma unroll
-                for (int a = 0; a < kMom; ++a) fs.N[lane][a] = N[a];
-            }
-            __syncthreads();
-            if (!active) continue;
-            F sfx = 0, sfy = 0, sfz = 0;
-            for (int f = g; f < k; f += groups) {
-                const F yx = fs.c[f][0] - tx, yy = fs.c[f][1] - ty, yz = fs.c[f][2] - tz;
-                const F* N = fs.N[f];  // read from LDS as the contraction needs them
-                F fx = 0, fy = 0, fz = 0;
-                if (ORDER == 0) m2p_order0<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-                else if (ORDER == 2) m2p_order2<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-                else m2p_order4<F>(yx, yy, yz, eps2, N, fx, fy, fz);
-                sfx += fx;
-                sfy += fy;
-                sfz += fz;
-            }
-            far_a[0] += static_cast<double>(sfx);
-            far_a[1] += static_cast<double>(sfy);
-            far_a[2] += static_cast<double>(sfz);
+#define GC_CHUNK(T)                                                                              \
+    far_leaf_chunk<F, ORDER, T>(fs, red, base, c, cx, cy, cz, eps2, mcom, kcoef, sidx, sx, sy, sz, \
+                                fo, nf, far, G, begin, end, ax, ay, az)
+        switch (far_targets_per_lane(c)) {  // uniform per block
+        case 3: GC_CHUNK(3); break;
+        case 2: GC_CHUNK(2); break;
+        default: GC_CHUNK(1); break;
         }
-        __syncthreads();
-        for (int d = 0; d < 3; ++d) red[d][lane] = active ? far_a[d] : 0.0;
-        __syncthreads();
-        if (g != 0) continue;
-        double a[3];
-        for (int d = 0; d < 3; ++d) {
-            double acc = red[d][tl];
-            for (int k = 1; k < groups; ++k) acc += red[d][tl + k * c];
-            a[d] = G * acc;
-        }
-        const int me = sidx[mine];
-        if (me < begin || me >= end) continue;
-        ax[me] = a[0];
-        ay[me] = a[1];
-        az[me] = a[2];
+#undef GC_CHUNK
     }
 }
 
@@ -854,7 +916,8 @@ struct FmmEngine::Impl {
     double* geom = nullptr;  // 4 per node: cx, cy, cz, hw
     double* mcom = nullptr;  // 4 per node: mass, com
     double* mom = nullptr;   // 35 per node
-    float* n32 = nullptr;    // 36 per node
+    double* kc64 = nullptr;  // kFarCoef per node: far-field coefficient tables
+    float* kc32 = nullptr;   // the same in FP32
     int *nb = nullptr, *ne = nullptr, *first_child = nullptr, *nchild = nullptr;
     // per-level scratch
     size_t lcap = 0;
@@ -890,7 +953,7 @@ struct FmmEngine::Impl {
 
     ~Impl() {
         void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
-                        n32, nb, ne, first_child, nchild, bounds, count, offset,
+                        kc64, kc32, nb, ne, first_child, nchild, bounds, count, offset,
                         scan_tmp, overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
                         near_off, frontier, far, near, icount, ioff, row_off, items, npart, stats_dev};
         for (void* p : ptrs) (void)hipFree(p);
@@ -906,7 +969,8 @@ struct FmmEngine::Impl {
         c = 4 * ncap; if ((e = grow(&geom, &c, 4 * nodes, 4 * keep, st)) != hipSuccess) return e;
         c = 4 * ncap; if ((e = grow(&mcom, &c, 4 * nodes, 0, st)) != hipSuccess) return e;
         c = kMom * ncap; if ((e = grow(&mom, &c, kMom * nodes, 0, st)) != hipSuccess) return e;
-        c = kN32 * ncap; if ((e = grow(&n32, &c, kN32 * nodes, 0, st)) != hipSuccess) return e;
+        c = kFarCoef * ncap; if ((e = grow(&kc64, &c, kFarCoef * nodes, 0, st)) != hipSuccess) return e;
+        c = kFarCoef * ncap; if ((e = grow(&kc32, &c, kFarCoef * nodes, 0, st)) != hipSuccess) return e;
         c = ncap; if ((e = grow(&nb, &c, nodes, keep, st)) != hipSuccess) return e;
         c = ncap; if ((e = grow(&ne, &c, nodes, keep, st)) != hipSuccess) return e;
         c = ncap; if ((e = grow(&first_child, &c, nodes, keep, st)) != hipSuccess) return e;
@@ -1035,8 +1099,10 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         const int lb = level_begin[level], le = level_begin[level + 1];
         if (le <= lb) continue;
         hipLaunchKernelGGL(upward_kernel, dim3((le - lb + 63) / 64), dim3(64), 0, st, lb, le, nb, ne,
-                           first_child, nchild, geom, sxs, sys, szs, sms, mcom, mom, n32);
+                           first_child, nchild, geom, sxs, sys, szs, sms, mcom, mom);
     }
+    hipLaunchKernelGGL(far_coef_kernel, dim3((nnodes + 63) / 64), dim3(64), 0, st, nnodes, mcom, mom,
+                       kc64, kc32);
 
     // leaves (mass > 0)
     if (static_cast<size_t>(nnodes) > leafcap) {
@@ -1236,24 +1302,24 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const dim3 grid(static_cast<unsigned>(nleaves)), block(kEvalLanes);
     const int b = static_cast<int>(begin), e = static_cast<int>(end);
     const double G = p.gravitational_constant, eps = p.softening;
-#define GC_FAR(F, O)                                                                             \
-    hipLaunchKernelGGL((eval_far_kernel<F, O>), grid, block, 0, st, leaves, geom, mcom, n32, mom, \
-                       nb, ne, idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, s.ax,  \
-                       s.ay, s.az)
+#define GC_FAR(F, O, KC)                                                                         \
+    hipLaunchKernelGGL((eval_far_kernel<F, O>), grid, block, 0, st, leaves, geom, mcom, KC, nb, ne, \
+                       idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, s.ax, s.ay,    \
+                       s.az)
 #define GC_NEAR(F, S)                                                                            \
     hipLaunchKernelGGL((eval_near_kernel<F, S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
                        items, leaves, geom, nb, ne, sxs, sys, szs, sms, near_off, near, G, eps,   \
                        npart)
     if (mixed) {
-        if (order == 0) GC_FAR(float, 0);
-        else if (order == 2) GC_FAR(float, 2);
-        else GC_FAR(float, 4);
+        if (order == 0) GC_FAR(float, 0, kc32);
+        else if (order == 2) GC_FAR(float, 2, kc32);
+        else GC_FAR(float, 4, kc32);
         if (safe) GC_NEAR(float, true);
         else GC_NEAR(float, false);
     } else {
-        if (order == 0) GC_FAR(double, 0);
-        else if (order == 2) GC_FAR(double, 2);
-        else GC_FAR(double, 4);
+        if (order == 0) GC_FAR(double, 0, kc64);
+        else if (order == 2) GC_FAR(double, 2, kc64);
+        else GC_FAR(double, 4, kc64);
         if (safe) GC_NEAR(double, true);
         else GC_NEAR(double, false);
     }

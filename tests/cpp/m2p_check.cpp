@@ -1,6 +1,8 @@
-// m2p_check.cpp — host check of the generated M2P (csrc/m2p_generated.inc)
-// against the oracle's restatement of the reference M2P, in FP64 (the
-// formula) and FP32 (the rounding of the device arithmetic).
+// m2p_check.cpp — host check of the device M2P (csrc/far_m2p.hpp: per-cell
+// coefficient tables and the radial-kernel evaluation) against the oracle's
+// restatement of the reference M2P, in FP64 (the formula) and FP32 (the
+// rounding of the device arithmetic), for well separated targets and for
+// targets at about the opening-angle limit.
 // Prints: max relative difference for p = 0, 2, 4 in double and float.
 #include <cmath>
 #include <cstdio>
@@ -11,26 +13,30 @@
 #define GC_M2P_INLINE inline
 #define GC_FMA(a, b, c) std::fma((a), (b), (c))
 #define GC_RSQ(s) (F(1) / std::sqrt(s))
-#include "m2p_generated.inc"
+#include "far_m2p.hpp"
 
 template <typename F>
 void eval(int p, const double* t, const double* com, double mass, const double* mom, double eps,
           double* out) {
     int e[35][3];
     orc_multipole_exponents(&e[0][0]);
-    F N[35];
+    double N[35];
     for (int k = 0; k < 35; ++k) {
         const double fact = std::tgamma(e[k][0] + 1.0) * std::tgamma(e[k][1] + 1.0) *
                             std::tgamma(e[k][2] + 1.0);
         const int d = e[k][0] + e[k][1] + e[k][2];
-        N[k] = F(d == 0 ? mass : (d == 1 ? 0.0 : mom[k] / fact));
+        N[k] = d == 0 ? mass : (d == 1 ? 0.0 : mom[k] / fact);
     }
+    double K64[kFarCoef];
+    far_coefficients(N, K64);
+    F K[kFarCoef];
+    for (int k = 0; k < kFarCoef; ++k) K[k] = F(K64[k]);
     F ax = 0, ay = 0, az = 0;
     const F y[3] = {F(com[0] - t[0]), F(com[1] - t[1]), F(com[2] - t[2])};
     const F e2 = F(eps * eps);
-    if (p == 0) m2p_order0<F>(y[0], y[1], y[2], e2, N, ax, ay, az);
-    if (p == 2) m2p_order2<F>(y[0], y[1], y[2], e2, N, ax, ay, az);
-    if (p == 4) m2p_order4<F>(y[0], y[1], y[2], e2, N, ax, ay, az);
+    if (p == 0) far_m2p<F, 0>(y[0], y[1], y[2], e2, K, ax, ay, az);
+    if (p == 2) far_m2p<F, 2>(y[0], y[1], y[2], e2, K, ax, ay, az);
+    if (p == 4) far_m2p<F, 4>(y[0], y[1], y[2], e2, K, ax, ay, az);
     out[0] = ax;
     out[1] = ay;
     out[2] = az;
@@ -56,7 +62,9 @@ int main() {
             for (int d = 0; d < 3; ++d) off[d] = pts[k][d] - c[d] / M;
             orc_p2m(mom, off, ms[k]);
         }
-        for (int d = 0; d < 3; ++d) t[d] = com[d] + 3.0 * nd(rng);
+        // every other trial a target at about twice the cell's spread
+        const double reach = trial % 2 ? 3.0 : 0.9;
+        for (int d = 0; d < 3; ++d) t[d] = com[d] + reach * nd(rng);
         const double eps = trial % 3 == 0 ? 0.0 : 0.05;
         for (int p = 0; p <= 4; p += 2) {
             double want[3], gd[3], gf[3];
