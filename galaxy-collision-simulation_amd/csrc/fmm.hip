@@ -659,24 +659,44 @@ __global__ __launch_bounds__(256) void near_item_fill_kernel(
     }
 }
 
-template <typename F, bool SAFE>
-__global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
-    const NearItem* __restrict__ items, const int* __restrict__ leaves,
-    const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
+// Near-field sources staged in LDS: offset from the target leaf's centre
+// and G m (zero past the end of the item's sources).
+template <typename F>
+struct alignas(4 * sizeof(F)) NearSrc {
+    F x, y, z, gm;
+};
+
+// One leaf chunk of c targets of a near item at TPL targets per lane (lane
+// (g, l) holds targets l, l + h, .., h = ceil(c / TPL)); group g takes
+// sources g, g + groups, .. of each staged round.  The self pair
+// (fmm_solver.cpp:295-310) is dropped without a branch: each target knows
+// its own position among the piece's sources (the target leaf is one of
+// its near leaves) and zeroes that pair's factor.
+template <typename F, bool SAFE, int TPL>
+__device__ __forceinline__ void near_leaf_chunk(
+    NearSrc<F> (&tile)[kNearStage], int (&qs)[kNearPiece], int (&qpre)[kNearPiece + 1],
+    double (&red)[3][kEvalLanes], const NearItem& item, int tb, int base, int c, double cx,
+    double cy, double cz, F eps2, const int* __restrict__ nb, const int* __restrict__ ne,
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const double* __restrict__ sm, const long long* __restrict__ near_off,
-    const int* __restrict__ near, double G, double eps, double* __restrict__ part) {
+    const double* __restrict__ sm, long long no, const int* __restrict__ near, double G,
+    double* __restrict__ part) {
 #pragma clang fp contract(off)
-    __shared__ Src<F> tile[kNearStage];
-    __shared__ int qs[kNearPiece], qpre[kNearPiece + 1];
-    __shared__ double red[3][kEvalLanes];
-    const NearItem item = items[blockIdx.x];
-    const int leaf = leaves[item.leaf];
     const int lane = threadIdx.x;
-    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
-    const F eps2 = static_cast<F>(eps * eps);
-    const int tb = nb[leaf], te = ne[leaf];
-    const long long no = near_off[item.leaf];
+    const int h = (c + TPL - 1) / TPL;
+    const int groups = kEvalLanes / h;
+    const int g = lane / h;
+    const int tl = lane - g * h;
+    const bool active = g < groups;
+    F tx[TPL], ty[TPL], tz[TPL];
+    double near_a[TPL][3];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+        const int ti = min(tl + k * h, c - 1);  // a spare slot repeats the last target
+        tx[k] = static_cast<F>(sx[base + ti] - cx);
+        ty[k] = static_cast<F>(sy[base + ti] - cy);
+        tz[k] = static_cast<F>(sz[base + ti] - cz);
+        near_a[k][0] = near_a[k][1] = near_a[k][2] = 0.0;
+    }
     // source j of the current piece: its sorted index (-1 past the end)
     auto locate = [&](int j, int total, int kq) -> int {
         if (j >= total) return -1;
@@ -688,78 +708,78 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
         }
         return qs[lo] + (j - qpre[lo]);
     };
-    for (int base = tb; base < te; base += kEvalLanes) {
-        const int c = min(kEvalLanes, te - base);
-        const int groups = kEvalLanes / c;
-        const int g = lane / c;
-        const int tl = lane - g * c;
-        const bool active = g < groups;
-        const int mine = base + tl;
-        const F tx = static_cast<F>(sx[mine] - cx);
-        const F ty = static_cast<F>(sy[mine] - cy);
-        const F tz = static_cast<F>(sz[mine] - cz);
-        double near_a[3] = {0.0, 0.0, 0.0};
-        for (int q0 = item.q0; q0 < item.q1; q0 += kNearPiece) {
-            const int kq = min(kNearPiece, item.q1 - q0);
+    for (int q0 = item.q0; q0 < item.q1; q0 += kNearPiece) {
+        const int kq = min(kNearPiece, item.q1 - q0);
+        __syncthreads();
+        int len = 0;
+        if (lane < kq) {
+            const int sl = near[no + q0 + lane];
+            qs[lane] = nb[sl];
+            len = ne[sl] - nb[sl];
+        }
+        int incl = len;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        qpre[lane + 1] = incl;
+        if (lane == 0) qpre[0] = 0;
+        const int total = __shfl(incl, 63, 64);
+        __syncthreads();
+        // the target leaf among this piece's near leaves, if it is there:
+        // target ti is then source qpre[qself] + (base - tb) + ti
+        const unsigned long long own = __ballot(lane < kq && qs[lane] == tb);
+        int selfpos[TPL];
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+            const int ti = min(tl + k * h, c - 1);
+            selfpos[k] = own ? qpre[__ffsll(static_cast<long long>(own)) - 1] + (base - tb) + ti : -1;
+        }
+        // registers hold the next round's sources
+        double nxv[kNearPer][4];
+        int nid[kNearPer];
+#pragma unroll
+        for (int u = 0; u < kNearPer; ++u) {
+            nid[u] = locate(lane + u * kEvalLanes, total, kq);
+            const int sj = nid[u] < 0 ? base : nid[u];
+            nxv[u][0] = sx[sj];
+            nxv[u][1] = sy[sj];
+            nxv[u][2] = sz[sj];
+            nxv[u][3] = sm[sj];
+        }
+        for (int cb = 0; cb < total; cb += kNearStage) {
             __syncthreads();
-            int len = 0;
-            if (lane < kq) {
-                const int sl = near[no + q0 + lane];
-                qs[lane] = nb[sl];
-                len = ne[sl] - nb[sl];
-            }
-            int incl = len;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(incl, o, 64);
-                if (lane >= o) incl += v;
-            }
-            qpre[lane + 1] = incl;
-            if (lane == 0) qpre[0] = 0;
-            const int total = __shfl(incl, 63, 64);
-            __syncthreads();
-            // registers hold the next round's sources
-            double nxv[kNearPer][4];
-            int nid[kNearPer];
 #pragma unroll
             for (int u = 0; u < kNearPer; ++u) {
-                nid[u] = locate(lane + u * kEvalLanes, total, kq);
-                const int sj = nid[u] < 0 ? mine : nid[u];
-                nxv[u][0] = sx[sj];
-                nxv[u][1] = sy[sj];
-                nxv[u][2] = sz[sj];
-                nxv[u][3] = sm[sj];
+                NearSrc<F> t;
+                t.x = static_cast<F>(nxv[u][0] - cx);
+                t.y = static_cast<F>(nxv[u][1] - cy);
+                t.z = static_cast<F>(nxv[u][2] - cz);
+                t.gm = nid[u] < 0 ? F(0) : static_cast<F>(G * nxv[u][3]);
+                tile[lane + u * kEvalLanes] = t;
             }
-            for (int cb = 0; cb < total; cb += kNearStage) {
-                __syncthreads();
+            __syncthreads();
+            if (cb + kNearStage < total) {
 #pragma unroll
                 for (int u = 0; u < kNearPer; ++u) {
-                    Src<F> t;
-                    t.x = static_cast<F>(nxv[u][0] - cx);
-                    t.y = static_cast<F>(nxv[u][1] - cy);
-                    t.z = static_cast<F>(nxv[u][2] - cz);
-                    t.gm = nid[u] < 0 ? F(0) : static_cast<F>(G * nxv[u][3]);
-                    t.id = nid[u];
-                    tile[lane + u * kEvalLanes] = t;
+                    nid[u] = locate(cb + kNearStage + lane + u * kEvalLanes, total, kq);
+                    const int sj = nid[u] < 0 ? base : nid[u];
+                    nxv[u][0] = sx[sj];
+                    nxv[u][1] = sy[sj];
+                    nxv[u][2] = sz[sj];
+                    nxv[u][3] = sm[sj];
                 }
-                __syncthreads();
-                if (cb + kNearStage < total) {
+            }
+            if (!active) continue;
+            const int cnt = min(kNearStage, total - cb);
+            F px[TPL], py[TPL], pz[TPL];
 #pragma unroll
-                    for (int u = 0; u < kNearPer; ++u) {
-                        nid[u] = locate(cb + kNearStage + lane + u * kEvalLanes, total, kq);
-                        const int sj = nid[u] < 0 ? mine : nid[u];
-                        nxv[u][0] = sx[sj];
-                        nxv[u][1] = sy[sj];
-                        nxv[u][2] = sz[sj];
-                        nxv[u][3] = sm[sj];
-                    }
-                }
-                if (!active) continue;
-                const int cnt = min(kNearStage, total - cb);
-                F px = 0, py = 0, pz = 0;
-                for (int jj = g; jj < cnt; jj += groups) {
-                    const Src<F> sj = tile[jj];
-                    if (sj.id == mine) continue;
-                    const F dx = sj.x - tx, dy = sj.y - ty, dz = sj.z - tz;
+            for (int q = 0; q < TPL; ++q) px[q] = py[q] = pz[q] = 0;
+            for (int jj = g; jj < cnt; jj += groups) {
+                const NearSrc<F> sj = tile[jj];
+#pragma unroll
+                for (int q = 0; q < TPL; ++q) {
+                    const F dx = sj.x - tx[q], dy = sj.y - ty[q], dz = sj.z - tz[q];
                     const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
                     F r;
                     if (SAFE) {
@@ -767,26 +787,63 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
                     } else {
                         r = gc_rsq(s2);
                     }
+                    r = cb + jj == selfpos[q] ? F(0) : r;
                     const F kk = (sj.gm * r) * (r * r);
-                    px = gc_fma(dx, kk, px);
-                    py = gc_fma(dy, kk, py);
-                    pz = gc_fma(dz, kk, pz);
+                    px[q] = gc_fma(dx, kk, px[q]);
+                    py[q] = gc_fma(dy, kk, py[q]);
+                    pz[q] = gc_fma(dz, kk, pz[q]);
                 }
-                near_a[0] += static_cast<double>(px);
-                near_a[1] += static_cast<double>(py);
-                near_a[2] += static_cast<double>(pz);
+            }
+#pragma unroll
+            for (int q = 0; q < TPL; ++q) {
+                near_a[q][0] += static_cast<double>(px[q]);
+                near_a[q][1] += static_cast<double>(py[q]);
+                near_a[q][2] += static_cast<double>(pz[q]);
             }
         }
+    }
+#pragma unroll
+    for (int q = 0; q < TPL; ++q) {
         __syncthreads();
-        for (int d = 0; d < 3; ++d) red[d][lane] = active ? near_a[d] : 0.0;
+        for (int d = 0; d < 3; ++d) red[d][lane] = active ? near_a[q][d] : 0.0;
         __syncthreads();
-        if (g != 0) continue;
-        double* row = part + 3 * (static_cast<size_t>(item.slot) + (base - tb) + tl);
+        const int ti = tl + q * h;
+        if (g != 0 || ti >= c) continue;
+        double* row = part + 3 * (static_cast<size_t>(item.slot) + (base - tb) + ti);
         for (int d = 0; d < 3; ++d) {
             double acc = red[d][tl];
-            for (int k = 1; k < groups; ++k) acc += red[d][tl + k * c];
+            for (int j = 1; j < groups; ++j) acc += red[d][tl + j * h];
             row[d] = acc;
         }
+    }
+}
+
+template <typename F, bool SAFE>
+__global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
+    const NearItem* __restrict__ items, const int* __restrict__ leaves,
+    const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const double* __restrict__ sm, const long long* __restrict__ near_off,
+    const int* __restrict__ near, double G, double eps, double* __restrict__ part) {
+    __shared__ NearSrc<F> tile[kNearStage];
+    __shared__ int qs[kNearPiece], qpre[kNearPiece + 1];
+    __shared__ double red[3][kEvalLanes];
+    const NearItem item = items[blockIdx.x];
+    const int leaf = leaves[item.leaf];
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    const F eps2 = static_cast<F>(eps * eps);
+    const int tb = nb[leaf], te = ne[leaf];
+    const long long no = near_off[item.leaf];
+    for (int base = tb; base < te; base += kEvalLanes) {
+        const int c = min(kEvalLanes, te - base);
+#define GC_CHUNK(T)                                                                              \
+    near_leaf_chunk<F, SAFE, T>(tile, qs, qpre, red, item, tb, base, c, cx, cy, cz, eps2, nb, ne,  \
+                                sx, sy, sz, sm, no, near, G, part)
+        // at most 2 targets per lane: a third costs more registers than the
+        // slots it fills save here (measured)
+        if (far_targets_per_lane(c) >= 2) GC_CHUNK(2);  // uniform per block
+        else GC_CHUNK(1);
+#undef GC_CHUNK
     }
 }
 
