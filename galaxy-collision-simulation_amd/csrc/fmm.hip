@@ -353,19 +353,33 @@ __device__ __forceinline__ bool well_separated(const double* __restrict__ geom, 
 // `overflow` and the host repeats the pass with larger buffers.
 // FILL == false: count far cells, near leaves and near source particles per
 // leaf; FILL == true: write the lists at the scanned offsets.
-template <bool FILL>
+// MODE 0 counts each leaf's far cells, near leaves and near sources;
+// MODE 1 writes the lists at far_off / near_off from an earlier count;
+// MODE 2 counts and at once stashes the lists in the wave's own scratch
+// region (wcap_far / wcap_near entries per wave, the leaf's start there in
+// lpos_far / lpos_near), for list_copy_kernel to move them to their places
+// once the counts are scanned.  Bit 1 of *overflow flags a wave whose
+// lists outgrew its region (counts stay exact; MODE 1 then writes the
+// lists), bit 0 a frontier overflow.
+template <int MODE>
 __global__ __launch_bounds__(64) void lists_kernel(
     int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
     const double* __restrict__ mcom, const int* __restrict__ first_child,
     const int* __restrict__ nchild, const int* __restrict__ nb, const int* __restrict__ ne,
     double theta, int* __restrict__ nfar, int* __restrict__ nnear, int* __restrict__ nsrc,
     const long long* __restrict__ far_off, const long long* __restrict__ near_off,
-    int* __restrict__ far, int* __restrict__ near, int* __restrict__ scratch, int fcap,
-    int* __restrict__ overflow) {
+    long long wcap_far, long long wcap_near, long long* __restrict__ lpos_far,
+    long long* __restrict__ lpos_near, int* __restrict__ far, int* __restrict__ near,
+    int* __restrict__ scratch, int fcap, int* __restrict__ overflow) {
+    constexpr bool COUNT = MODE != 1, WRITE = MODE != 0;
     const int lane = threadIdx.x;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int* bufa = scratch + static_cast<size_t>(blockIdx.x) * 2 * fcap;
     int* bufb = bufa + fcap;
+    // MODE 2: this wave's stash regions and how much of them is used
+    int* const wfar = far + static_cast<size_t>(blockIdx.x) * wcap_far;
+    int* const wnear = near + static_cast<size_t>(blockIdx.x) * wcap_near;
+    long long used_f = 0, used_n = 0;
     for (int li = blockIdx.x; li < nleaves; li += gridDim.x) {
         const int t = leaves[li];
         int* cur = bufa;
@@ -374,7 +388,21 @@ __global__ __launch_bounds__(64) void lists_kernel(
         __syncthreads();  // one wave: orders the frontier writes before the reads
         int ncur = 1;
         long long cf = 0, cn = 0, cs = 0;
-        const long long fo = FILL ? far_off[li] : 0, no = FILL ? near_off[li] : 0;
+        long long fo = 0, no = 0, flim = 0, nlim = 0;
+        int* fdst = far;
+        int* ndst = near;
+        if (MODE == 1) {
+            fo = far_off[li];
+            no = near_off[li];
+            flim = nlim = 1ll << 62;
+        } else if (MODE == 2) {
+            fdst = wfar;
+            ndst = wnear;
+            fo = used_f;
+            no = used_n;
+            flim = wcap_far - used_f;
+            nlim = wcap_near - used_n;
+        }
         bool ovf = false;
         while (ncur > 0 && !ovf) {
             int nnext = 0;
@@ -394,13 +422,15 @@ __global__ __launch_bounds__(64) void lists_kernel(
                 }
                 const unsigned long long mfar = __ballot(kind == 0);
                 const unsigned long long mnear = __ballot(kind == 1);
-                if (FILL) {
-                    if (kind == 0) far[fo + cf + __popcll(mfar & below)] = node;
-                    if (kind == 1) near[no + cn + __popcll(mnear & below)] = node;
+                if (WRITE) {
+                    const long long pf = cf + __popcll(mfar & below);
+                    const long long pn = cn + __popcll(mnear & below);
+                    if (kind == 0 && pf < flim) fdst[fo + pf] = node;
+                    if (kind == 1 && pn < nlim) ndst[no + pn] = node;
                 }
                 cf += __popcll(mfar);
                 cn += __popcll(mnear);
-                if (!FILL) {
+                if (COUNT) {
                     int src = kind == 1 ? ne[node] - nb[node] : 0;
                     for (int o = 32; o > 0; o >>= 1) src += __shfl_xor(src, o, 64);
                     cs += src;
@@ -433,12 +463,57 @@ __global__ __launch_bounds__(64) void lists_kernel(
             if (lane == 0) atomicOr(overflow, 1);
             continue;
         }
-        if (!FILL && lane == 0) {
+        if (COUNT && lane == 0) {
             nfar[li] = static_cast<int>(cf);
             nnear[li] = static_cast<int>(cn);
             nsrc[li] = static_cast<int>(cs);
         }
+        if (MODE == 2) {
+            if (lane == 0) {
+                lpos_far[li] = fo;
+                lpos_near[li] = no;
+                if (cf > flim || cn > nlim) atomicOr(overflow, 2);
+            }
+            used_f += min(cf, flim);
+            used_n += min(cn, nlim);
+        }
     }
+}
+
+// Moves leaf li's stashed lists (lists_kernel<2>) to far_off / near_off:
+// one block per leaf, coalesced.
+__global__ __launch_bounds__(256) void list_copy_kernel(
+    int nleaves, int lwaves, const int* __restrict__ nfar, const int* __restrict__ nnear,
+    const long long* __restrict__ far_off, const long long* __restrict__ near_off,
+    const long long* __restrict__ lpos_far, const long long* __restrict__ lpos_near,
+    long long wcap_far, long long wcap_near, const int* __restrict__ wfar,
+    const int* __restrict__ wnear, int* __restrict__ far, int* __restrict__ near) {
+    const int li = blockIdx.x;
+    if (li >= nleaves) return;
+    const int w = li % lwaves;
+    const int* sf = wfar + static_cast<size_t>(w) * wcap_far + lpos_far[li];
+    const int* sn = wnear + static_cast<size_t>(w) * wcap_near + lpos_near[li];
+    int* df = far + far_off[li];
+    int* dn = near + near_off[li];
+    for (int k = threadIdx.x; k < nfar[li]; k += 256) df[k] = sf[k];
+    for (int k = threadIdx.x; k < nnear[li]; k += 256) dn[k] = sn[k];
+}
+
+// per-wave list totals of lists_kernel's leaf-to-wave assignment (li mod
+// lwaves), to size the stash regions
+__global__ __launch_bounds__(256) void wave_totals_kernel(int nleaves, int lwaves,
+                                                          const int* __restrict__ nfar,
+                                                          const int* __restrict__ nnear,
+                                                          long long* __restrict__ wtot) {
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= lwaves) return;
+    long long f = 0, n = 0;
+    for (int li = w; li < nleaves; li += lwaves) {
+        f += nfar[li];
+        n += nnear[li];
+    }
+    wtot[w] = f;
+    wtot[lwaves + w] = n;
 }
 
 // ---- 7. evaluation ----------------------------------------------------------
@@ -993,6 +1068,12 @@ struct FmmEngine::Impl {
     int fcap = 4096;
     size_t farcap = 0, nearcap = 0;
     int *far = nullptr, *near = nullptr;
+    // one-pass list build: per-wave stash regions (0: count first), sized
+    // from the per-wave totals of the last two-pass build
+    long long wcap_far = 0, wcap_near = 0;
+    size_t stash_far_cap = 0, stash_near_cap = 0, lpos_cap = 0;
+    int *stash_far = nullptr, *stash_near = nullptr;
+    long long *lpos_far = nullptr, *lpos_near = nullptr, *wtot = nullptr;
     // near-field work items and their partial sums
     int *icount = nullptr, *ioff = nullptr;
     long long* row_off = nullptr;
@@ -1004,7 +1085,6 @@ struct FmmEngine::Impl {
     // host-side results of the last build
     std::vector<int> level_begin;
     int nnodes = 0, nleaves = 0;
-    long long total_far = 0, total_near = 0;
     gc_fmm_stats last{};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
@@ -1012,7 +1092,7 @@ struct FmmEngine::Impl {
         void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
                         kc64, kc32, nb, ne, first_child, nchild, bounds, count, offset,
                         scan_tmp, overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
-                        near_off, frontier, far, near, icount, ioff, row_off, items, npart, stats_dev};
+                        near_off, frontier, far, near, stash_far, stash_near, lpos_far, lpos_near, wtot, icount, ioff, row_off, items, npart, stats_dev};
         for (void* p : ptrs) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -1198,14 +1278,40 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipMemcpyAsync(&lastf[1], flag + nnodes - 1, sizeof(int), hipMemcpyDeviceToHost, st), "leaf count");
     FMM_HIP(hipStreamSynchronize(st), "leaf count");
     nleaves = lastf[0] + lastf[1];
-    if (nleaves == 0) {
-        total_far = total_near = 0;
-        return GC_OK;
-    }
+    if (nleaves == 0) return GC_OK;
 
-    // 6: lists (count, scan, fill); persistent waves, larger frontiers on
-    // overflow
+    // 6: lists; persistent waves, larger frontiers on overflow.  Once a
+    // two-pass build (count, scan, write) has sized the per-wave stash
+    // regions, the lists are counted and stashed in one traversal and moved
+    // to their places after the scan; a wave whose lists outgrow its region
+    // falls back to the writing traversal.
     const int lwaves = std::min(nleaves, 4096);
+    const bool stash = wcap_far > 0 && wcap_near > 0;
+    if (stash) {
+        const size_t fneed = static_cast<size_t>(lwaves) * wcap_far;
+        const size_t nneed = static_cast<size_t>(lwaves) * wcap_near;
+        if (fneed > stash_far_cap) {
+            (void)hipFree(stash_far);
+            stash_far = nullptr;
+            stash_far_cap = fneed;
+            FMM_HIP(hipMalloc(&stash_far, sizeof(int) * stash_far_cap), "allocate list stash");
+        }
+        if (nneed > stash_near_cap) {
+            (void)hipFree(stash_near);
+            stash_near = nullptr;
+            stash_near_cap = nneed;
+            FMM_HIP(hipMalloc(&stash_near, sizeof(int) * stash_near_cap), "allocate list stash");
+        }
+    }
+    if (static_cast<size_t>(nleaves) > lpos_cap) {
+        (void)hipFree(lpos_far);
+        (void)hipFree(lpos_near);
+        lpos_far = lpos_near = nullptr;
+        lpos_cap = static_cast<size_t>(nleaves) * 2;
+        FMM_HIP(hipMalloc(&lpos_far, sizeof(long long) * lpos_cap), "allocate list stash");
+        FMM_HIP(hipMalloc(&lpos_near, sizeof(long long) * lpos_cap), "allocate list stash");
+    }
+    int lovf = 0;
     for (;;) {
         const size_t need = static_cast<size_t>(lwaves) * 2 * fcap;
         if (need > frontier_ints) {
@@ -1216,13 +1322,19 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
             frontier_ints = need;
         }
         FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
-        hipLaunchKernelGGL((lists_kernel<false>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves,
-                           geom, mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc,
-                           far_off, near_off, far, near, frontier, fcap, overflow);
-        int lovf = 0;
+        if (stash)
+            hipLaunchKernelGGL((lists_kernel<2>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves,
+                               geom, mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc,
+                               far_off, near_off, wcap_far, wcap_near, lpos_far, lpos_near,
+                               stash_far, stash_near, frontier, fcap, overflow);
+        else
+            hipLaunchKernelGGL((lists_kernel<0>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves,
+                               geom, mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc,
+                               far_off, near_off, 0ll, 0ll, lpos_far, lpos_near, far, near,
+                               frontier, fcap, overflow);
         FMM_HIP(hipMemcpyAsync(&lovf, overflow, sizeof(int), hipMemcpyDeviceToHost, st), "read flag");
         FMM_HIP(hipStreamSynchronize(st), "list counts");
-        if (!lovf) break;
+        if (!(lovf & 1)) break;
         if (fcap >= (1 << 24)) {
             if (err) *err = "interaction-list frontier beyond 16M nodes";
             return GC_ERR_UNSUPPORTED;
@@ -1230,23 +1342,21 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         fcap *= 4;
     }
     // 64-bit offsets: widen the counts with a scan over long long
-    {
-        auto widen = [&](const int* in, long long* out) -> hipError_t {
-            size_t need = 0;
-            hipcub::TransformInputIterator<long long, ToI64, const int*> it(in, ToI64());
-            hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, out, nleaves, st);
+    auto widen = [&](const int* in, long long* out) -> hipError_t {
+        size_t need = 0;
+        hipcub::TransformInputIterator<long long, ToI64, const int*> it(in, ToI64());
+        hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, out, nleaves, st);
+        if (e != hipSuccess) return e;
+        if (need > scan_tmp_bytes) {
+            (void)hipFree(scan_tmp);
+            scan_tmp_bytes = need;
+            e = hipMalloc(&scan_tmp, scan_tmp_bytes);
             if (e != hipSuccess) return e;
-            if (need > scan_tmp_bytes) {
-                (void)hipFree(scan_tmp);
-                scan_tmp_bytes = need;
-                e = hipMalloc(&scan_tmp, scan_tmp_bytes);
-                if (e != hipSuccess) return e;
-            }
-            return hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, it, out, nleaves, st);
-        };
-        FMM_HIP(widen(nfar, far_off), "scan far lists");
-        FMM_HIP(widen(nnear, near_off), "scan near lists");
-    }
+        }
+        return hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, it, out, nleaves, st);
+    };
+    FMM_HIP(widen(nfar, far_off), "scan far lists");
+    FMM_HIP(widen(nnear, near_off), "scan near lists");
     long long lo[2] = {0, 0};
     int lc[2] = {0, 0};
     FMM_HIP(hipMemcpyAsync(&lo[0], far_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "list size");
@@ -1254,21 +1364,47 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipMemcpyAsync(&lc[0], nfar + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
     FMM_HIP(hipMemcpyAsync(&lc[1], nnear + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
     FMM_HIP(hipStreamSynchronize(st), "list sizes");
-    total_far = lo[0] + lc[0];
-    total_near = lo[1] + lc[1];
+    const long long total_far = lo[0] + lc[0], total_near = lo[1] + lc[1];
     if (static_cast<size_t>(total_far) + 1 > farcap) {
         (void)hipFree(far);
+        far = nullptr;
         farcap = static_cast<size_t>(total_far) + 1 + static_cast<size_t>(total_far) / 4;
         FMM_HIP(hipMalloc(&far, sizeof(int) * farcap), "allocate far lists");
     }
     if (static_cast<size_t>(total_near) + 1 > nearcap) {
         (void)hipFree(near);
+        near = nullptr;
         nearcap = static_cast<size_t>(total_near) + 1 + static_cast<size_t>(total_near) / 4;
         FMM_HIP(hipMalloc(&near, sizeof(int) * nearcap), "allocate near lists");
     }
-    hipLaunchKernelGGL((lists_kernel<true>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves, geom,
-                       mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
-                       near_off, far, near, frontier, fcap, overflow);
+    if (stash && !(lovf & 2)) {
+        hipLaunchKernelGGL(list_copy_kernel, dim3(nleaves), dim3(256), 0, st, nleaves, lwaves, nfar,
+                           nnear, far_off, near_off, lpos_far, lpos_near, wcap_far, wcap_near,
+                           stash_far, stash_near, far, near);
+    } else {
+        hipLaunchKernelGGL((lists_kernel<1>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves, geom,
+                           mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc, far_off,
+                           near_off, 0ll, 0ll, lpos_far, lpos_near, far, near, frontier, fcap,
+                           overflow);
+        // size the stash regions for the next builds from this build's
+        // per-wave totals, while they stay within 8x the lists' size
+        if (!wtot) FMM_HIP(hipMalloc(&wtot, sizeof(long long) * 2 * 4096), "allocate wave totals");
+        hipLaunchKernelGGL(wave_totals_kernel, dim3((lwaves + 255) / 256), dim3(256), 0, st, nleaves,
+                           lwaves, nfar, nnear, wtot);
+        std::vector<long long> wt(2 * static_cast<size_t>(lwaves));
+        FMM_HIP(hipMemcpyAsync(wt.data(), wtot, sizeof(long long) * wt.size(), hipMemcpyDeviceToHost, st),
+                "wave totals");
+        FMM_HIP(hipStreamSynchronize(st), "wave totals");
+        long long mf = 0, mn = 0;
+        for (int w = 0; w < lwaves; ++w) {
+            mf = std::max(mf, wt[w]);
+            mn = std::max(mn, wt[lwaves + w]);
+        }
+        wcap_far = mf + mf / 4 + 1024;
+        wcap_near = mn + mn / 4 + 1024;
+        if (static_cast<long long>(lwaves) * (wcap_far + wcap_near) > 8 * (total_far + total_near) + (1 << 20))
+            wcap_far = wcap_near = 0;
+    }
 
     // near-field work items: leaf li gets max(1, ceil(nnear / kNearItem))
     // items, each with one partial row per target of the leaf
