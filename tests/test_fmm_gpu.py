@@ -3,8 +3,9 @@
 The device tree, moments and lists follow the reference's geometric rules,
 so the FmmStats counts (node, leaf, far and near list totals) must equal the
 oracle's exactly.  The FP64 evaluation then agrees with the oracle to
-rounding (different but exact formulas: derivative tensors vs polynomial
-products), the FP32 ("mixed") evaluation to FP32 rounding.
+rounding (the same M2P through a different but exact formula: the radial
+structure of the softened kernel instead of explicit derivative tensors),
+the FP32 ("mixed") evaluation to FP32 rounding.
 """
 import numpy as np
 import pytest
@@ -117,3 +118,31 @@ def test_plummer_full_size_sampled(gpu):
     direct = O.direct_targets(pos, m, 1.0, 0.01, 0, 4096, threads=0)
     err = rel_err(got[:4096], direct[:4096])
     assert np.median(err) < 1e-4 and err.max() < 1e-2
+
+
+def test_one_traversal_lists_match_two_pass_and_regrow(gpu):
+    # An engine's first build counts, scans and then writes the interaction
+    # lists; later builds count and stash them per wave in one traversal
+    # (regions sized by the last two-pass build) and move them into place.
+    # On the same state both give bit-identical forces.  A problem 60x
+    # larger on the same engine outgrows the regions (each wave then holds
+    # several leaves' lists), so its first build falls back to writing; it
+    # must still match the FP64 evaluation, and its next build (regions
+    # resized) must repeat it bit for bit.
+    pos, vel, m = gpu.generators.plummer(3000, seed=11)
+    big = gpu.generators.plummer(200000, seed=12)
+    with gpu.Engine("hip-fmm", "mixed", softening=0.01, theta=0.5, leaf_capacity=16) as eng:
+        eng.upload(pos, vel, m)
+        eng.compute_accelerations()
+        a1 = eng.download()[2]
+        eng.compute_accelerations()
+        assert np.array_equal(eng.download()[2], a1)
+        eng.upload(*big)
+        eng.compute_accelerations()
+        b1 = eng.download()[2]
+        eng.compute_accelerations()
+        assert np.array_equal(eng.download()[2], b1)
+    b64, _ = _fmm(gpu, "fp64", big[0], big[2], 0.01, 0.5, 16)
+    assert rel_err(b1, b64).max() < MIXED_TOL
+    want, _ = O.fmm(pos, m, 1.0, 0.01, 0.5, 16, 32, 4)
+    assert rel_err(a1, want).max() < MIXED_TOL
