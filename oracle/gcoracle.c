@@ -149,6 +149,9 @@ int orc_compute_accelerations(const orc_solver* s, size_t n, const double* pos,
     case 2:
         return orc_fmm(n, pos, mass, s->G, s->eps, s->theta, s->leaf_capacity,
                        s->max_depth, s->expansion_order, 0, n, acc, NULL);
+    case 3:
+        return orc_tree(n, pos, mass, s->G, s->eps, s->theta, s->leaf_capacity,
+                        s->max_depth, s->expansion_order, 0, n, acc, NULL);
     default:
         return -1;
     }
@@ -712,6 +715,138 @@ int orc_fmm(size_t n, const double* pos, const double* mass, double G,
         stats->max_depth_reached = t.max_depth_reached;
     }
     free(leaves.v);
+    free(t.nodes);
+    free(t.idx);
+    free(t.scratch);
+    return 0;
+}
+
+/* ===================================================================== */
+/* Barnes-Hut tree (cpp/fmm/quadtree.cpp)                                 */
+/* ===================================================================== */
+
+typedef struct {
+    const otree* t;
+    double G, eps, theta;
+    int order;
+    size_t m2p, p2p;
+} bh_ctx;
+
+/* quadtree.cpp:193-243 — accumulate_from_node: a massless node gives
+ * nothing; a leaf sums its particles (skipping the target) in order; an
+ * internal node is one M2P when the target is outside its cube, at a
+ * non-zero distance from its centre of mass and width / distance < theta,
+ * and otherwise the sum of its children 0..7. */
+static void bh_accumulate(bh_ctx* b, long node, size_t target, const double* x, double out[3]) {
+    const otree* t = b->t;
+    const onode* nd = &t->nodes[node];
+    out[0] = out[1] = out[2] = 0.0;
+    if (nd->mass <= 0.0) return;
+    if (nd->child0 < 0) {
+        double f[3];
+        for (size_t p = nd->pbeg; p < nd->pbeg + nd->pcnt; ++p) {
+            const size_t sid = t->idx[p];
+            if (sid == target) continue;
+            orc_softened_acceleration(x, t->pos + 3 * sid, t->mass[sid], b->G, b->eps, f);
+            out[0] += f[0];
+            out[1] += f[1];
+            out[2] += f[2];
+            b->p2p++;
+        }
+        return;
+    }
+    const double dx = nd->com[0] - x[0], dy = nd->com[1] - x[1], dz = nd->com[2] - x[2];
+    const double distance = sqrt(dx * dx + dy * dy + dz * dz);
+    const double width = 2.0 * nd->hw;
+    /* quadtree.cpp:85-90 — contains */
+    const int inside = fabs(x[0] - nd->c[0]) <= nd->hw && fabs(x[1] - nd->c[1]) <= nd->hw &&
+                       fabs(x[2] - nd->c[2]) <= nd->hw;
+    if (!inside && distance > 0.0 && width / distance < b->theta) {
+        orc_m2p(x, nd->com, nd->mass, nd->mom, b->G, b->eps, b->order, out);
+        b->m2p++;
+        return;
+    }
+    double f[3];
+    for (int ch = 0; ch < 8; ++ch) {
+        bh_accumulate(b, nd->child0 + ch, target, x, f);
+        out[0] += f[0];
+        out[1] += f[1];
+        out[2] += f[2];
+    }
+}
+
+/* quadtree.cpp:12-35 (constructor clamps, compute), :37-81 (build: the same
+ * root cube and insertion rule as the FMM, so the same tree), :143-191
+ * (mass, centre of mass, P2M/M2M about the COM), and
+ * compute_tree_accelerations (:245-254, max_depth 32 there).  Every target
+ * in [begin, end) gets its own walk from the root. */
+int orc_tree(size_t n, const double* pos, const double* mass, double G, double eps,
+             double theta, size_t leaf_capacity, int max_depth, int expansion_order,
+             size_t begin, size_t end, double* acc, orc_tree_stats* stats) {
+    build_tables();
+    if (leaf_capacity < 1) leaf_capacity = 1;
+    if (max_depth < 1) max_depth = 1;
+    if (expansion_order < 0) expansion_order = 0;
+    if (expansion_order > 4) expansion_order = 4;
+    if (end > n) end = n;
+    if (stats) memset(stats, 0, sizeof(*stats));
+    for (size_t i = begin; i < end; ++i) acc[3 * i] = acc[3 * i + 1] = acc[3 * i + 2] = 0.0;
+    if (n == 0 || begin >= end) return 0;
+
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            const double x = pos[3 * i + d];
+            if (x < lo[d]) lo[d] = x;
+            if (hi[d] < x) hi[d] = x;
+        }
+    otree t;
+    memset(&t, 0, sizeof(t));
+    t.pos = pos;
+    t.mass = mass;
+    t.leaf_capacity = leaf_capacity;
+    t.max_depth = max_depth;
+    t.idx = (size_t*)malloc(n * sizeof(size_t));
+    t.scratch = (size_t*)malloc(n * sizeof(size_t));
+    if (!t.idx || !t.scratch) {
+        free(t.idx);
+        free(t.scratch);
+        return -2;
+    }
+    for (size_t i = 0; i < n; ++i) t.idx[i] = i;
+    long root = tree_push(&t);
+    {
+        onode* r = &t.nodes[root];
+        for (int d = 0; d < 3; ++d) r->c[d] = (lo[d] + hi[d]) * 0.5;
+        double ext = hi[0] - lo[0];
+        if (ext < hi[1] - lo[1]) ext = hi[1] - lo[1];
+        if (ext < hi[2] - lo[2]) ext = hi[2] - lo[2];
+        double hw = 0.5 * ext;
+        if (hw < eps + 1.0e-6) hw = eps + 1.0e-6;
+        r->hw = hw * 1.0001;
+    }
+    tree_build(&t, root, 0, n, 0);
+    tree_mass(&t, root);
+    tree_moments(&t, root);
+
+    size_t m2p = 0, p2p = 0;
+    const long long lo_i = (long long)begin, hi_i = (long long)end;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : m2p, p2p)
+    for (long long i = lo_i; i < hi_i; ++i) {
+        bh_ctx b = {&t, G, eps, theta, expansion_order, 0, 0};
+        bh_accumulate(&b, root, (size_t)i, pos + 3 * i, acc + 3 * i);
+        m2p += b.m2p;
+        p2p += b.p2p;
+    }
+    if (stats) {
+        size_t leaves = 0;
+        for (size_t k = 0; k < t.count; ++k) leaves += t.nodes[k].child0 < 0 && t.nodes[k].pcnt > 0;
+        stats->node_count = t.count;
+        stats->leaf_count = leaves;
+        stats->m2p_evaluations = m2p;
+        stats->p2p_pairs = p2p;
+        stats->max_depth_reached = t.max_depth_reached;
+    }
     free(t.nodes);
     free(t.idx);
     free(t.scratch);

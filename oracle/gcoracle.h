@@ -54,7 +54,7 @@ int orc_direct_targets_omp(size_t n, const double* pos, const double* mass,
 void orc_kick(size_t n, double* vel, const double* acc, double dt);
 void orc_drift(size_t n, double* pos, const double* vel, double dt);
 
-/* solver: 0 = direct symmetric, 1 = direct targets, 2 = fmm */
+/* solver: 0 = direct symmetric, 1 = direct targets, 2 = fmm, 3 = tree */
 typedef struct {
     int solver;
     double G, eps;
@@ -94,6 +94,17 @@ int orc_fmm(size_t n, const double* pos, const double* mass, double G,
             double eps, double theta, size_t leaf_capacity, int max_depth,
             int expansion_order, size_t begin, size_t end, double* acc,
             orc_fmm_stats* stats);
+
+/* ---- Barnes-Hut tree (cpp/fmm/quadtree.cpp) --------------------------- */
+typedef struct {
+    size_t node_count, leaf_count;  /* leaves holding particles */
+    size_t m2p_evaluations, p2p_pairs;
+    int max_depth_reached;
+} orc_tree_stats;
+
+int orc_tree(size_t n, const double* pos, const double* mass, double G, double eps,
+             double theta, size_t leaf_capacity, int max_depth, int expansion_order,
+             size_t begin, size_t end, double* acc, orc_tree_stats* stats);
 
 #ifdef __cplusplus
 }

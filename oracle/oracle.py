@@ -37,7 +37,7 @@ class Solver(C.Structure):
                 ("max_depth", C.c_int), ("expansion_order", C.c_int)]
 
 
-SOLVER_DIRECT_SYMMETRIC, SOLVER_DIRECT_TARGETS, SOLVER_FMM = 0, 1, 2
+SOLVER_DIRECT_SYMMETRIC, SOLVER_DIRECT_TARGETS, SOLVER_FMM, SOLVER_TREE = 0, 1, 2, 3
 
 _lib = None
 _ref = None
@@ -208,6 +208,30 @@ def fmm(pos, mass, G=1.0, eps=0.02, theta=0.6, leaf_capacity=16, max_depth=32,
     return acc, stats
 
 
+class TreeStats(C.Structure):
+    _fields_ = [("node_count", C.c_size_t), ("leaf_count", C.c_size_t),
+                ("m2p_evaluations", C.c_size_t), ("p2p_pairs", C.c_size_t),
+                ("max_depth_reached", C.c_int)]
+
+
+def tree(pos, mass, G=1.0, eps=0.02, theta=0.6, leaf_capacity=16, max_depth=32,
+         expansion_order=4, begin=0, end=None):
+    """Barnes-Hut accelerations (cpp/fmm/quadtree.cpp) and walk statistics."""
+    pos, mass = _c64(pos), _c64(mass)
+    n = len(mass)
+    end = n if end is None else end
+    acc = np.zeros_like(pos)
+    st = TreeStats()
+    rc = lib().orc_tree(_size_t(n), _p(pos), _p(mass), C.c_double(G), C.c_double(eps),
+                        C.c_double(theta), _size_t(leaf_capacity), C.c_int(max_depth),
+                        C.c_int(expansion_order), _size_t(begin), _size_t(end), _p(acc),
+                        C.byref(st))
+    if rc:
+        raise RuntimeError(f"oracle tree failed: {rc}")
+    stats = {k: getattr(st, k) for k, _ in TreeStats._fields_}
+    return acc, stats
+
+
 # ---------------------------------------------------------------- reference
 def ref_available() -> bool:
     return ref_lib() is not None
@@ -258,6 +282,15 @@ def ref_fmm(pos, mass, G=1.0, eps=0.02, theta=0.6, leaf_capacity=16, max_depth=3
                       C.c_int(expansion_order), _size_t(begin), _size_t(end), _p(acc), st)
     return acc, dict(node_count=st[0], leaf_count=st[1], far_interactions=st[2],
                      near_interactions=st[3])
+
+
+def ref_tree(pos, mass, G=1.0, eps=0.02, theta=0.6, leaf_capacity=16, expansion_order=4):
+    pos, mass = _c64(pos), _c64(mass)
+    acc = np.zeros_like(pos)
+    ref_lib().ref_tree(_size_t(len(mass)), _p(pos), _p(mass), C.c_double(G), C.c_double(eps),
+                       C.c_double(theta), _size_t(leaf_capacity), C.c_int(expansion_order),
+                       _p(acc))
+    return acc
 
 
 def ref_p2m(moments, offset, mass):

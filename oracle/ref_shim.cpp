@@ -11,6 +11,7 @@
 #include "direct/direct_solver.hpp"
 #include "fmm/fmm_solver.hpp"
 #include "fmm/multipole.hpp"
+#include "fmm/quadtree.hpp"
 
 #include <cstddef>
 #include <vector>
@@ -89,7 +90,7 @@ void ref_direct_targets(std::size_t n, const double* pos, const double* mass, do
     }
 }
 
-// solver: 0 direct (symmetric), 2 fmm
+// solver: 0 direct (symmetric), 2 fmm, 3 tree
 int ref_leapfrog(int solver, std::size_t n, double* pos, double* vel, double* acc,
                  const double* mass, double G, double eps, double theta, std::size_t leaf,
                  int order, double dt, int steps) {
@@ -99,6 +100,8 @@ int ref_leapfrog(int solver, std::size_t n, double* pos, double* vel, double* ac
     auto compute = [&](std::vector<Particle>& s) {
         if (solver == 2) {
             compute_fmm_accelerations(s, p, o);
+        } else if (solver == 3) {
+            compute_tree_accelerations(s, p, theta, leaf, order);
         } else {
             compute_direct_accelerations(s, p);
         }
@@ -146,6 +149,14 @@ int ref_fmm(std::size_t n, const double* pos, const double* mass, double G, doub
         stats4[3] = solver.stats().near_interactions;
     }
     return 0;
+}
+
+// compute_tree_accelerations (cpp/fmm/quadtree.cpp:245-254): every particle
+void ref_tree(std::size_t n, const double* pos, const double* mass, double G, double eps,
+              double theta, std::size_t leaf, int order, double* acc) {
+    auto ps = to_particles(n, pos, nullptr, nullptr, mass);
+    compute_tree_accelerations(ps, physics(G, eps), theta, leaf, order);
+    from_particles(ps, nullptr, nullptr, acc);
 }
 
 void ref_p2m(double* moments, const double* offset, double mass) {

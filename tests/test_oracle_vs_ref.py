@@ -72,3 +72,35 @@ def test_fmm_depth_cap_with_duplicates():
     got, st = O.fmm(pos, m, 1.0, 0.02, 0.5, 8, 6, 4)
     want, rst = O.ref_fmm(pos, m, 1.0, 0.02, 0.5, 8, 6, 4)
     assert np.array_equal(got, want) and st["node_count"] == rst["node_count"]
+
+
+@pytest.mark.parametrize("theta,leaf,order,eps", [(0.6, 16, 4, 0.02), (0.35, 4, 4, 0.01),
+                                                  (0.8, 32, 2, 0.0), (0.6, 1, 0, 0.02),
+                                                  (1.2, 8, 3, 0.05), (0.5, 8, 1, 0.02)])
+def test_tree(theta, leaf, order, eps):
+    # the Barnes-Hut walk per particle (cpp/fmm/quadtree.cpp): massless
+    # particles in massless leaves still get accelerations; p = 1 and 3 run
+    # as 2 and 4 (multipole.cpp:275-278)
+    pos, m = _cloud(1200, 7)
+    m[10:30] = 0.0
+    got, _ = O.tree(pos, m, 1.0, eps, theta, leaf, 32, order)
+    assert np.array_equal(got, O.ref_tree(pos, m, 1.0, eps, theta, leaf, order))
+
+
+def test_tree_duplicates_at_depth_cap_and_leapfrog():
+    # 40 coincident particles with leaves of 4 reach the depth cap of 32;
+    # with eps = 0 their pairs are skipped (s2 == 0) but every other pair
+    # counts; then 5 KDK steps through each solver's own loop
+    pos, m = _cloud(300, 8)
+    pos[:40] = pos[0]
+    got, st = O.tree(pos, m, 1.0, 0.0, 0.6, 4, 32, 4)
+    assert np.array_equal(got, O.ref_tree(pos, m, 1.0, 0.0, 0.6, 4, 4))
+    assert st["max_depth_reached"] == 32
+    vel = np.random.default_rng(9).normal(size=pos.shape) * 0.1
+    pos[:40] += np.random.default_rng(10).normal(size=(40, 3)) * 1e-3
+    acc, _ = O.tree(pos, m, 1.0, 0.02, 0.6, 8, 32, 4)
+    solver = O.make_solver(O.SOLVER_TREE, 1.0, 0.02, 0.6, 8, 32, 4)
+    mine = O.leapfrog(solver, pos, vel, acc, m, 0.002, 5)
+    ref = O.ref_leapfrog(3, pos, vel, acc, m, 1.0, 0.02, 0.002, 5, 0.6, 8, 4)
+    for a, b in zip(mine, ref):
+        assert np.array_equal(a, b)
