@@ -62,7 +62,7 @@ bool is_gfx950(int device) {
 gc_status check_params(gc_ctx* c, const gc_params* p) {
     if (!p) return fail(c, GC_ERR_INVALID_ARGUMENT, "params: null");
     if (!(p->softening >= 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "softening must be non-negative");
-    if (p->solver != GC_SOLVER_DIRECT && p->solver != GC_SOLVER_FMM)
+    if (p->solver != GC_SOLVER_DIRECT && p->solver != GC_SOLVER_FMM && p->solver != GC_SOLVER_TREE)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "unknown solver kind");
     if (p->precision != GC_PRECISION_FP64 && p->precision != GC_PRECISION_MIXED)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "unknown precision");
@@ -198,12 +198,14 @@ gc_status launch_status(gc_ctx* c, const char* what) {
 // a(x) for targets [begin, end); optionally the closing half kick.
 gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double half_dt) {
     const gc_params& p = c->params;
-    if (p.solver == GC_SOLVER_FMM) {
+    if (p.solver == GC_SOLVER_FMM || p.solver == GC_SOLVER_TREE) {
         if (!c->fmm) c->fmm = new (std::nothrow) gc::FmmEngine();
         if (!c->fmm) return fail(c, GC_ERR_OUT_OF_MEMORY, "allocate FMM engine");
         std::string msg;
         const int rc = c->fmm->compute(c->s, p, begin, end, close_kick, half_dt, c->stream, &msg);
-        if (rc != GC_OK) return fail(c, static_cast<gc_status>(rc), "hip-fmm: " + msg);
+        if (rc != GC_OK)
+            return fail(c, static_cast<gc_status>(rc),
+                        (p.solver == GC_SOLVER_TREE ? "hip-tree: " : "hip-fmm: ") + msg);
         return GC_OK;
     }
     if (p.precision == GC_PRECISION_FP64) {
@@ -492,7 +494,7 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
     if (reps <= 0 || !ms) return fail(c, GC_ERR_INVALID_ARGUMENT, "time_force_kernel: bad arguments");
     const size_t n = c->s.n;
     if (n == 0) return fail(c, GC_ERR_STATE, "time_force_kernel: no particles");
-    if (c->params.solver == GC_SOLVER_FMM) {
+    if (c->params.solver == GC_SOLVER_FMM || c->params.solver == GC_SOLVER_TREE) {
         if (!c->fmm) c->fmm = new (std::nothrow) gc::FmmEngine();
         if (!c->fmm) return fail(c, GC_ERR_OUT_OF_MEMORY, "allocate FMM engine");
         std::string msg;

@@ -977,6 +977,174 @@ __global__ __launch_bounds__(256) void gather_sorted_kernel(
 
 // zero accelerations of [begin, end) (particles in massless leaves keep 0,
 // as in fmm_solver.cpp:31 + :210-226)
+// ---- Barnes-Hut (cpp/fmm/quadtree.cpp) on the same tree -------------------
+// The reference's tree solver builds the FMM's tree (same root cube and
+// insertion rule), and walks it once per target particle
+// (accumulate_from_node, quadtree.cpp:193-243): a massless node gives
+// nothing; a leaf gives its particles' pair forces (self skipped); an
+// internal node gives one M2P when the target lies outside its cube, at a
+// non-zero distance from its centre of mass and width / distance < theta,
+// and otherwise its children's sums.
+//
+// Here one wave walks for the <= 64 targets of a leaf chunk, one per lane,
+// depth first with a stack in LDS of (node, mask of lanes that still need
+// it).  At an internal node each lane takes the reference's decision for
+// its own target (in FP64, so decisions match it exactly); lanes that accept
+// evaluate the M2P from the node's coefficient table (far_m2p.hpp, read by
+// scalar loads since the node is uniform), the others carry the node's
+// children.  A leaf gives the pair forces to the lanes still carrying it.
+// Every target therefore sees exactly the reference's cells and leaves;
+// only the order of the sums differs.
+constexpr int kTreeStack = 256;  // > 21 levels x 7 siblings + 1
+
+// width / distance < theta on the reference's roundings (quadtree.cpp:
+// 222-228): decided from width^2 against theta^2 d2 when they differ by
+// more than any rounding can move them, else by the exact expression.
+__device__ __forceinline__ bool tree_accepts(double dx, double dy, double dz, double width,
+                                             double theta) {
+#pragma clang fp contract(off)
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 > 0.0)) return false;  // distance > 0
+    const double lhs = width * width, rhs = theta * theta * d2;
+    if (lhs < rhs * (1.0 - 1e-12)) return true;
+    if (lhs > rhs * (1.0 + 1e-12)) return false;
+    return width / __builtin_sqrt(d2) < theta;
+}
+
+template <typename F, int ORDER, bool SAFE>
+__global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
+    int nnodes, const double* __restrict__ geom, const double* __restrict__ mcom,
+    const F* __restrict__ kcoef, const int* __restrict__ nb, const int* __restrict__ ne,
+    const int* __restrict__ first_child, const int* __restrict__ nchild,
+    const int* __restrict__ sidx, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, const double* __restrict__ sm, double G, double eps,
+    double theta, int begin, int end, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, int close_kick, double half_dt,
+    unsigned long long* __restrict__ counters) {
+#pragma clang fp contract(off)
+    __shared__ int stack_node[kTreeStack];
+    __shared__ unsigned long long stack_mask[kTreeStack];
+    const int leaf = blockIdx.x;
+    if (leaf >= nnodes || nchild[leaf] != 0) return;
+    const int tb = nb[leaf], te = ne[leaf];
+    if (te <= tb) return;
+    const int lane = threadIdx.x;
+    const F eps2 = static_cast<F>(eps * eps);
+    // offsets are taken from the target leaf's centre, so FP32 differences
+    // stay small for nearby cells and sources
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    unsigned long long n_m2p = 0, n_p2p = 0;
+    for (int base = tb; base < te; base += kEvalLanes) {
+        const int c = min(kEvalLanes, te - base);
+        const bool active = lane < c;
+        const int me = base + (active ? lane : 0);  // sorted index of this lane's target
+        const double x = sx[me], y = sy[me], z = sz[me];
+        const F tx = static_cast<F>(x - cx), ty = static_cast<F>(y - cy), tz = static_cast<F>(z - cz);
+        double a[3] = {0.0, 0.0, 0.0};
+        int sp = 0;
+        if (lane == 0) {
+            stack_node[0] = 0;
+            stack_mask[0] = c == 64 ? ~0ull : ((1ull << c) - 1ull);
+        }
+        __syncthreads();
+        sp = 1;
+        while (sp > 0) {
+            --sp;
+            const int node = __builtin_amdgcn_readfirstlane(stack_node[sp]);
+            const unsigned long long mask = stack_mask[sp];
+            __syncthreads();  // the slot may be overwritten by the pushes below
+            const bool mine = (mask >> lane) & 1ull;
+            const double mass = mcom[4 * node];
+            if (!(mass > 0.0)) continue;
+            const int nc = nchild[node];
+            if (nc == 0) {
+                // leaf: pair forces for the lanes still carrying it
+                const int sb = nb[node], se = ne[node];
+                F px = 0, py = 0, pz = 0;
+                for (int j = sb; j < se; ++j) {
+                    const F qx = static_cast<F>(sx[j] - cx), qy = static_cast<F>(sy[j] - cy),
+                            qz = static_cast<F>(sz[j] - cz);
+                    const F gm = static_cast<F>(sm[j]);  // G is applied to the whole sum
+                    const F dx = qx - tx, dy = qy - ty, dz = qz - tz;
+                    const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+                    F r;
+                    if (SAFE) {
+                        r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
+                    } else {
+                        r = gc_rsq(s2);
+                    }
+                    r = (j == me || !mine) ? F(0) : r;
+                    const F kk = (gm * r) * (r * r);
+                    px = gc_fma(dx, kk, px);
+                    py = gc_fma(dy, kk, py);
+                    pz = gc_fma(dz, kk, pz);
+                }
+                a[0] += static_cast<double>(px);
+                a[1] += static_cast<double>(py);
+                a[2] += static_cast<double>(pz);
+                // directed pairs, self excluded: only the target leaf holds the targets
+                const unsigned long long carried = static_cast<unsigned long long>(__popcll(mask));
+                n_p2p += carried * (se - sb) - (node == leaf ? carried : 0ull);
+                continue;
+            }
+            const double ccx = geom[4 * node], ccy = geom[4 * node + 1], ccz = geom[4 * node + 2];
+            const double hw = geom[4 * node + 3];
+            const double comx = mcom[4 * node + 1], comy = mcom[4 * node + 2],
+                         comz = mcom[4 * node + 3];
+            bool accept = false;
+            if (mine) {
+                const bool inside = __builtin_fabs(x - ccx) <= hw && __builtin_fabs(y - ccy) <= hw &&
+                                    __builtin_fabs(z - ccz) <= hw;
+                accept = !inside && tree_accepts(comx - x, comy - y, comz - z, 2.0 * hw, theta);
+            }
+            const unsigned long long amask = __ballot(accept);
+            if (amask) {
+                const F* K = kcoef + static_cast<size_t>(kFarCoef) * node;
+                const F yx = static_cast<F>(comx - cx) - tx, yy = static_cast<F>(comy - cy) - ty,
+                        yz = static_cast<F>(comz - cz) - tz;
+                F fx = 0, fy = 0, fz = 0;
+                if (accept) far_m2p<F, ORDER>(yx, yy, yz, eps2, K, fx, fy, fz);
+                a[0] += static_cast<double>(fx);
+                a[1] += static_cast<double>(fy);
+                a[2] += static_cast<double>(fz);
+                n_m2p += static_cast<unsigned long long>(__popcll(amask));
+            }
+            const unsigned long long rest = mask & ~amask;
+            if (rest) {
+                // children in reverse so that child 0 is walked first
+                const int fc = first_child[node];
+                if (lane < nc) {
+                    stack_node[sp + nc - 1 - lane] = fc + lane;
+                    stack_mask[sp + nc - 1 - lane] = rest;
+                }
+                sp += nc;
+                __syncthreads();
+            }
+        }
+        if (active) {
+            const int id = sidx[me];
+            if (id >= begin && id < end) {
+                const double gx = G * a[0], gy = G * a[1], gz = G * a[2];
+                ax[id] = gx;
+                ay[id] = gy;
+                az[id] = gz;
+                if (close_kick) {
+                    vx[id] = vx[id] + gx * half_dt;
+                    vy[id] = vy[id] + gy * half_dt;
+                    vz[id] = vz[id] + gz * half_dt;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        atomicAdd(counters + 0, n_m2p);
+        atomicAdd(counters + 1, n_p2p);
+        atomicAdd(counters + 2, 1ull);  // leaves holding particles
+    }
+}
+
 __global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, double* __restrict__ ax,
                                                          double* __restrict__ ay,
                                                          double* __restrict__ az) {
@@ -1082,6 +1250,10 @@ struct FmmEngine::Impl {
     double* npart = nullptr;
     int nitems = 0;
     unsigned long long* stats_dev = nullptr;
+    // Barnes-Hut work counts (M2P, pairs, leaves) of the last evaluation, in
+    // pinned host memory; read once the evaluation has finished
+    unsigned long long* tree_counts = nullptr;
+    bool tree_counts_pending = false;
     // host-side results of the last build
     std::vector<int> level_begin;
     int nnodes = 0, nleaves = 0;
@@ -1094,6 +1266,7 @@ struct FmmEngine::Impl {
                         scan_tmp, overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
                         near_off, frontier, far, near, stash_far, stash_near, lpos_far, lpos_near, wtot, icount, ioff, row_off, items, npart, stats_dev};
         for (void* p : ptrs) (void)hipFree(p);
+        if (tree_counts) (void)hipHostFree(tree_counts);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -1159,6 +1332,8 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         FMM_HIP(hipMalloc(&sort_tmp, sort_tmp_bytes), "allocate radix sort storage");
         if (!overflow) FMM_HIP(hipMalloc(&overflow, sizeof(int)), "allocate flag");
         if (!stats_dev) FMM_HIP(hipMalloc(&stats_dev, sizeof(unsigned long long) * 4), "allocate stats");
+        if (!tree_counts)
+            FMM_HIP(hipHostMalloc(&tree_counts, sizeof(unsigned long long) * 4), "allocate counts");
     }
     const size_t need_nodes = std::max<size_t>(1024, 2 * static_cast<size_t>(n) / cap + 64);
     if (need_nodes > ncap) FMM_HIP(grow_nodes(need_nodes, 0, st), "allocate nodes");
@@ -1240,6 +1415,22 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     }
     hipLaunchKernelGGL(far_coef_kernel, dim3((nnodes + 63) / 64), dim3(64), 0, st, nnodes, mcom, mom,
                        kc64, kc32);
+    if (p.solver == GC_SOLVER_TREE) {
+        // the Barnes-Hut walk needs no lists; its work counts come from the
+        // evaluation (tree_stats)
+        std::vector<int> nch(static_cast<size_t>(nnodes));
+        FMM_HIP(hipMemcpyAsync(nch.data(), nchild, sizeof(int) * nnodes, hipMemcpyDeviceToHost, st),
+                "read tree");
+        FMM_HIP(hipStreamSynchronize(st), "tree");
+        internal = 0;
+        for (int v : nch) internal += v > 0 ? 1 : 0;
+        last = gc_fmm_stats{};
+        last.node_count = 1 + 8 * static_cast<size_t>(internal);
+        last.max_depth = depth;
+        last.tree_nodes = static_cast<size_t>(nnodes);
+        nleaves = nnodes;  // evaluation grid: every node, leaves holding particles act
+        return GC_OK;
+    }
 
     // leaves (mass > 0)
     if (static_cast<size_t>(nnodes) > leafcap) {
@@ -1495,6 +1686,35 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const dim3 grid(static_cast<unsigned>(nleaves)), block(kEvalLanes);
     const int b = static_cast<int>(begin), e = static_cast<int>(end);
     const double G = p.gravitational_constant, eps = p.softening;
+    if (p.solver == GC_SOLVER_TREE) {
+        FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear counts");
+        const double theta = p.theta;
+#define GC_TREE(F, O, S, KC)                                                                     \
+    hipLaunchKernelGGL((tree_eval_kernel<F, O, S>), dim3(static_cast<unsigned>(nnodes)), block, 0,  \
+                       st, nnodes, geom, mcom, KC, nb, ne, first_child, nchild, idx_sorted, sxs,    \
+                       sys, szs, sms, G, eps, theta, b, e, s.ax, s.ay, s.az, s.vx, s.vy, s.vz,     \
+                       close_kick ? 1 : 0, half_dt, stats_dev)
+#define GC_TREE_O(F, S, KC)                                                                      \
+    do {                                                                                         \
+        if (order == 0) GC_TREE(F, 0, S, KC);                                                    \
+        else if (order == 2) GC_TREE(F, 2, S, KC);                                               \
+        else GC_TREE(F, 4, S, KC);                                                               \
+    } while (0)
+        if (mixed) {
+            if (safe) GC_TREE_O(float, true, kc32);
+            else GC_TREE_O(float, false, kc32);
+        } else {
+            if (safe) GC_TREE_O(double, true, kc64);
+            else GC_TREE_O(double, false, kc64);
+        }
+#undef GC_TREE_O
+#undef GC_TREE
+        FMM_HIP(hipGetLastError(), "launch tree evaluation");
+        FMM_HIP(hipMemcpyAsync(tree_counts, stats_dev, sizeof(unsigned long long) * 3,
+                               hipMemcpyDeviceToHost, st), "tree counts");
+        tree_counts_pending = true;
+        return GC_OK;
+    }
 #define GC_FAR(F, O, KC)                                                                         \
     hipLaunchKernelGGL((eval_far_kernel<F, O>), grid, block, 0, st, leaves, geom, mcom, KC, nb, ne, \
                        idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, s.ax, s.ay,    \
@@ -1528,7 +1748,15 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
 FmmEngine::FmmEngine() : impl_(new Impl()) {}
 FmmEngine::~FmmEngine() { delete impl_; }
 
-const gc_fmm_stats& FmmEngine::stats() const { return impl_->last; }
+const gc_fmm_stats& FmmEngine::stats() {
+    if (impl_->tree_counts_pending && impl_->ev[2] && hipEventSynchronize(impl_->ev[2]) == hipSuccess) {
+        impl_->last.m2p_evaluations = impl_->tree_counts[0];
+        impl_->last.p2p_pairs = impl_->tree_counts[1];
+        impl_->last.leaf_count = impl_->tree_counts[2];
+        impl_->tree_counts_pending = false;
+    }
+    return impl_->last;
+}
 
 int FmmEngine::compute(const DeviceState& s, const gc_params& p, size_t begin, size_t end,
                        bool close_kick, double half_dt, hipStream_t st, std::string* err) {
@@ -1568,7 +1796,7 @@ int FmmEngine::time_kernels(const DeviceState& s, const gc_params& p, int reps, 
     float t = 0.f;
     FMM_HIP(hipEventElapsedTime(&t, impl_->ev[3], impl_->ev[4]), "elapsed");
     *ms = static_cast<double>(t) / reps;
-    if (interactions) *interactions = static_cast<double>(impl_->last.p2p_pairs);
+    if (interactions) *interactions = static_cast<double>(stats().p2p_pairs);
     return GC_OK;
 }
 

@@ -20,7 +20,7 @@ GC_ERR_OUT_OF_MEMORY = 4
 GC_ERR_UNSUPPORTED = 5
 GC_ERR_STATE = 6
 
-GC_SOLVER_DIRECT, GC_SOLVER_FMM = 0, 1
+GC_SOLVER_DIRECT, GC_SOLVER_FMM, GC_SOLVER_TREE = 0, 1, 2
 GC_PRECISION_FP64, GC_PRECISION_MIXED = 0, 1
 GC_SOURCES_AUTO, GC_SOURCES_LDS, GC_SOURCES_SCALAR = 0, 1, 2
 GC_PAIRS_AUTO, GC_PAIRS_TARGETS, GC_PAIRS_SYMMETRIC = 0, 1, 2

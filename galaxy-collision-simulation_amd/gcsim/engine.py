@@ -22,7 +22,8 @@ _dp = C.POINTER(C.c_double)
 
 SOLVERS = {"hip-direct": L.GC_SOLVER_DIRECT, "hip": L.GC_SOLVER_DIRECT,
            "rocm-direct": L.GC_SOLVER_DIRECT, "hip-fmm": L.GC_SOLVER_FMM,
-           "rocm-fmm": L.GC_SOLVER_FMM}
+           "rocm-fmm": L.GC_SOLVER_FMM, "hip-tree": L.GC_SOLVER_TREE,
+           "rocm-tree": L.GC_SOLVER_TREE}
 PAIRS = {"auto": L.GC_PAIRS_AUTO, "targets": L.GC_PAIRS_TARGETS, "symmetric": L.GC_PAIRS_SYMMETRIC}
 SOURCES = {"auto": L.GC_SOURCES_AUTO, "lds": L.GC_SOURCES_LDS, "scalar": L.GC_SOURCES_SCALAR}
 PRECISIONS = {"fp64": L.GC_PRECISION_FP64, "double": L.GC_PRECISION_FP64,
@@ -52,7 +53,7 @@ def make_params(solver="hip-direct", precision="fp64", G=1.0, softening=0.02, th
                 leaf_capacity=16, max_depth=32, expansion_order=4) -> L.Params:
     key = solver.lower()
     if key not in SOLVERS:
-        raise ValueError(f"Unknown solver '{solver}'. Use hip-direct or hip-fmm.")
+        raise ValueError(f"Unknown solver '{solver}'. Use hip-direct, hip-fmm or hip-tree.")
     pkey = precision.lower()
     if pkey not in PRECISIONS:
         raise ValueError(f"Unknown precision '{precision}'. Use fp64 or mixed.")

@@ -42,8 +42,15 @@ typedef enum {
     GC_ERR_STATE = 6
 } gc_status;
 
-/* solver choice: the `solver = "hip-direct" | "hip-fmm"` config values */
-typedef enum { GC_SOLVER_DIRECT = 0, GC_SOLVER_FMM = 1 } gc_solver_kind;
+/* solver choice: the `solver = "hip-direct" | "hip-fmm" | "hip-tree"` config
+ * values.  GC_SOLVER_TREE is the reference's Barnes-Hut solver
+ * (compute_tree_accelerations, cpp/fmm/quadtree.cpp:245-254, selected by
+ * solver = "tree"): the FMM's tree, walked per target particle with the
+ * acceptance width / |com - x| < theta outside the cell, M2P of order
+ * expansion_order, pair forces in opened leaves.  It uses theta,
+ * leaf_capacity, max_depth and expansion_order below (the reference fixes
+ * max_depth at 32). */
+typedef enum { GC_SOLVER_DIRECT = 0, GC_SOLVER_FMM = 1, GC_SOLVER_TREE = 2 } gc_solver_kind;
 
 /* GC_PRECISION_FP64: FP64 pairs, per target ascending-j sums, the same
  *   operation order as compute_direct_accelerations_for_targets
@@ -204,7 +211,10 @@ gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms_per_launc
                                      double* interactions_per_launch);
 
 /* FmmStats (cpp/fmm/fmm_solver.hpp:20-25) of the last hip-fmm force
- * evaluation, plus the particle-level work it implied. */
+ * evaluation, plus the particle-level work it implied.  After a hip-tree
+ * evaluation: node_count, tree_nodes and max_depth as for hip-fmm;
+ * leaf_count = leaves holding particles; far/near_interactions = 0 (no
+ * lists); m2p_evaluations and p2p_pairs from the walks. */
 typedef struct {
     size_t node_count;        /* the reference's count: 1 + 8 * internal nodes */
     size_t leaf_count;        /* leaves with mass > 0 (target leaves) */
