@@ -112,6 +112,60 @@ def workload_fmm(args):
     }
 
 
+def workload_tree(args):
+    """Barnes-Hut (the reference's tree solver) on the FMM workload's Plummer
+    sphere N = 2^20 with the tree's defaults (theta 0.6, leaves <= 16, p = 4),
+    eps 0.01: 3 KDK steps at dt 1e-3, FP32 evaluation and FP64 state;
+    accuracy on 4096 sampled targets against FP64 direct sums and the
+    oracle's tree."""
+    import gcsim
+
+    n, eps, dt, steps = 1 << 20, 0.01, 1e-3, 3
+    pos, vel, mass = gcsim.generators.plummer(n, seed=1)
+    eng = gcsim.Engine("hip-tree", "mixed", softening=eps, theta=0.6, leaf_capacity=16,
+                       expansion_order=4)
+    eng.upload(pos, vel, mass)
+    eng.compute_accelerations()
+    _, _, a0 = eng.download()
+    st0 = eng.fmm_stats()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.step(dt, steps)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    st = eng.fmm_stats()
+    k_ms, _ = eng.time_force_kernel(reps=2)
+    eng.close()
+    O = _oracle()
+    threads = O.direct_targets_omp_threads(0)
+    sample = 4096
+    direct = O.direct_targets(pos, mass, 1.0, eps, 0, sample, threads=threads)
+    err = _rel_err(a0[:sample], direct[:sample])
+    tc = time.perf_counter()
+    acc_cpu, _ = O.tree(pos, mass, 1.0, eps, 0.6, 16, 32, 4, begin=0, end=sample)
+    cpu_s = time.perf_counter() - tc
+    err_vs_cpu = _rel_err(a0[:sample], acc_cpu[:sample])
+    flops = 20.0 * st0["p2p_pairs"] + 440.0 * st0["m2p_evaluations"]
+    return {
+        "workload": "tree: Plummer N=2^20, leaf<=16, p=4, theta=0.6, eps=0.01, dt=1e-3, 3 KDK",
+        "ms_per_step": wall * 1e3 / steps,
+        "eval_kernel_ms": k_ms,
+        "build_ms": st["build_ms"], "eval_ms": st["eval_ms"],
+        "p2p_pairs": st0["p2p_pairs"], "m2p_evaluations": st0["m2p_evaluations"],
+        "Gp2p_per_s_eval": st0["p2p_pairs"] / (k_ms * 1e-3) / 1e9,
+        "Gm2p_per_s_eval": st0["m2p_evaluations"] / (k_ms * 1e-3) / 1e9,
+        "tflops_eval_20_440": flops / (k_ms * 1e-3) / 1e12,
+        "stats": {k: st0[k] for k in ("node_count", "leaf_count", "max_depth")},
+        "accuracy_vs_fp64_direct_4096": {"max": float(err.max()), "median": float(np.median(err)),
+                                         "p95": float(np.percentile(err, 95))},
+        "fp32_vs_oracle_tree_4096_max": float(err_vs_cpu.max()),
+        "cpu_baseline": {"kind": "port", "cores": threads,
+                         "sample": f"oracle tree, full build, {sample} targets",
+                         "seconds": cpu_s,
+                         "extrapolated_s_per_evaluation": cpu_s * n / sample},
+    }
+
+
 def workload_batch(args):
     """256 collisions, N log-uniform in [512, 8192], eps 0.05, dt 1e-3: one
     persistent launch per step, hipGraph 10-step chunks, autotuned schedule;
@@ -301,7 +355,7 @@ def main() -> int:
     ap.add_argument("--pairs", default="auto", choices=["auto", "targets", "symmetric"],
                     help="mixed force form: per target, or each unordered pair once")
     ap.add_argument("--workload", default="direct",
-                    choices=["direct", "fmm", "batch", "adjoint", "drift", "copy"],
+                    choices=["direct", "fmm", "tree", "batch", "adjoint", "drift", "copy"],
                     help="direct = the headline line; the others print their own records")
     args = ap.parse_args()
 
@@ -317,7 +371,8 @@ def main() -> int:
         print("bench.py: no gfx950 device visible", file=sys.stderr)
         return 1
     if args.workload != "direct":
-        fn = {"fmm": workload_fmm, "batch": workload_batch, "adjoint": workload_adjoint,
+        fn = {"fmm": workload_fmm, "tree": workload_tree, "batch": workload_batch,
+              "adjoint": workload_adjoint,
               "drift": workload_drift, "copy": workload_copy}[args.workload]
         print(json.dumps(fn(args)))
         return 0

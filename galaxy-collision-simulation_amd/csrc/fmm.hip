@@ -986,9 +986,11 @@ __global__ __launch_bounds__(256) void gather_sorted_kernel(
 // non-zero distance from its centre of mass and width / distance < theta,
 // and otherwise its children's sums.
 //
-// Here one wave walks for the <= 64 targets of a leaf chunk, one per lane,
-// depth first with a stack in LDS of (node, mask of lanes that still need
-// it).  At an internal node each lane takes the reference's decision for
+// Here one wave walks for the targets of a group, one per lane: the
+// largest subtrees holding at most 64 particles (a node of <= 64 whose
+// parent holds more, or the root), so that leaves of a few particles share
+// a wave and every target lies in the group's cube.  The walk is depth
+// first with a stack in LDS of (node, mask of lanes that still need it).  At an internal node each lane takes the reference's decision for
 // its own target (in FP64, so decisions match it exactly); lanes that accept
 // evaluate the M2P from the node's coefficient table (far_m2p.hpp, read by
 // scalar loads since the node is uniform), the others carry the node's
@@ -1013,10 +1015,10 @@ __device__ __forceinline__ bool tree_accepts(double dx, double dy, double dz, do
 
 template <typename F, int ORDER, bool SAFE>
 __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
-    int nnodes, const double* __restrict__ geom, const double* __restrict__ mcom,
-    const F* __restrict__ kcoef, const int* __restrict__ nb, const int* __restrict__ ne,
-    const int* __restrict__ first_child, const int* __restrict__ nchild,
-    const int* __restrict__ sidx, const double* __restrict__ sx, const double* __restrict__ sy,
+    int nnodes, const int* __restrict__ group, const double* __restrict__ geom,
+    const double* __restrict__ mcom, const F* __restrict__ kcoef, const int* __restrict__ nb,
+    const int* __restrict__ ne, const int* __restrict__ first_child,
+    const int* __restrict__ nchild, const int* __restrict__ sidx, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, const double* __restrict__ sm, double G, double eps,
     double theta, int begin, int end, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, double* __restrict__ vx, double* __restrict__ vy,
@@ -1025,8 +1027,8 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
 #pragma clang fp contract(off)
     __shared__ int stack_node[kTreeStack];
     __shared__ unsigned long long stack_mask[kTreeStack];
-    const int leaf = blockIdx.x;
-    if (leaf >= nnodes || nchild[leaf] != 0) return;
+    const int leaf = blockIdx.x;  // the group's root node
+    if (leaf >= nnodes || !group[leaf]) return;
     const int tb = nb[leaf], te = ne[leaf];
     if (te <= tb) return;
     const int lane = threadIdx.x;
@@ -1083,9 +1085,11 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
                 a[0] += static_cast<double>(px);
                 a[1] += static_cast<double>(py);
                 a[2] += static_cast<double>(pz);
-                // directed pairs, self excluded: only the target leaf holds the targets
+                // directed pairs, self excluded
                 const unsigned long long carried = static_cast<unsigned long long>(__popcll(mask));
-                n_p2p += carried * (se - sb) - (node == leaf ? carried : 0ull);
+                const unsigned long long self =
+                    static_cast<unsigned long long>(__popcll(mask & __ballot(me >= sb && me < se)));
+                n_p2p += carried * (se - sb) - self;
                 continue;
             }
             const double ccx = geom[4 * node], ccy = geom[4 * node + 1], ccz = geom[4 * node + 2];
@@ -1141,8 +1145,30 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
     if (lane == 0) {
         atomicAdd(counters + 0, n_m2p);
         atomicAdd(counters + 1, n_p2p);
-        atomicAdd(counters + 2, 1ull);  // leaves holding particles
     }
+}
+
+// Walk groups: the root if it holds <= 64 particles, else every child of a
+// node holding more than 64 that holds at most 64 (empty children are not
+// materialised).  Also counts the leaves holding particles.
+__global__ __launch_bounds__(256) void tree_group_kernel(int nnodes, const int* __restrict__ nb,
+                                                         const int* __restrict__ ne,
+                                                         const int* __restrict__ first_child,
+                                                         const int* __restrict__ nchild,
+                                                         int* __restrict__ group,
+                                                         unsigned long long* __restrict__ counters) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= nnodes) return;
+    const int cnt = ne[k] - nb[k];
+    if (k == 0) group[0] = cnt <= kEvalLanes ? 1 : 0;
+    const int nc = nchild[k];
+    if (nc == 0) {
+        if (cnt > 0) atomicAdd(counters + 2, 1ull);
+        return;
+    }
+    const int fc = first_child[k];
+    for (int c = fc; c < fc + nc; ++c)
+        group[c] = (cnt > kEvalLanes && ne[c] - nb[c] <= kEvalLanes) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, double* __restrict__ ax,
@@ -1269,6 +1295,27 @@ struct FmmEngine::Impl {
         if (tree_counts) (void)hipHostFree(tree_counts);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+    }
+
+    // Per-node leaf and list-count arrays for `nodes` nodes (contents not kept).
+    hipError_t grow_leaf_arrays(size_t nodes) {
+        if (nodes <= leafcap) return hipSuccess;
+        void* ptrs[] = {flag, flag_pos, leaves, nfar, nnear, nsrc, far_off, near_off};
+        for (void* q : ptrs) (void)hipFree(q);
+        flag = flag_pos = leaves = nfar = nnear = nsrc = nullptr;
+        far_off = near_off = nullptr;
+        leafcap = 0;
+        const size_t cap = nodes * 2;
+        hipError_t e = hipMalloc(&flag, sizeof(int) * cap);
+        if (e == hipSuccess) e = hipMalloc(&flag_pos, sizeof(int) * cap);
+        if (e == hipSuccess) e = hipMalloc(&leaves, sizeof(int) * cap);
+        if (e == hipSuccess) e = hipMalloc(&nfar, sizeof(int) * cap);
+        if (e == hipSuccess) e = hipMalloc(&nnear, sizeof(int) * cap);
+        if (e == hipSuccess) e = hipMalloc(&nsrc, sizeof(int) * cap);
+        if (e == hipSuccess) e = hipMalloc(&far_off, sizeof(long long) * cap);
+        if (e == hipSuccess) e = hipMalloc(&near_off, sizeof(long long) * cap);
+        if (e == hipSuccess) leafcap = cap;
+        return e;
     }
 
     // Grows every node array to `nodes` entries keeping the first `keep`
@@ -1433,19 +1480,7 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     }
 
     // leaves (mass > 0)
-    if (static_cast<size_t>(nnodes) > leafcap) {
-        void* ptrs[] = {flag, flag_pos, leaves, nfar, nnear, nsrc, far_off, near_off};
-        for (void* q : ptrs) (void)hipFree(q);
-        leafcap = static_cast<size_t>(nnodes) * 2;
-        FMM_HIP(hipMalloc(&flag, sizeof(int) * leafcap), "allocate leaves");
-        FMM_HIP(hipMalloc(&flag_pos, sizeof(int) * leafcap), "allocate leaves");
-        FMM_HIP(hipMalloc(&leaves, sizeof(int) * leafcap), "allocate leaves");
-        FMM_HIP(hipMalloc(&nfar, sizeof(int) * leafcap), "allocate lists");
-        FMM_HIP(hipMalloc(&nnear, sizeof(int) * leafcap), "allocate lists");
-        FMM_HIP(hipMalloc(&nsrc, sizeof(int) * leafcap), "allocate lists");
-        FMM_HIP(hipMalloc(&far_off, sizeof(long long) * leafcap), "allocate lists");
-        FMM_HIP(hipMalloc(&near_off, sizeof(long long) * leafcap), "allocate lists");
-    }
+    FMM_HIP(grow_leaf_arrays(static_cast<size_t>(nnodes)), "allocate leaves");
     {
         size_t need = 0;
         FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, flag_pos, nnodes, st), "size scan");
@@ -1688,10 +1723,13 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const double G = p.gravitational_constant, eps = p.softening;
     if (p.solver == GC_SOLVER_TREE) {
         FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear counts");
+        FMM_HIP(grow_leaf_arrays(static_cast<size_t>(nnodes)), "allocate walk groups");
+        hipLaunchKernelGGL(tree_group_kernel, dim3((nnodes + 255) / 256), dim3(256), 0, st, nnodes,
+                           nb, ne, first_child, nchild, flag, stats_dev);
         const double theta = p.theta;
 #define GC_TREE(F, O, S, KC)                                                                     \
     hipLaunchKernelGGL((tree_eval_kernel<F, O, S>), dim3(static_cast<unsigned>(nnodes)), block, 0,  \
-                       st, nnodes, geom, mcom, KC, nb, ne, first_child, nchild, idx_sorted, sxs,    \
+                       st, nnodes, flag, geom, mcom, KC, nb, ne, first_child, nchild, idx_sorted, sxs,    \
                        sys, szs, sms, G, eps, theta, b, e, s.ax, s.ay, s.az, s.vx, s.vy, s.vz,     \
                        close_kick ? 1 : 0, half_dt, stats_dev)
 #define GC_TREE_O(F, S, KC)                                                                      \

@@ -22,7 +22,9 @@ gc_params make_params(const PhysicsParams& p, HipForceSolver::Kind kind, HipPrec
     gc_default_params(&g);
     g.gravitational_constant = p.gravitational_constant;
     g.softening = p.softening;
-    g.solver = kind == HipForceSolver::Kind::fmm ? GC_SOLVER_FMM : GC_SOLVER_DIRECT;
+    g.solver = kind == HipForceSolver::Kind::fmm    ? GC_SOLVER_FMM
+               : kind == HipForceSolver::Kind::tree ? GC_SOLVER_TREE
+                                                    : GC_SOLVER_DIRECT;
     g.precision = prec == HipPrecision::mixed ? GC_PRECISION_MIXED : GC_PRECISION_FP64;
     g.theta = o.theta;
     g.leaf_capacity = static_cast<int>(std::min<std::size_t>(o.leaf_capacity, 1u << 30));
@@ -71,6 +73,10 @@ bool uses_hip_fmm_solver(const std::string& solver) {
     return solver == "hip-fmm" || solver == "rocm-fmm";
 }
 
+bool uses_hip_tree_solver(const std::string& solver) {
+    return solver == "hip-tree" || solver == "rocm-tree";
+}
+
 HipPrecision parse_hip_precision(const std::string& value) {
     std::string v(value);
     std::transform(v.begin(), v.end(), v.begin(),
@@ -116,6 +122,18 @@ void compute_hip_fmm_accelerations_for_targets(std::vector<Particle>& particles,
                                                std::size_t end, FmmOptions options,
                                                HipPrecision precision) {
     run_force(particles, params, HipForceSolver::Kind::fmm, precision, options, begin, end);
+}
+
+void compute_hip_tree_accelerations(std::vector<Particle>& particles, const PhysicsParams& params,
+                                    double theta, std::size_t leaf_capacity, int expansion_order,
+                                    HipPrecision precision) {
+    FmmOptions options;
+    options.theta = theta;
+    options.leaf_capacity = leaf_capacity;
+    options.max_depth = 32;
+    options.expansion_order = expansion_order;
+    run_force(particles, params, HipForceSolver::Kind::tree, precision, options, 0,
+              particles.size());
 }
 
 HipForceSolver::HipForceSolver(const PhysicsParams& params, Kind kind, HipPrecision precision,

@@ -1,7 +1,7 @@
 // hip_solver.hpp — C++ force-solver interface of the gfx950 engine, shaped
-// like the reference's solver free functions so that `hip-direct` and
-// `hip-fmm` slot into its `solver = ...` dispatch
-// (cpp/core/main.cpp:83-139) next to `direct` and `fmm`.
+// like the reference's solver free functions so that `hip-direct`,
+// `hip-fmm` and `hip-tree` slot into its `solver = ...` dispatch
+// (cpp/core/main.cpp:83-139) next to `direct`, `fmm` and `tree`.
 //
 // Same conventions as the reference (cpp/direct/direct_solver.hpp:22-28,
 // cpp/fmm/fmm_solver.hpp:71-83, cpp/cuda/cuda_solver.hpp:10-12):
@@ -81,6 +81,7 @@ enum class HipPrecision { fp64, mixed };
 // Solver-name predicates in the style of cpp/core/main.cpp:62-73.
 bool uses_hip_direct_solver(const std::string& solver);
 bool uses_hip_fmm_solver(const std::string& solver);
+bool uses_hip_tree_solver(const std::string& solver);
 // "fp64" | "double" -> fp64, "mixed" | "fp32" -> mixed; throws otherwise.
 HipPrecision parse_hip_precision(const std::string& value);
 
@@ -102,11 +103,17 @@ void compute_hip_fmm_accelerations_for_targets(std::vector<Particle>& particles,
                                                const PhysicsParams& params, std::size_t begin,
                                                std::size_t end, FmmOptions options = {},
                                                HipPrecision precision = HipPrecision::mixed);
+// compute_tree_accelerations (cpp/fmm/quadtree.hpp:56-62): Barnes-Hut with
+// the same defaults (theta 0.6, leaf 16, order 4; depth cap 32)
+void compute_hip_tree_accelerations(std::vector<Particle>& particles, const PhysicsParams& params,
+                                    double theta = 0.6, std::size_t leaf_capacity = 16,
+                                    int expansion_order = 4,
+                                    HipPrecision precision = HipPrecision::mixed);
 
 // Resident-state solver: upload once, step many times, download on demand.
 class HipForceSolver {
 public:
-    enum class Kind { direct, fmm };
+    enum class Kind { direct, fmm, tree };
     HipForceSolver(const PhysicsParams& params, Kind kind, HipPrecision precision,
                    FmmOptions options = {}, int device = 0);
     ~HipForceSolver();

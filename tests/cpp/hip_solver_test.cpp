@@ -52,7 +52,8 @@ int main() {
         }
         require(threw, "hip-direct without a device throws std::runtime_error");
         require(uses_hip_direct_solver("hip-direct") && uses_hip_fmm_solver("hip-fmm") &&
-                    !uses_hip_direct_solver("direct"),
+                    uses_hip_tree_solver("hip-tree") && !uses_hip_direct_solver("direct") &&
+                    !uses_hip_tree_solver("tree"),
                 "solver name predicates");
         bool bad = false;
         try {
@@ -127,6 +128,15 @@ int main() {
     require(same, "resident hip-direct fp64 KDK is bit-identical to the oracle");
     const Diagnostics d = solver.diagnostics();
     require(d.total_mass > 0.0 && std::isfinite(d.total_energy), "device diagnostics finite");
+
+    // hip-tree: the reference's tree solver (compute_tree_accelerations)
+    // through the same free-function shape, against the oracle's restatement
+    std::vector<double> tref(3 * 80);
+    orc_tree(80, pos.data(), mass.data(), 1.0, 0.03, 0.6, 4, 32, 4, 0, 80, tref.data(), nullptr);
+    auto tree = cube;
+    for (int i = 0; i < 80; ++i) tree[i].position = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    compute_hip_tree_accelerations(tree, soft, 0.6, 4, 4, HipPrecision::fp64);
+    require(rel_err_mean(tree, tref) < 1.0e-12, "hip-tree fp64 matches the tree solver");
 
     std::printf(failures ? "hip_solver_test FAILED\n" : "hip_solver_test passed\n");
     return failures ? 1 : 0;
