@@ -325,6 +325,93 @@ def workload_drift(args):
     return out
 
 
+def workload_output(args):
+    """Run output from resident state: the reference's run loop
+    (cpp/core/main.cpp:163-184) on the N = 65,536 two-disk collision, mixed
+    direct, dt 5e-4, 200 KDK steps with snapshot_every 10 (21 snapshots and
+    diagnostics rows in the reference's CSV formats), against the same 200
+    steps without output.  cpu_baseline: the reference's SnapshotWriter
+    compiled from its source (oracle/_ref) writing one snapshot of the same
+    state."""
+    import shutil
+    import tempfile
+
+    import gcsim
+    from gcsim.writer import SnapshotWriter, output_steps, run_with_output
+
+    n, eps, dt, steps, every = 65536, 0.02, 5e-4, 200, 10
+    pos, vel, mass = gcsim.generators.config1(n=n)
+    groups = (np.arange(n) >= n // 2).astype(np.uint32)
+    root = Path(tempfile.mkdtemp(prefix="gc_output_"))
+    out = {"workload": "output: two-disk N=65536 mixed direct, dt=5e-4, 200 KDK steps, "
+                       "snapshot_every=10 (21 snapshots + diagnostics rows)"}
+    try:
+        eng = gcsim.Engine("hip-direct", "mixed", softening=eps)
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations()
+        eng.step(dt, 20)  # warm-up
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.step(dt, steps)
+        eng.synchronize()
+        bare = time.perf_counter() - t0
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations()
+        eng.synchronize()
+        t0 = time.perf_counter()
+        with SnapshotWriter(root / "async") as w:
+            w.write_metadata(name="output_bench", solver="hip-direct", particle_count=n,
+                             steps=steps, dt=dt, snapshot_every=every, softening=eps)
+            run_with_output(eng, w, steps, dt, every, group_ids=groups)
+            t_issue = time.perf_counter() - t0
+        with_out = time.perf_counter() - t0
+        nout = len(output_steps(steps, every))
+        snap_bytes = (root / "async" / "snapshot_000010.csv").stat().st_size
+        # one diagnostics evaluation, for the split of the output cost
+        eng.synchronize()
+        t0 = time.perf_counter()
+        eng.diagnostics()
+        diag_s = time.perf_counter() - t0
+        # formatting alone: one snapshot from host arrays
+        p, v, a = eng.download()
+        t0 = time.perf_counter()
+        with SnapshotWriter(root / "host") as w:
+            w.write_snapshot(0, 0.0, p, v, a, mass, group_ids=groups)
+        fmt_s = time.perf_counter() - t0
+        eng.close()
+        out.update({
+            "ms_per_step_no_output": bare * 1e3 / steps,
+            "ms_per_step_with_output": with_out * 1e3 / steps,
+            "output_overhead_ms_per_snapshot": (with_out - bare) * 1e3 / nout,
+            "loop_issue_s": t_issue, "snapshots": nout, "snapshot_bytes": snap_bytes,
+            "diagnostics_ms": diag_s * 1e3,
+            "host_snapshot_write_ms": fmt_s * 1e3,
+            "host_snapshot_MBps": snap_bytes / fmt_s / 1e6,
+        })
+        O = _oracle()
+        if O.ref_available():
+            diag = np.zeros(13)
+            t0 = time.perf_counter()
+            O.ref_write_run(root / "ref", dict(name="output_bench", solver="direct", steps=steps,
+                                               dt=dt, snapshot_every=every,
+                                               gravitational_constant=1.0, softening=eps,
+                                               tree_theta=0.6, tree_leaf_capacity=16,
+                                               fmm_expansion_order=4),
+                            0, 0.0, p, v, a, mass, groups, diag, snapshots=3)
+            ref_s = (time.perf_counter() - t0) / 3
+            same = (root / "ref" / "snapshot_000000.csv").read_bytes() == \
+                (root / "host" / "snapshot_000000.csv").read_bytes()
+            out["cpu_baseline"] = {"kind": "reference", "cores": 1,
+                                   "sample": "SnapshotWriter::write_snapshot of the same "
+                                             "N=65536 state, 3 snapshots",
+                                   "ms_per_snapshot": ref_s * 1e3,
+                                   "MBps": snap_bytes / ref_s / 1e6,
+                                   "bytes_identical_to_ours": same}
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    return out
+
+
 def workload_copy(args):
     """Calibration for the HBM counters: calibration_copy_kernel (plain
     float4 copy) over 256 MiB, so FETCH_SIZE / WRITE_SIZE per dispatch can
@@ -355,7 +442,8 @@ def main() -> int:
     ap.add_argument("--pairs", default="auto", choices=["auto", "targets", "symmetric"],
                     help="mixed force form: per target, or each unordered pair once")
     ap.add_argument("--workload", default="direct",
-                    choices=["direct", "fmm", "tree", "batch", "adjoint", "drift", "copy"],
+                    choices=["direct", "fmm", "tree", "batch", "adjoint", "drift", "output",
+                             "copy"],
                     help="direct = the headline line; the others print their own records")
     args = ap.parse_args()
 
@@ -373,7 +461,8 @@ def main() -> int:
     if args.workload != "direct":
         fn = {"fmm": workload_fmm, "tree": workload_tree, "batch": workload_batch,
               "adjoint": workload_adjoint,
-              "drift": workload_drift, "copy": workload_copy}[args.workload]
+              "drift": workload_drift, "output": workload_output,
+              "copy": workload_copy}[args.workload]
         print(json.dumps(fn(args)))
         return 0
 

@@ -240,6 +240,50 @@ gc_status need_particles(gc_ctx* c) {
 
 }  // namespace
 
+namespace gc {
+
+gc_status set_error(const std::string& msg, gc_status code) { return fail(nullptr, code, msg); }
+
+// One stream-ordered copy of the resident state block (x y z vx vy vz ax ay
+// az m, each *cap doubles) into dst; ev, if given, is recorded after it.
+// dst == nullptr only reports n and cap.
+gc_status stage_state(gc_ctx* c, double* dst, size_t dst_doubles, size_t* n, size_t* cap,
+                      hipEvent_t ev) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    *n = c->s.n;
+    *cap = c->s.cap;
+    if (!dst) return GC_OK;
+    if (dst_doubles < 10 * c->s.cap) return fail(c, GC_ERR_INVALID_ARGUMENT, "stage_state: buffer too small");
+    if (c->s.cap > 0)
+        GC_HIP(c, hipMemcpyAsync(dst, c->s.base, sizeof(double) * 10 * c->s.cap,
+                                 hipMemcpyDeviceToHost, c->stream), "stage state");
+    if (ev) GC_HIP(c, hipEventRecord(ev, c->stream), "stage state event");
+    return GC_OK;
+}
+
+// The diagnostics kernels of the resident state, their 13 results copied
+// (gc_diagnostics field order) into dst on the context's stream, then ev.
+gc_status stage_diagnostics(gc_ctx* c, double* dst, hipEvent_t ev, size_t* n) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    *n = c->s.n;
+    if (c->s.n == 0) {
+        std::memset(dst, 0, sizeof(gc_diagnostics));
+    } else {
+        launch_diagnostics(c->s, c->params.gravitational_constant, c->params.softening,
+                           c->diag_scratch, c->diag_scratch_doubles, c->diag_out, c->stream);
+        st = launch_status(c, "launch diagnostics kernels");
+        if (st != GC_OK) return st;
+        GC_HIP(c, hipMemcpyAsync(dst, c->diag_out, sizeof(gc_diagnostics), hipMemcpyDeviceToHost,
+                                 c->stream), "stage diagnostics");
+    }
+    GC_HIP(c, hipEventRecord(ev, c->stream), "stage diagnostics event");
+    return GC_OK;
+}
+
+}  // namespace gc
+
 extern "C" {
 
 void gc_default_params(gc_params* p) {
@@ -276,6 +320,7 @@ const char* gc_status_string(gc_status s) {
     case GC_ERR_OUT_OF_MEMORY: return "out of device memory";
     case GC_ERR_UNSUPPORTED: return "unsupported";
     case GC_ERR_STATE: return "invalid state";
+    case GC_ERR_IO: return "output file error";
     }
     return "unknown status";
 }

@@ -7,15 +7,19 @@
 //   potential    : PE = -sum_{i<j} G m_i m_j / sqrt(|x_j - x_i|^2 + eps^2),
 //                  grid (target tile I, source chunk c): one target per
 //                  lane over the sources j > i of kPotChunk tiles from
-//                  tile I on, staged in LDS, Neumaier-compensated per lane,
-//                  one partial per block (zero for chunks left of I); a
-//                  row pass folds each target tile's chunks in order;
+//                  tile I on, staged in LDS, sum_j m_j / d_ij per lane
+//                  times G m_i, one partial per block (zero for chunks left
+//                  of I); a row pass folds each target tile's chunks in
+//                  order;
 //   final        : one block folds the row sums and the linear partials in
 //                  fixed order with compensation and writes the
 //                  gc_diagnostics record.
-// Deterministic (no atomics).  Per-pair arithmetic follows the reference
-// ((G*m_i)*m_j / d, d = sqrt(((dx*dx + dy*dy) + dz*dz) + eps^2)); only the
-// summation order differs.
+// Deterministic (no atomics).  The reference forms each pair as
+// (G*m_i)*m_j / sqrt(((dx*dx + dy*dy) + dz*dz) + eps^2); the potential here
+// takes 1/d from a Newton-refined reciprocal square root (within 2 ulp) and
+// applies G m_i once per target, so it agrees with the reference to
+// rounding, like the summation order.  The linear sums follow the
+// reference's per-particle arithmetic.
 #include "gc_internal.hpp"
 
 #include <algorithm>
@@ -86,11 +90,58 @@ struct alignas(32) Src64 {
 
 constexpr int kPotChunk = 4;  // source tiles per block
 
+// 1/sqrt(x): the hardware estimate (about 23 correct bits) refined by two
+// Newton steps y <- y + y (1/2 - e/2), e = (x y) y, within 2 ulp.  Forming
+// (x y) y keeps every intermediate near sqrt(x) or 1, so normal and
+// subnormal x both stay in range.  Returns 0 unless x is a positive finite
+// number (a coincident pair at eps = 0 is skipped, as in the reference; an
+// overflowed distance contributes 0, as m / inf does there).
+__device__ __forceinline__ double inv_dist(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double e = (x * y) * y;
+        y = __builtin_fma(y, __builtin_fma(-0.5, e, 0.5), y);
+    }
+    // class bits 7 and 8: positive subnormal, positive normal
+    return __builtin_amdgcn_class(x, (1 << 7) | (1 << 8)) ? y : 0.0;
+}
+
+// Sources of one LDS tile against this lane's target: sum of m_j / d_ij;
+// DIAG: the target's own tile, only j > i counts.
+template <bool DIAG>
+__device__ __forceinline__ void potential_tile(const Src64* tile, int cnt, int jl_min, double xi,
+                                               double yi, double zi, double eps2, double& s0,
+                                               double& s1) {
+    int jj = 0;
+    for (; jj + 2 <= cnt; jj += 2) {
+        const Src64 a = tile[jj], b = tile[jj + 1];
+        double dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
+        const double ra = inv_dist(__builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2))));
+        dx = b.x - xi;
+        dy = b.y - yi;
+        dz = b.z - zi;
+        const double rb = inv_dist(__builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2))));
+        s0 = __builtin_fma(DIAG && jj <= jl_min ? 0.0 : a.m, ra, s0);
+        s1 = __builtin_fma(DIAG && jj + 1 <= jl_min ? 0.0 : b.m, rb, s1);
+    }
+    if (jj < cnt) {
+        const Src64 a = tile[jj];
+        const double dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
+        const double ra = inv_dist(__builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2))));
+        s0 = __builtin_fma(DIAG && jj <= jl_min ? 0.0 : a.m, ra, s0);
+    }
+}
+
+// Grid (target tile I, source chunk c): lane i sums m_j / d_ij over the
+// sources j > i of chunk c's tiles (from tile I on) in two interleaved
+// FP64 sums (all terms positive: each sum's relative error is at most
+// 512 ulp, typically ~20); the block's partial is G m_i times that,
+// summed over its lanes.
 __global__ __launch_bounds__(256) void diag_potential_kernel(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ m, double G, double eps2,
     double* __restrict__ partial) {
-#pragma clang fp contract(off)
     __shared__ Src64 tile[256];
     __shared__ double red[4];
     const int t = threadIdx.x;
@@ -104,8 +155,7 @@ __global__ __launch_bounds__(256) void diag_potential_kernel(
     }
     const bool active = i < n;
     const double xi = active ? x[i] : 0.0, yi = active ? y[i] : 0.0, zi = active ? z[i] : 0.0;
-    const double gmi = active ? G * m[i] : 0.0;
-    double s = 0.0, cc = 0.0;
+    double s0 = 0.0, s1 = 0.0;
     for (int jt = jt0; jt < jt1; ++jt) {
         const int j0 = jt * 256;
         __syncthreads();
@@ -119,16 +169,12 @@ __global__ __launch_bounds__(256) void diag_potential_kernel(
         }
         __syncthreads();
         const int cnt = min(256, n - j0);
-        for (int jj = 0; jj < cnt; ++jj) {
-            const Src64 q = tile[jj];
-            if (j0 + jj <= i) continue;
-            const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
-            const double d = __builtin_sqrt((dx * dx + dy * dy + dz * dz) + eps2);
-            if (d == 0.0) continue;
-            neumaier(s, cc, gmi * q.m / d);
-        }
+        if (jt == I)
+            potential_tile<true>(tile, cnt, t, xi, yi, zi, eps2, s0, s1);
+        else
+            potential_tile<false>(tile, cnt, -1, xi, yi, zi, eps2, s0, s1);
     }
-    double v = active ? (s + cc) : 0.0;
+    double v = active ? (G * m[i]) * (s0 + s1) : 0.0;
     v = wave_sum(v);
     if ((t & 63) == 0) red[t >> 6] = v;
     __syncthreads();

@@ -108,7 +108,7 @@ void launch_vjp_mixed(const DeviceState& s, MixedLayout& L, float4* lpack, doubl
                       int jsplit, double G, double eps, const double* lam, int lstride,
                       double* xbar, double* mbar, int ostride, bool accumulate, hipStream_t st);
 int vjp_mixed_splits(size_t n);
-// diagnostics.hip: out[13] as gc_diagnostics order (M, KE, P3, C3, L3, PE)
+// diagnostics.hip: out[13] in gc_diagnostics field order (KE, PE, E, M, P3, C3, L3)
 void launch_diagnostics(const DeviceState& s, double G, double eps, double* scratch,
                         size_t scratch_doubles, double* out_dev, hipStream_t st);
 size_t diagnostics_scratch_doubles(size_t n);

@@ -3,6 +3,7 @@
 from ._lib import GcError, load  # noqa: F401
 from .engine import Engine, device_count, make_params, solver_available  # noqa: F401
 from . import generators  # noqa: F401
+from .writer import SnapshotWriter  # noqa: F401
 
-__all__ = ["Engine", "GcError", "device_count", "generators", "load", "make_params",
-           "solver_available"]
+__all__ = ["Engine", "GcError", "SnapshotWriter", "device_count", "generators", "load",
+           "make_params", "solver_available"]

@@ -19,6 +19,7 @@ GC_ERR_HIP = 3
 GC_ERR_OUT_OF_MEMORY = 4
 GC_ERR_UNSUPPORTED = 5
 GC_ERR_STATE = 6
+GC_ERR_IO = 7
 
 GC_SOLVER_DIRECT, GC_SOLVER_FMM, GC_SOLVER_TREE = 0, 1, 2
 GC_PRECISION_FP64, GC_PRECISION_MIXED = 0, 1
@@ -69,6 +70,16 @@ class Collision(C.Structure):
                 ("impact_parameter", C.c_double), ("speed", C.c_double),
                 ("seed", C.c_uint64)]
 
+
+class RunInfo(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("solver", C.c_char_p), ("particle_count", C.c_size_t),
+                ("steps", C.c_int), ("dt", C.c_double), ("snapshot_every", C.c_int),
+                ("dim", C.c_int), ("gravitational_constant", C.c_double),
+                ("softening", C.c_double), ("tree_theta", C.c_double),
+                ("tree_leaf_capacity", C.c_size_t), ("fmm_expansion_order", C.c_int)]
+
+
+_u32p = C.POINTER(C.c_uint32)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -124,6 +135,17 @@ SIGNATURES = {
     "gc_get_pairs": (C.c_int, [C.c_void_p]),
     "gc_batch_set_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gc_batch_get_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gc_writer_open": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p]),
+    "gc_writer_metadata": (C.c_int, [C.c_void_p, C.POINTER(RunInfo)]),
+    "gc_writer_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, _u32p]),
+    "gc_writer_snapshot_host": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_size_t, _dp, _dp,
+                                          _dp, _dp, _u32p]),
+    "gc_writer_diagnostics": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double]),
+    "gc_writer_diagnostics_values": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_size_t,
+                                               C.POINTER(Diagnostics)]),
+    "gc_writer_set_threads": (C.c_int, [C.c_void_p, C.c_int]),
+    "gc_writer_flush": (C.c_int, [C.c_void_p]),
+    "gc_writer_close": (C.c_int, [C.c_void_p]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),
     "gc_generate_disks": (C.c_size_t, [C.POINTER(DiskGalaxy), C.c_int, C.c_double, C.c_double,
                                        C.c_uint64, _dp, _dp, _dp]),

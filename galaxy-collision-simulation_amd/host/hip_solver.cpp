@@ -201,4 +201,64 @@ Diagnostics HipForceSolver::diagnostics() {
 
 std::size_t HipForceSolver::size() const { return gc_particle_count(ctx_); }
 
+HipSnapshotWriter::HipSnapshotWriter(const std::string& directory) {
+    check(gc_writer_open(&w_, directory.c_str()), nullptr, "HIP snapshot writer");
+}
+
+HipSnapshotWriter::~HipSnapshotWriter() { (void)gc_writer_close(w_); }
+
+void HipSnapshotWriter::write_metadata(const gc_run_info& info) {
+    check(gc_writer_metadata(w_, &info), nullptr, "HIP snapshot writer");
+}
+
+void HipSnapshotWriter::write_snapshot(int step, double time,
+                                       const std::vector<Particle>& particles) {
+    std::vector<double> pos, vel, mass;
+    pack(particles, pos, vel, mass);
+    const std::size_t n = particles.size();
+    std::vector<double> acc(3 * n);
+    std::vector<std::uint32_t> groups(n);
+    for (std::size_t i = 0; i < n; ++i) {
+        acc[3 * i] = particles[i].acceleration.x;
+        acc[3 * i + 1] = particles[i].acceleration.y;
+        acc[3 * i + 2] = particles[i].acceleration.z;
+        groups[i] = particles[i].group_id;
+    }
+    check(gc_writer_snapshot_host(w_, step, time, n, pos.data(), vel.data(), acc.data(),
+                                  mass.data(), groups.data()),
+          nullptr, "HIP snapshot writer");
+}
+
+void HipSnapshotWriter::write_snapshot(int step, double time, HipForceSolver& solver,
+                                       const std::vector<std::uint32_t>* group_ids) {
+    if (group_ids && group_ids->size() != solver.size())
+        throw std::runtime_error("HIP snapshot writer: group_ids size differs from particle count");
+    check(gc_writer_snapshot(w_, solver.handle(), step, time, group_ids ? group_ids->data() : nullptr),
+          nullptr, "HIP snapshot writer");
+}
+
+void HipSnapshotWriter::write_diagnostics(int step, double time, const Diagnostics& d,
+                                          std::size_t particle_count) {
+    gc_diagnostics g;
+    g.kinetic_energy = d.kinetic_energy;
+    g.potential_energy = d.potential_energy;
+    g.total_energy = d.total_energy;
+    g.total_mass = d.total_mass;
+    const Vec2* v[3] = {&d.momentum, &d.center_of_mass, &d.angular_momentum};
+    double* o[3] = {g.momentum, g.center_of_mass, g.angular_momentum};
+    for (int k = 0; k < 3; ++k) {
+        o[k][0] = v[k]->x;
+        o[k][1] = v[k]->y;
+        o[k][2] = v[k]->z;
+    }
+    check(gc_writer_diagnostics_values(w_, step, time, particle_count, &g), nullptr,
+          "HIP snapshot writer");
+}
+
+void HipSnapshotWriter::write_diagnostics(int step, double time, HipForceSolver& solver) {
+    check(gc_writer_diagnostics(w_, solver.handle(), step, time), nullptr, "HIP snapshot writer");
+}
+
+void HipSnapshotWriter::flush() { check(gc_writer_flush(w_), nullptr, "HIP snapshot writer"); }
+
 }  // namespace fmmgalaxy

@@ -134,4 +134,50 @@ private:
     gc_ctx* ctx_{nullptr};
 };
 
+// SnapshotWriter (cpp/io/snapshot_writer.hpp:13-27) over gc_writer: the
+// same files, byte for byte, plus snapshots and diagnostics taken from a
+// HipForceSolver's resident state (the snapshot is staged by one device
+// copy and written by the writer's own thread).  Errors throw
+// std::runtime_error; file errors of queued snapshots surface at flush()
+// or destruction (the destructor swallows them: call flush() to see them).
+class HipSnapshotWriter {
+public:
+    explicit HipSnapshotWriter(const std::string& directory);
+    ~HipSnapshotWriter();
+    HipSnapshotWriter(const HipSnapshotWriter&) = delete;
+    HipSnapshotWriter& operator=(const HipSnapshotWriter&) = delete;
+
+    void write_metadata(const gc_run_info& info);
+    // SnapshotWriter::write_metadata(const SimulationConfig&, std::size_t):
+    // any config type with the reference's SimulationConfig fields
+    template <class Config>
+    void write_metadata(const Config& config, std::size_t particle_count) {
+        gc_run_info info{};
+        info.name = config.name.c_str();
+        info.solver = config.solver.c_str();
+        info.particle_count = particle_count;
+        info.steps = config.steps;
+        info.dt = config.dt;
+        info.snapshot_every = config.snapshot_every;
+        info.dim = config.dim;
+        info.gravitational_constant = config.physics.gravitational_constant;
+        info.softening = config.physics.softening;
+        info.tree_theta = config.tree_theta;
+        info.tree_leaf_capacity = config.tree_leaf_capacity;
+        info.fmm_expansion_order = config.fmm_expansion_order;
+        write_metadata(info);
+    }
+    void write_snapshot(int step, double time, const std::vector<Particle>& particles);
+    // group_ids: solver.size() entries, or null for all 0
+    void write_snapshot(int step, double time, HipForceSolver& solver,
+                        const std::vector<std::uint32_t>* group_ids = nullptr);
+    void write_diagnostics(int step, double time, const Diagnostics& diagnostics,
+                           std::size_t particle_count);
+    void write_diagnostics(int step, double time, HipForceSolver& solver);
+    void flush();
+
+private:
+    gc_writer* w_{nullptr};
+};
+
 }  // namespace fmmgalaxy

@@ -39,7 +39,8 @@ typedef enum {
     GC_ERR_HIP = 3,
     GC_ERR_OUT_OF_MEMORY = 4,
     GC_ERR_UNSUPPORTED = 5,
-    GC_ERR_STATE = 6
+    GC_ERR_STATE = 6,
+    GC_ERR_IO = 7
 } gc_status;
 
 /* solver choice: the `solver = "hip-direct" | "hip-fmm" | "hip-tree"` config
@@ -316,6 +317,67 @@ size_t gc_generate_plummer(size_t n, double total_mass, double scale, double G, 
 /* Batch sampler: N log-uniform in [n_min, n_max] (even), inclination
  * U[0, pi/2], impact parameter U[0.5, 2], speed U[0.2, 0.5]. */
 int gc_sample_collisions(int runs, uint64_t seed, int n_min, int n_max, gc_collision* out);
+
+/* ---- run output (SnapshotWriter, cpp/io/snapshot_writer.hpp) ----
+ * The reference's output directory, byte for byte: metadata.json,
+ * diagnostics.csv (one row per call) and snapshot_NNNNNN.csv (id, group_id,
+ * mass, x y z, vx vy vz, ax ay az at 17 significant digits), which its
+ * Python tools (python/utils/snapshots.py) read.  gc_writer_snapshot
+ * stages the resident state with one stream-ordered copy into a pinned
+ * buffer and returns; the writer's own thread formats and writes the file
+ * while the device steps on (two snapshots and 16 diagnostics rows in
+ * flight at most).  A writer is
+ * driven from one thread at a time, and its device snapshots come from
+ * contexts on one device.  File errors of queued snapshots are reported
+ * by the next gc_writer_flush or gc_writer_close (GC_ERR_IO). */
+typedef struct gc_writer gc_writer;
+
+/* SimulationConfig fields metadata.json records (cpp/core/config.hpp:19-34) */
+typedef struct {
+    const char* name;
+    const char* solver;
+    size_t particle_count;
+    int steps;
+    double dt;
+    int snapshot_every;
+    int dim;
+    double gravitational_constant;
+    double softening;
+    double tree_theta;
+    size_t tree_leaf_capacity;
+    int fmm_expansion_order;
+} gc_run_info;
+
+/* SnapshotWriter::SnapshotWriter (snapshot_writer.cpp:31-48): creates the
+ * directory and diagnostics.csv with its header line. */
+gc_status gc_writer_open(gc_writer** out, const char* directory);
+/* SnapshotWriter::write_metadata (snapshot_writer.cpp:50-70) */
+gc_status gc_writer_metadata(gc_writer* w, const gc_run_info* info);
+/* SnapshotWriter::write_snapshot (snapshot_writer.cpp:72-96) of the
+ * context's resident state, asynchronously; group_ids (n entries, NULL =
+ * all 0) are copied before the call returns. */
+gc_status gc_writer_snapshot(gc_writer* w, gc_ctx* ctx, int step, double time,
+                             const uint32_t* group_ids);
+/* The same from host arrays (pos, vel, acc: n x 3; mass: n), copied before
+ * the call returns. */
+gc_status gc_writer_snapshot_host(gc_writer* w, int step, double time, size_t n,
+                                  const double* pos, const double* vel, const double* acc,
+                                  const double* mass, const uint32_t* group_ids);
+/* SnapshotWriter::write_diagnostics (snapshot_writer.cpp:98-121) of the
+ * context's resident state: the gc_diagnostics_compute kernels and a copy
+ * of their results queued on the context's stream, the row written by the
+ * writer's thread (rows keep call order). */
+gc_status gc_writer_diagnostics(gc_writer* w, gc_ctx* ctx, int step, double time);
+/* The same row from values the caller holds. */
+gc_status gc_writer_diagnostics_values(gc_writer* w, int step, double time, size_t n,
+                                       const gc_diagnostics* d);
+/* Host threads that format one snapshot's rows (0 = min(8, hardware
+ * threads), the default). */
+gc_status gc_writer_set_threads(gc_writer* w, int threads);
+/* Waits until every queued snapshot is on disk and flushes diagnostics.csv. */
+gc_status gc_writer_flush(gc_writer* w);
+/* Flushes, closes the files and frees the writer (NULL is a no-op). */
+gc_status gc_writer_close(gc_writer* w);
 
 #ifdef __cplusplus
 }

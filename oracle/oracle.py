@@ -333,5 +333,26 @@ def ref_generate(galaxies, G, eps, seed):
     return pos[:got], vel[:got], mass[:got], grp[:got]
 
 
+def ref_write_run(directory, meta, step, time, pos, vel, acc, mass, groups, diag, snapshots=1):
+    """The reference's SnapshotWriter (cpp/io/snapshot_writer.cpp) on
+    `directory`: metadata.json from `meta` (metadata.json keys), `snapshots`
+    copies of one snapshot (steps step, step + 1, ...) and one
+    diagnostics row (`diag`: 13 values in diagnostics.csv column order)."""
+    n = len(mass)
+    g = np.ascontiguousarray(groups if groups is not None else np.zeros(n), dtype=np.uint32)
+    d = _c64(diag)
+    assert d.shape == (13,)
+    rc = ref_lib().ref_write_run(
+        os.fsencode(str(directory)), meta["name"].encode(), meta["solver"].encode(),
+        C.c_int(meta["steps"]), C.c_double(meta["dt"]), C.c_int(meta["snapshot_every"]),
+        C.c_double(meta["gravitational_constant"]), C.c_double(meta["softening"]),
+        C.c_double(meta["tree_theta"]), _size_t(meta["tree_leaf_capacity"]),
+        C.c_int(meta["fmm_expansion_order"]), C.c_int(step), C.c_double(time), _size_t(n),
+        _p(_c64(pos)), _p(_c64(vel)), _p(_c64(acc)), _p(_c64(mass)),
+        g.ctypes.data_as(C.POINTER(C.c_uint)), _p(d), C.c_int(snapshots))
+    if rc:
+        raise RuntimeError("reference SnapshotWriter threw")
+
+
 __all__ = [n for n in dir() if not n.startswith("_")] + ["HERE"]
 os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")

@@ -12,6 +12,7 @@
 #include "fmm/fmm_solver.hpp"
 #include "fmm/multipole.hpp"
 #include "fmm/quadtree.hpp"
+#include "io/snapshot_writer.hpp"
 
 #include <cstddef>
 #include <vector>
@@ -215,6 +216,50 @@ std::size_t ref_generate(const double* gal, int ngal, double G, double eps, unsi
         if (group) group[i] = ps[i].group_id;
     }
     return ps.size();
+}
+
+// SnapshotWriter (cpp/io/snapshot_writer.cpp): a writer on dir, then
+// write_metadata, write_snapshot(step, time, particles) and one
+// write_diagnostics row (diag: total_mass, kinetic, potential, total energy,
+// momentum xyz, centre of mass xyz, angular momentum xyz).  snapshots > 1
+// repeats the snapshot as steps step, step + 1, ... (timing).  Returns 0, or
+// 1 when the writer throws.
+int ref_write_run(const char* dir, const char* name, const char* solver, int steps, double dt,
+                  int snapshot_every, double G, double eps, double theta, std::size_t leaf,
+                  int order, int step, double time, std::size_t n, const double* pos,
+                  const double* vel, const double* acc, const double* mass,
+                  const unsigned* groups, const double* diag, int snapshots) {
+    try {
+        SimulationConfig cfg;
+        cfg.name = name;
+        cfg.solver = solver;
+        cfg.steps = steps;
+        cfg.dt = dt;
+        cfg.snapshot_every = snapshot_every;
+        cfg.physics = physics(G, eps);
+        cfg.tree_theta = theta;
+        cfg.tree_leaf_capacity = leaf;
+        cfg.fmm_expansion_order = order;
+        cfg.output.directory = dir;
+        cfg.output.format = "csv";
+        std::vector<Particle> ps = to_particles(n, pos, vel, acc, mass);
+        for (std::size_t i = 0; i < n; ++i) ps[i].group_id = groups ? groups[i] : 0u;
+        SnapshotWriter w(cfg);
+        w.write_metadata(cfg, n);
+        for (int k = 0; k < snapshots; ++k) w.write_snapshot(step + k, time, ps);
+        Diagnostics d;
+        d.total_mass = diag[0];
+        d.kinetic_energy = diag[1];
+        d.potential_energy = diag[2];
+        d.total_energy = diag[3];
+        d.momentum = {diag[4], diag[5], diag[6]};
+        d.center_of_mass = {diag[7], diag[8], diag[9]};
+        d.angular_momentum = {diag[10], diag[11], diag[12]};
+        w.write_diagnostics(step, time, d, n);
+    } catch (...) {
+        return 1;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -44,7 +44,7 @@ def test_defaults_match_reference():
 
 def test_status_strings():
     lib = gcsim.load()
-    for code in range(7):
+    for code in range(8):
         assert lib.gc_status_string(code)
     assert b"unknown" in lib.gc_status_string(99)
 

@@ -316,6 +316,33 @@ def test_diagnostics_large_n(gpu):
         assert abs(d[k] - w) <= 1e-12 * max(abs(w), scale[k]), k
 
 
+@pytest.mark.parametrize("case", ["coincident_eps0", "subnormal_r2", "overflow_r2", "ragged"])
+def test_diagnostics_edge_cases(gpu, case):
+    # the potential's 1/d (refined hardware reciprocal square root) at the
+    # ends of the FP64 range, against the oracle's sqrt-and-divide
+    rng = np.random.default_rng(3)
+    n, eps = 300, 0.0
+    pos, vel, m = rng.normal(size=(n, 3)), rng.normal(size=(n, 3)) * 0.1, rng.uniform(0.5, 1, n)
+    if case == "coincident_eps0":
+        pos[7] = pos[200]  # d = 0: skipped
+        pos[8] = pos[9]
+    elif case == "subnormal_r2":
+        pos[200] = 0.0
+        pos[7] = [1e-160, 0.0, 0.0]  # r^2 = 1e-320, subnormal; d = 1e-160
+    elif case == "overflow_r2":
+        pos[7] = [1e200, 0.0, 0.0]  # r^2 = inf: those pairs contribute 0
+    else:
+        n, eps = 257, 0.01
+        pos, vel, m = pos[:n], vel[:n], m[:n]
+    with _engine(gpu, "fp64", eps) as eng:
+        eng.upload(pos, vel, m)
+        d = eng.diagnostics()
+    want = O.diagnostics(pos, vel, m, 1.0, eps)
+    assert np.isfinite(want["potential_energy"]) and np.isfinite(d["potential_energy"])
+    for k in ("potential_energy", "kinetic_energy", "total_energy"):
+        assert abs(d[k] - want[k]) <= 1e-12 * abs(want[k]), (k, d[k], want[k])
+
+
 def test_empty_and_reupload(gpu):
     with _engine(gpu, "mixed", 0.02) as eng:
         eng.upload(np.zeros((0, 3)), None, np.zeros(0))
