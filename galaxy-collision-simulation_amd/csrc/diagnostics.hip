@@ -194,23 +194,31 @@ __global__ __launch_bounds__(256) void diag_rows_kernel(int tiles, int chunks,
 }
 
 // Writes the gc_diagnostics field order: KE, PE, E, M, P[3], COM[3], L[3].
-__global__ __launch_bounds__(64) void diag_final_kernel(int nlin, const double* __restrict__ lin,
-                                                         int npot, const double* __restrict__ pot,
-                                                         double* __restrict__ out) {
+// Wave k folds quantity k (k < kLin: the linear partials; k == kLin: the
+// potential rows): lane l takes entries l, l + 64, ... in order, then the
+// lanes' compensated pairs meet in a fixed butterfly (each step symmetric
+// in its two operands, so every lane ends with the same, fixed result).
+__global__ __launch_bounds__(64 * (kLin + 1)) void diag_final_kernel(
+    int nlin, const double* __restrict__ lin, int npot, const double* __restrict__ pot,
+    double* __restrict__ out) {
 #pragma clang fp contract(off)
     __shared__ double tot[kLin + 1];
-    const int k = threadIdx.x;
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double s = 0.0, c = 0.0;
     if (k < kLin) {
-        double s = 0.0, c = 0.0;
-        for (int b = 0; b < nlin; ++b) neumaier(s, c, lin[b * kLin + k]);
-        tot[k] = s + c;
-    } else if (k == kLin) {
-        double s = 0.0, c = 0.0;
-        for (int b = 0; b < npot; ++b) neumaier(s, c, pot[b]);
-        tot[kLin] = -(s + c);
+        for (int b = lane; b < nlin; b += 64) neumaier(s, c, lin[b * kLin + k]);
+    } else {
+        for (int b = lane; b < npot; b += 64) neumaier(s, c, pot[b]);
     }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double s2 = __shfl_xor(s, o, 64), c2 = __shfl_xor(c, o, 64);
+        c += c2;
+        neumaier(s, c, s2);
+    }
+    if (lane == 0) tot[k] = k < kLin ? s + c : -(s + c);
     __syncthreads();
-    if (k == 0) {
+    if (threadIdx.x == 0) {
         const double M = tot[0];
         out[0] = tot[1];
         out[1] = tot[kLin];
@@ -255,7 +263,8 @@ void launch_diagnostics(const DeviceState& s, double G, double eps, double* scra
     hipLaunchKernelGGL(diag_potential_kernel, dim3(T, C), dim3(256), 0, st, n, s.x, s.y, s.z, s.m,
                        G, eps * eps, pot);
     hipLaunchKernelGGL(diag_rows_kernel, dim3((T + 255) / 256), dim3(256), 0, st, T, C, pot, rows);
-    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(64), 0, st, nl, lin, T, rows, out_dev);
+    hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(64 * (kLin + 1)), 0, st, nl, lin, T, rows,
+                       out_dev);
 }
 
 }  // namespace gc
