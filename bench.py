@@ -412,6 +412,71 @@ def workload_output(args):
     return out
 
 
+def workload_ic(args):
+    """Initial conditions written in HBM by the device generator (one
+    Philox4x32-10 stream per galaxy, one thread per particle) against host
+    generation plus upload: the reference's disk model (two galaxies of
+    the README collision) and the Plummer sphere at N = 2^20, and the
+    two-disk N = 65,536 collision.  cpu_baseline: the reference's
+    generate_galaxies compiled from its source (oracle/_ref) on the same
+    two galaxies."""
+    import gcsim
+
+    reps, n = 5, 1 << 20
+    gal = [dict(n_particles=n // 2, mass=1.0, radius=0.85, position=[-0.72, -0.10, 0.06],
+                velocity=[0.34, 0.10, -0.015], orientation=0.25, group_id=0, thickness=0.045,
+                inclination=0.62),
+           dict(n_particles=n // 2, mass=1.0, radius=0.85, position=[0.72, 0.10, -0.06],
+                velocity=[-0.34, -0.10, 0.015], orientation=3.42, group_id=1, thickness=0.045,
+                inclination=-0.72)]
+    out = {"workload": "ic: reference disks and Plummer N=2^20, two-disk N=65536; "
+                       "device generation vs host generation + upload"}
+    eng = gcsim.Engine("hip-direct", "mixed", softening=0.025)
+
+    def timed(fn):
+        fn()
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        eng.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    cases = {
+        "reference_disks_2^20": (
+            lambda: eng.generate_reference_galaxies(gal, 20260502),
+            lambda: gcsim.generators.reference_galaxies(gal, 1.0, 0.025, 20260502)[:3], n),
+        "plummer_2^20": (lambda: eng.generate_plummer(n, seed=1),
+                         lambda: gcsim.generators.plummer(n, seed=1), n),
+        "two_disk_65536": (lambda: eng.generate_two_disk(32768, seed=20260502),
+                           lambda: gcsim.generators.config1(), 65536),
+    }
+    for name, (dev, host, count) in cases.items():
+        dev_s = timed(dev)
+        t0 = time.perf_counter()
+        p, v, m = host()
+        host_gen = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        eng.upload(p, v, m)
+        eng.synchronize()
+        up = time.perf_counter() - t0
+        out[name] = {"device_ms": dev_s * 1e3, "host_generate_ms": host_gen * 1e3,
+                     "upload_ms": up * 1e3, "particles_per_s_device": count / dev_s,
+                     "state_GBps_device": 80.0 * count / dev_s / 1e9,
+                     "speedup_vs_host_plus_upload": (host_gen + up) / dev_s}
+        print(f"ic {name} done", file=sys.stderr, flush=True)
+    eng.close()
+    O = _oracle()
+    if O.ref_available():
+        t0 = time.perf_counter()
+        O.ref_generate(gal, 1.0, 0.025, 20260502)
+        ref_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"kind": "reference", "cores": 1,
+                               "sample": "generate_galaxies, the same two galaxies (N=2^20)",
+                               "ms": ref_s * 1e3, "particles_per_s": n / ref_s}
+    return out
+
+
 def workload_copy(args):
     """Calibration for the HBM counters: calibration_copy_kernel (plain
     float4 copy) over 256 MiB, so FETCH_SIZE / WRITE_SIZE per dispatch can
@@ -443,7 +508,7 @@ def main() -> int:
                     help="mixed force form: per target, or each unordered pair once")
     ap.add_argument("--workload", default="direct",
                     choices=["direct", "fmm", "tree", "batch", "adjoint", "drift", "output",
-                             "copy"],
+                             "ic", "copy"],
                     help="direct = the headline line; the others print their own records")
     args = ap.parse_args()
 
@@ -461,7 +526,7 @@ def main() -> int:
     if args.workload != "direct":
         fn = {"fmm": workload_fmm, "tree": workload_tree, "batch": workload_batch,
               "adjoint": workload_adjoint,
-              "drift": workload_drift, "output": workload_output,
+              "drift": workload_drift, "output": workload_output, "ic": workload_ic,
               "copy": workload_copy}[args.workload]
         print(json.dumps(fn(args)))
         return 0

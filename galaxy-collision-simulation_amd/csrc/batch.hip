@@ -18,6 +18,7 @@
 // once into a hipGraph and replayed.  The (chunk, blocks) schedule is chosen
 // by timing candidates on the batch itself (gc_batch_autotune).
 #include "gc_internal.hpp"
+#include "ic_models.hpp"
 #include "mixed_tiles.hpp"
 
 #include <algorithm>
@@ -510,6 +511,53 @@ gc_status gc_batch_upload(gc_batch* b, const double* pos, const double* vel, con
         for (size_t k = 0; k < b->counts[r]; ++k) m[b->offsets[r] + k] = mass[src++];
     GB_HIP(b, hipMemcpy(b->s.m, m.data(), sizeof(double) * b->total, hipMemcpyHostToDevice), "upload masses");
     GB_HIP(b, hipMemset(b->s.ax, 0, sizeof(double) * 3 * b->total), "clear accelerations");
+    return GC_OK;
+}
+
+gc_status gc_batch_generate_disks(gc_batch* b, const gc_disk_galaxy* galaxies,
+                                  int galaxies_per_run, const uint64_t* seeds) {
+    if (!b || !galaxies || !seeds || galaxies_per_run <= 0)
+        return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_generate_disks: bad arguments");
+    std::vector<IcSegment> segs;
+    long long begin = 0;
+    for (int r = 0; r < b->runs; ++r) {
+        size_t in_run = 0;
+        for (int k = 0; k < galaxies_per_run; ++k) {
+            const gc_disk_galaxy& g = galaxies[static_cast<size_t>(r) * galaxies_per_run + k];
+            if (g.n_particles == 0) continue;
+            IcSegment sg;
+            sg.model = kIcTwoDisk;
+            sg.stream = static_cast<unsigned>(k);
+            sg.seed = seeds[r];
+            sg.begin = begin;
+            sg.count = static_cast<long long>(g.n_particles);
+            sg.dst = static_cast<long long>(b->offsets[r] + in_run);
+            ic::two_disk_params(g, b->params.gravitational_constant, b->params.softening, sg.p);
+            segs.push_back(sg);
+            begin += sg.count;
+            in_run += g.n_particles;
+        }
+        if (in_run != b->counts[r])
+            return bfail(b, GC_ERR_INVALID_ARGUMENT,
+                         "batch_generate_disks: run " + std::to_string(r) + " has " +
+                             std::to_string(b->counts[r]) + " particles, its galaxies " +
+                             std::to_string(in_run));
+    }
+    GB_HIP(b, hipSetDevice(b->device), "select device");
+    // padding slots stay massless and at rest
+    GB_HIP(b, hipMemsetAsync(b->s.base, 0, sizeof(double) * 10 * b->total, b->stream), "clear state");
+    IcSegment* d = nullptr;
+    GB_HIP(b, hipMalloc(&d, sizeof(IcSegment) * segs.size()), "batch_generate_disks");
+    hipError_t e = hipMemcpyAsync(d, segs.data(), sizeof(IcSegment) * segs.size(),
+                                  hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) {
+        launch_ic(d, static_cast<int>(segs.size()), begin, b->s, b->stream);
+        e = hipGetLastError();
+    }
+    const hipError_t e2 = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e == hipSuccess) e = e2;
+    GB_HIP(b, e, "batch_generate_disks");
     return GC_OK;
 }
 

@@ -4,12 +4,14 @@
 // HIP error becomes GC_ERR_HIP with a message from gc_last_error.
 #include "gc_internal.hpp"
 #include "gc_fmm.hpp"
+#include "ic_models.hpp"
 
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 using namespace gc;
 
@@ -235,6 +237,34 @@ gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double hal
 gc_status need_particles(gc_ctx* c) {
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     GC_HIP(c, hipSetDevice(c->device), "select device");
+    return GC_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Replaces the resident state with n particles drawn on the device from
+// segs (begin/count cover [0, n), dst = begin).
+gc_status generate_state(gc_ctx* c, std::vector<IcSegment>& segs, size_t n, const char* what) {
+    gc_status st = need_particles(c);
+    if (st != GC_OK) return st;
+    if (n > static_cast<size_t>(1) << 30)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, std::string(what) + ": particle count exceeds 2^30");
+    st = ensure_capacity(c, n);
+    if (st != GC_OK || n == 0) return st;
+    IcSegment* d = nullptr;
+    GC_HIP(c, hipMalloc(&d, sizeof(IcSegment) * segs.size()), what);
+    hipError_t e = hipMemcpyAsync(d, segs.data(), sizeof(IcSegment) * segs.size(),
+                                  hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        launch_ic(d, static_cast<int>(segs.size()), static_cast<long long>(n), c->s, c->stream);
+        e = hipGetLastError();
+    }
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail(c, GC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
     return GC_OK;
 }
 
@@ -500,6 +530,69 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
         if (st != GC_OK) return st;
     }
     return launch_status(c, "launch step kernels");
+}
+
+gc_status gc_generate_galaxies_device(gc_ctx* c, const gc_galaxy_config* galaxies, int count,
+                                      uint64_t seed) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (count < 0 || (count > 0 && !galaxies))
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_galaxies_device: bad galaxy list");
+    std::vector<IcSegment> segs;
+    size_t n = 0;
+    for (int k = 0; k < count; ++k) {
+        if (ic::ref_disk_empty(galaxies[k])) continue;
+        IcSegment sg;
+        sg.model = kIcRefDisk;
+        sg.stream = static_cast<unsigned>(k);
+        sg.seed = seed;
+        sg.begin = sg.dst = static_cast<long long>(n);
+        sg.count = static_cast<long long>(galaxies[k].n_particles);
+        ic::ref_disk_params(galaxies[k], c->params.gravitational_constant, c->params.softening, sg.p);
+        segs.push_back(sg);
+        n += galaxies[k].n_particles;
+    }
+    return generate_state(c, segs, n, "generate_galaxies_device");
+}
+
+gc_status gc_generate_disks_device(gc_ctx* c, const gc_disk_galaxy* galaxies, int count,
+                                   uint64_t seed) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (count < 0 || (count > 0 && !galaxies))
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_disks_device: bad galaxy list");
+    std::vector<IcSegment> segs;
+    size_t n = 0;
+    for (int k = 0; k < count; ++k) {
+        if (galaxies[k].n_particles == 0) continue;
+        IcSegment sg;
+        sg.model = kIcTwoDisk;
+        sg.stream = static_cast<unsigned>(k);
+        sg.seed = seed;
+        sg.begin = sg.dst = static_cast<long long>(n);
+        sg.count = static_cast<long long>(galaxies[k].n_particles);
+        ic::two_disk_params(galaxies[k], c->params.gravitational_constant, c->params.softening, sg.p);
+        segs.push_back(sg);
+        n += galaxies[k].n_particles;
+    }
+    return generate_state(c, segs, n, "generate_disks_device");
+}
+
+gc_status gc_generate_plummer_device(gc_ctx* c, size_t n, double total_mass, double scale,
+                                     uint64_t seed) {
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (!(scale > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_plummer_device: scale must be positive");
+    std::vector<IcSegment> segs;
+    if (n > 0) {
+        IcSegment sg;
+        sg.model = kIcPlummer;
+        sg.seed = seed;
+        sg.count = static_cast<long long>(n);
+        sg.p[ic::kPlN] = static_cast<double>(n);
+        sg.p[ic::kPlMass] = total_mass;
+        sg.p[ic::kPlScale] = scale;
+        sg.p[ic::kPlG] = c->params.gravitational_constant;
+        segs.push_back(sg);
+    }
+    return generate_state(c, segs, n, "generate_plummer_device");
 }
 
 gc_status gc_synchronize(gc_ctx* c) {

@@ -63,6 +63,19 @@ struct LaunchShape {
     int pairs = 0;    // GC_PAIRS_*: 0 automatic, 1 per target, 2 each pair once
 };
 
+// One model over a run of particles for the device generator (ic.hip).
+enum IcModel { kIcRefDisk = 0, kIcTwoDisk = 1, kIcPlummer = 2 };
+constexpr int kIcParams = 20;
+constexpr unsigned kIcPlummerTries = 1000;  // rejection pairs per particle, at most
+struct IcSegment {
+    int model = kIcRefDisk;
+    unsigned stream = 0;             // Philox counter word 2
+    unsigned long long seed = 0;     // Philox key
+    long long begin = 0, count = 0;  // particles [begin, begin + count) of the launch
+    long long dst = 0;               // state slot of the segment's first particle
+    double p[kIcParams] = {};        // model parameters (ic_models.hpp layouts)
+};
+
 // ---- kernel launchers (defined in the .hip files) ------------------------
 // direct_fp64.hip
 void launch_direct_fp64(const DeviceState& s, double G, double eps, size_t begin, size_t end,
@@ -101,6 +114,9 @@ void launch_scatter_xyz(const double* interleaved, double* x, double* y, double*
 void launch_gather_xyz(const double* x, const double* y, const double* z, double* interleaved,
                        size_t n, hipStream_t st);
 hipError_t run_calibration_copy(int device, size_t bytes, int reps, double* ms);
+// ic.hip: segs in device memory, sorted by begin, covering [0, total)
+void launch_ic(const IcSegment* segs, int nseg, long long total, const DeviceState& s,
+               hipStream_t st);
 // adjoint.hip: lam / xbar are SoA with strides lstride / ostride, mbar n.
 void launch_vjp_fp64(const DeviceState& s, double G, double eps, const double* lam, int lstride,
                      double* xbar, double* mbar, int ostride, bool accumulate, hipStream_t st);

@@ -65,6 +65,13 @@ class DiskGalaxy(C.Structure):
                 ("position", C.c_double * 3), ("velocity", C.c_double * 3)]
 
 
+class GalaxyConfig(C.Structure):
+    _fields_ = [("n_particles", C.c_size_t), ("mass", C.c_double), ("radius", C.c_double),
+                ("position", C.c_double * 3), ("velocity", C.c_double * 3),
+                ("orientation", C.c_double), ("group_id", C.c_uint32),
+                ("thickness", C.c_double), ("inclination", C.c_double)]
+
+
 class Collision(C.Structure):
     _fields_ = [("n_particles", C.c_size_t), ("inclination", C.c_double),
                 ("impact_parameter", C.c_double), ("speed", C.c_double),
@@ -146,7 +153,18 @@ SIGNATURES = {
     "gc_writer_set_threads": (C.c_int, [C.c_void_p, C.c_int]),
     "gc_writer_flush": (C.c_int, [C.c_void_p]),
     "gc_writer_close": (C.c_int, [C.c_void_p]),
+    "gc_default_galaxy_config": (None, [C.POINTER(GalaxyConfig)]),
+    "gc_generate_galaxies": (C.c_size_t, [C.POINTER(GalaxyConfig), C.c_int, C.c_double, C.c_double,
+                                          C.c_uint64, _dp, _dp, _dp, C.POINTER(C.c_uint32)]),
     "gc_default_disk_galaxy": (None, [C.POINTER(DiskGalaxy)]),
+    "gc_generate_galaxies_device": (C.c_int, [C.c_void_p, C.POINTER(GalaxyConfig), C.c_int,
+                                              C.c_uint64]),
+    "gc_generate_disks_device": (C.c_int, [C.c_void_p, C.POINTER(DiskGalaxy), C.c_int,
+                                           C.c_uint64]),
+    "gc_generate_plummer_device": (C.c_int, [C.c_void_p, C.c_size_t, C.c_double, C.c_double,
+                                             C.c_uint64]),
+    "gc_batch_generate_disks": (C.c_int, [C.c_void_p, C.POINTER(DiskGalaxy), C.c_int,
+                                          C.POINTER(C.c_uint64)]),
     "gc_generate_disks": (C.c_size_t, [C.POINTER(DiskGalaxy), C.c_int, C.c_double, C.c_double,
                                        C.c_uint64, _dp, _dp, _dp]),
     "gc_generate_plummer": (C.c_size_t, [C.c_size_t, C.c_double, C.c_double, C.c_double,

@@ -66,6 +66,21 @@ class Batch:
         self._check(self._lib.gc_batch_upload(self._b, _ptr(pos), None if v is None else _ptr(v),
                                               _ptr(mass)), "batch upload")
 
+    def generate_collisions(self, runs):
+        """Draw every run's two-disk collision on the device (runs: the
+        dicts of generators.collisions, in batch order; each run's n must
+        equal its particle count here)."""
+        from .generators import collision_galaxies
+
+        gals = (L.DiskGalaxy * (2 * len(runs)))()
+        seeds = (C.c_uint64 * len(runs))()
+        for r, run in enumerate(runs):
+            pair = collision_galaxies(run)
+            gals[2 * r], gals[2 * r + 1] = pair[0], pair[1]
+            seeds[r] = int(run["seed"])
+        self._check(self._lib.gc_batch_generate_disks(self._b, gals, 2, seeds),
+                    "batch generate collisions")
+
     def download(self):
         pos, vel, acc = (np.empty((self.n, 3)) for _ in range(3))
         self._check(self._lib.gc_batch_download(self._b, _ptr(pos), _ptr(vel), _ptr(acc)),

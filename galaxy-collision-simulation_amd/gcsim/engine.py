@@ -192,6 +192,29 @@ class Engine:
             self._check(self._lib.gc_compute_accelerations_for_targets(self._ctx, b, e),
                         "compute accelerations for targets")
 
+    # -- initial conditions drawn on the device (counter-based Philox stream;
+    #    the state is replaced, accelerations zeroed)
+    def generate_reference_galaxies(self, galaxies, seed=42):
+        """generate_disk_galaxy's model for each GalaxyConfig dict."""
+        from .generators import galaxy_configs
+
+        arr = galaxy_configs(galaxies)
+        self._check(self._lib.gc_generate_galaxies_device(self._ctx, arr, len(galaxies), int(seed)),
+                    "generate galaxies")
+
+    def generate_two_disk(self, n_per_galaxy=500, seed=42, **galaxy_kw):
+        """The two-disk collision (generators.two_disk_galaxies' keywords)."""
+        from .generators import two_disk_galaxies
+
+        gals = two_disk_galaxies(n_per_galaxy, **galaxy_kw)
+        self._check(self._lib.gc_generate_disks_device(self._ctx, gals, 2, int(seed)),
+                    "generate two-disk")
+
+    def generate_plummer(self, n, total_mass=1.0, scale=1.0, seed=1):
+        self._check(self._lib.gc_generate_plummer_device(self._ctx, int(n), float(total_mass),
+                                                         float(scale), int(seed)),
+                    "generate plummer")
+
     def step(self, dt, steps=1):
         self._check(self._lib.gc_step(self._ctx, float(dt), int(steps)), "step")
 

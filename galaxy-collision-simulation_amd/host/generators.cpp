@@ -18,16 +18,32 @@
 // plummer: Aarseth, Henon & Wielen (1974), G = M = a = 1 scaled by
 //   (mass, scale): r = (U^(-2/3) - 1)^(-1/2); speed q * sqrt(2) (1+r^2)^(-1/4)
 //   with q drawn by rejection from q^2 (1-q^2)^(7/2); isotropic directions.
+//
+// reference galaxies: generate_galaxies (cpp/core/initial_conditions.cpp
+//   :37-114) restated: one std::mt19937_64(seed) shared by the galaxies in
+//   order; per galaxy a fresh normal distribution.  The draws follow the
+//   standard library the reference is built with (GNU libstdc++, the C++17
+//   <random> of GCC 11):
+//     uniform_real_distribution(0, 1) = generate_canonical<double, 53>:
+//       one 64-bit draw u, u / 2^64 in double, 1.0 replaced by the largest
+//       double below 1;
+//     normal_distribution(0, 1) = Marsaglia's polar method: x, y = 2 c - 1
+//       from two canonical draws until 0 < x^2 + y^2 <= 1, then
+//       f = sqrt(-2 ln r2 / r2); returns y f and keeps x f for the next call.
+//   With the same libm the particles equal the reference's bit for bit.
 #include "gcsim.h"
+#include "ic_models.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <random>
 
+#pragma STDC FP_CONTRACT OFF
+
 namespace {
 
-constexpr double kPi = 3.14159265358979323846;
+using gc::ic::kPi;
 
 struct Rng {
     std::mt19937_64 eng;
@@ -41,53 +57,53 @@ struct Rng {
     }
 };
 
-void rotate_x(double angle, double* v) {
-    const double c = std::cos(angle), s = std::sin(angle);
-    const double y = c * v[1] - s * v[2];
-    const double z = s * v[1] + c * v[2];
-    v[1] = y;
-    v[2] = z;
-}
-
 // One galaxy: central mass at index 0, then n_disk disk particles.
 void one_disk(Rng& rng, const gc_disk_galaxy& g, double G, double eps, double* pos, double* vel,
               double* mass) {
-    const double h = g.scale_length;
-    const double ea = std::exp(-g.r_min / h), eb = std::exp(-g.r_max / h);
-    const size_t nd = g.n_particles > 0 ? g.n_particles - 1 : 0;
-    const double md = g.disk_particle_mass > 0.0 ? g.disk_particle_mass
-                                                 : (nd > 0 ? g.disk_mass / static_cast<double>(nd) : 0.0);
-    const double disk_total = md * static_cast<double>(nd);
+    double p[gc::ic::kTdCount];
+    gc::ic::two_disk_params(g, G, eps, p);
     for (size_t k = 0; k < g.n_particles; ++k) {
-        double p[3] = {0, 0, 0}, v[3] = {0, 0, 0}, m;
-        if (k == 0) {
-            m = g.central_mass;
-        } else {
-            m = md;
-            const double u = rng.uniform01();
-            const double R = -h * std::log(ea - u * (ea - eb));
-            const double phi = 2.0 * kPi * rng.uniform01();
-            const double zz = g.z_sigma * rng.normal();
-            p[0] = R * std::cos(phi);
-            p[1] = R * std::sin(phi);
-            p[2] = zz;
-            const double cdf = (ea - std::exp(-R / h)) / (ea - eb);
-            const double menc = g.central_mass + disk_total * cdf;
-            const double s2 = R * R + eps * eps;
-            const double vc = std::sqrt(G * menc * R * R / (s2 * std::sqrt(s2)));
-            v[0] = -std::sin(phi) * vc + g.v_sigma * rng.normal();
-            v[1] = std::cos(phi) * vc + g.v_sigma * rng.normal();
-            v[2] = g.v_sigma * rng.normal();
+        double ur = 0, uphi = 0, nz = 0, nvx = 0, nvy = 0, nvz = 0;
+        if (k > 0) {
+            ur = rng.uniform01();
+            uphi = rng.uniform01();
+            nz = rng.normal();
+            nvx = rng.normal();
+            nvy = rng.normal();
+            nvz = rng.normal();
         }
-        rotate_x(g.inclination, p);
-        rotate_x(g.inclination, v);
-        for (int d = 0; d < 3; ++d) {
-            pos[3 * k + d] = g.position[d] + p[d];
-            vel[3 * k + d] = g.velocity[d] + v[d];
-        }
-        mass[k] = m;
+        gc::ic::two_disk_particle(p, k, ur, uphi, nz, nvx, nvy, nvz, pos + 3 * k, vel + 3 * k,
+                                  mass + k);
     }
 }
+
+// generate_canonical<double, 53> over std::mt19937_64 (GNU libstdc++)
+double canonical(std::mt19937_64& eng) {
+    const double r = static_cast<double>(eng()) / 18446744073709551616.0;  // 2^64
+    return r >= 1.0 ? std::nextafter(1.0, 0.0) : r;
+}
+
+// normal_distribution<double>(0, 1) (GNU libstdc++): Marsaglia's polar method
+struct PolarNormal {
+    bool saved_available = false;
+    double saved = 0.0;
+    double operator()(std::mt19937_64& eng) {
+        if (saved_available) {
+            saved_available = false;
+            return saved;
+        }
+        double x, y, r2;
+        do {
+            x = 2.0 * canonical(eng) - 1.0;
+            y = 2.0 * canonical(eng) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        const double f = std::sqrt(-2 * std::log(r2) / r2);
+        saved = x * f;
+        saved_available = true;
+        return y * f;
+    }
+};
 
 }  // namespace
 
@@ -124,31 +140,63 @@ size_t gc_generate_plummer(size_t n, double total_mass, double scale, double G, 
                            double* pos, double* vel, double* mass) {
     if (n == 0 || !pos || !vel || !mass) return 0;
     Rng rng(seed);
-    const double vscale = std::sqrt(G * total_mass / scale);
+    const double p[gc::ic::kPlCount] = {static_cast<double>(n), total_mass, scale, G};
     for (size_t i = 0; i < n; ++i) {
         double u = rng.uniform01();
         while (u <= 0.0) u = rng.uniform01();
-        const double r = 1.0 / std::sqrt(std::pow(u, -2.0 / 3.0) - 1.0);
         double ct = rng.uniform(-1.0, 1.0), ph = rng.uniform(0.0, 2.0 * kPi);
-        double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
-        pos[3 * i] = scale * r * st * std::cos(ph);
-        pos[3 * i + 1] = scale * r * st * std::sin(ph);
-        pos[3 * i + 2] = scale * r * ct;
+        double r;
+        gc::ic::plummer_position(p, u, ct, ph, &r, pos + 3 * i);
         double q, g;
         do {
             q = rng.uniform01();
             g = 0.1 * rng.uniform01();
-        } while (g > q * q * std::pow(1.0 - q * q, 3.5));
-        const double speed = q * std::sqrt(2.0) * std::pow(1.0 + r * r, -0.25);
+        } while (!gc::ic::plummer_accept(q, g));
         ct = rng.uniform(-1.0, 1.0);
         ph = rng.uniform(0.0, 2.0 * kPi);
-        st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
-        vel[3 * i] = vscale * speed * st * std::cos(ph);
-        vel[3 * i + 1] = vscale * speed * st * std::sin(ph);
-        vel[3 * i + 2] = vscale * speed * ct;
-        mass[i] = total_mass / static_cast<double>(n);
+        gc::ic::plummer_velocity(p, r, q, ct, ph, vel + 3 * i, mass + i);
     }
     return n;
+}
+
+void gc_default_galaxy_config(gc_galaxy_config* c) {
+    if (!c) return;
+    // GalaxyConfig defaults (cpp/core/initial_conditions.hpp:14-24)
+    c->n_particles = 256;
+    c->mass = 1.0;
+    c->radius = 1.0;
+    for (int d = 0; d < 3; ++d) c->position[d] = c->velocity[d] = 0.0;
+    c->orientation = 0.0;
+    c->group_id = 0;
+    c->thickness = 0.0;
+    c->inclination = 0.0;
+}
+
+size_t gc_generate_galaxies(const gc_galaxy_config* galaxies, int count, double G, double eps,
+                            uint64_t seed, double* pos, double* vel, double* mass,
+                            uint32_t* group_ids) {
+    if (count < 0 || (count > 0 && !galaxies) || !pos || !vel || !mass) return 0;
+    std::mt19937_64 eng(seed);
+    size_t off = 0;
+    for (int g = 0; g < count; ++g) {
+        const gc_galaxy_config& c = galaxies[g];
+        if (gc::ic::ref_disk_empty(c)) continue;  // no particles, no draws
+        double p[gc::ic::kRdCount];
+        gc::ic::ref_disk_params(c, G, eps, p);
+        PolarNormal normal;
+        for (size_t i = 0; i < c.n_particles; ++i, ++off) {
+            const double ur = canonical(eng);
+            const double ut = canonical(eng);
+            const double nz = c.thickness > 0.0 ? normal(eng) : 0.0;
+            const double nvx = normal(eng);
+            const double nvy = normal(eng);
+            const double nvz = normal(eng);
+            gc::ic::ref_disk_particle(p, ur, ut, nz, nvx, nvy, nvz, pos + 3 * off, vel + 3 * off,
+                                      mass + off);
+            if (group_ids) group_ids[off] = c.group_id;
+        }
+    }
+    return off;
 }
 
 int gc_sample_collisions(int runs, uint64_t seed, int n_min, int n_max, gc_collision* out) {

@@ -308,7 +308,56 @@ typedef struct {
     uint64_t seed;           /* per-run generator seed */
 } gc_collision;
 
+/* GalaxyConfig (cpp/core/initial_conditions.hpp:14-24) */
+typedef struct {
+    size_t n_particles;   /* default 256 */
+    double mass;          /* total galaxy mass, default 1 */
+    double radius;        /* disk radius (exponential scale radius / 3), default 1 */
+    double position[3], velocity[3];
+    double orientation;   /* rotation about z (radians), applied last */
+    uint32_t group_id;
+    double thickness;     /* Gaussian z scale; 0 = flat (no draw) */
+    double inclination;   /* rotation about x (radians), applied first */
+} gc_galaxy_config;
+
+void gc_default_galaxy_config(gc_galaxy_config* c);
+/* generate_galaxies (cpp/core/initial_conditions.cpp:94-114) on the host:
+ * one mt19937_64(seed) over the galaxies in order, GNU libstdc++'s
+ * uniform and normal draws; bit-identical to the reference built with the
+ * same libm.  Writes sum(n_particles) particles (galaxies with n = 0,
+ * mass <= 0 or radius <= 0 contribute none) and returns that count;
+ * group_ids may be NULL.  G and eps are PhysicsParams. */
+size_t gc_generate_galaxies(const gc_galaxy_config* galaxies, int count, double G, double eps,
+                            uint64_t seed, double* pos, double* vel, double* mass,
+                            uint32_t* group_ids);
+
 void gc_default_disk_galaxy(gc_disk_galaxy* g);
+
+/* ---- initial conditions in HBM (no reference counterpart: the reference
+ * generates on the host with one sequential mt19937_64) ----
+ * Each particle draws from the counter-based Philox4x32-10 stream keyed by
+ * the seed, with counter (particle index within its galaxy, galaxy index,
+ * draw block), so particles are independent and one launch writes the
+ * resident state (accelerations zeroed) with nothing crossing PCIe.  The
+ * models are the host generators' (positions and velocities from the same
+ * formulas); the particles differ from the host generators' because the
+ * draws do.  G and eps come from the context's (batch's) parameters. */
+/* generate_disk_galaxy's model per gc_galaxy_config (empty galaxies skipped) */
+gc_status gc_generate_galaxies_device(gc_ctx* ctx, const gc_galaxy_config* galaxies, int count,
+                                      uint64_t seed);
+/* the two-disk model of gc_generate_disks */
+gc_status gc_generate_disks_device(gc_ctx* ctx, const gc_disk_galaxy* galaxies, int count,
+                                   uint64_t seed);
+/* the Plummer model of gc_generate_plummer */
+gc_status gc_generate_plummer_device(gc_ctx* ctx, size_t n, double total_mass, double scale,
+                                     uint64_t seed);
+/* Run r's particles drawn on the device: galaxies[r * galaxies_per_run ..]
+ * in order (the two-disk model of gc_generate_disks_device, stream = the
+ * galaxy's index within its run) from seeds[r]; each run's galaxies must
+ * hold exactly its particle count.  Replaces velocities and masses too and
+ * zeroes accelerations. */
+gc_status gc_batch_generate_disks(gc_batch* b, const gc_disk_galaxy* galaxies,
+                                  int galaxies_per_run, const uint64_t* seeds);
 /* Writes sum(n_particles) particles; returns that count (0 on bad input). */
 size_t gc_generate_disks(const gc_disk_galaxy* galaxies, int count, double G, double eps,
                          uint64_t seed, double* pos, double* vel, double* mass);

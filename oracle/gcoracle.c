@@ -10,6 +10,7 @@
  * same traversal order, because bit-level parity with the reference's golden
  * files depends on all three.
  */
+#define _GNU_SOURCE /* sincos */
 #include "gcoracle.h"
 
 #include <math.h>
@@ -850,5 +851,280 @@ int orc_tree(size_t n, const double* pos, const double* mass, double G, double e
     free(t.nodes);
     free(t.idx);
     free(t.scratch);
+    return 0;
+}
+
+
+/* ===================================================================== */
+/* initial conditions                                                     */
+/* ===================================================================== */
+
+/* MT19937-64 (Nishimura 2000; the C++ standard's mt19937_64). */
+#define MT_N 312
+#define MT_M 156
+typedef struct {
+    uint64_t mt[MT_N];
+    int i;
+} orc_mt64;
+
+static void mt64_seed(orc_mt64* s, uint64_t seed) {
+    s->mt[0] = seed;
+    for (int k = 1; k < MT_N; ++k)
+        s->mt[k] = 6364136223846793005ULL * (s->mt[k - 1] ^ (s->mt[k - 1] >> 62)) + (uint64_t)k;
+    s->i = MT_N;
+}
+
+static uint64_t mt64_next(orc_mt64* s) {
+    if (s->i >= MT_N) {
+        for (int k = 0; k < MT_N; ++k) {
+            const uint64_t x = (s->mt[k] & 0xFFFFFFFF80000000ULL) |
+                               (s->mt[(k + 1) % MT_N] & 0x7FFFFFFFULL);
+            uint64_t xa = x >> 1;
+            if (x & 1ULL) xa ^= 0xB5026F5AA96619E9ULL;
+            s->mt[k] = s->mt[(k + MT_M) % MT_N] ^ xa;
+        }
+        s->i = 0;
+    }
+    uint64_t y = s->mt[s->i++];
+    y ^= (y >> 29) & 0x5555555555555555ULL;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ULL;
+    y ^= (y << 37) & 0xFFF7EEE000000000ULL;
+    y ^= y >> 43;
+    return y;
+}
+
+uint64_t orc_mt64_nth(uint64_t seed, size_t n) {
+    orc_mt64 s;
+    mt64_seed(&s, seed);
+    uint64_t y = 0;
+    for (size_t k = 0; k < n; ++k) y = mt64_next(&s);
+    return y;
+}
+
+/* uniform_real_distribution<double>(0, 1) of GNU libstdc++:
+ * generate_canonical<double, 53> takes one 64-bit draw, u / 2^64. */
+static double mt64_canonical(orc_mt64* s) {
+    const double r = (double)mt64_next(s) / 18446744073709551616.0;
+    return r >= 1.0 ? nextafter(1.0, 0.0) : r;
+}
+
+/* normal_distribution<double>(0, 1) of GNU libstdc++: Marsaglia's polar
+ * method, the second value of each accepted pair kept for the next call. */
+typedef struct {
+    int saved_available;
+    double saved;
+} orc_polar;
+
+static double polar_normal(orc_polar* p, orc_mt64* s) {
+    if (p->saved_available) {
+        p->saved_available = 0;
+        return p->saved;
+    }
+    double x, y, r2;
+    do {
+        x = 2.0 * mt64_canonical(s) - 1.0;
+        y = 2.0 * mt64_canonical(s) - 1.0;
+        r2 = x * x + y * y;
+    } while (r2 > 1.0 || r2 == 0.0);
+    const double f = sqrt(-2 * log(r2) / r2);
+    p->saved = x * f;
+    p->saved_available = 1;
+    return y * f;
+}
+
+static const double kOrcPi = 3.141592653589793238462643383279502884;
+
+/* rotate_x then rotate_z (initial_conditions.cpp:12-26); the reference's
+ * GCC build evaluates each cos/sin pair of one angle with sincos. */
+static void orient(double orientation, double inclination, double v[3]) {
+    double s, c;
+    sincos(inclination, &s, &c);
+    const double y1 = c * v[1] - s * v[2];
+    const double z1 = s * v[1] + c * v[2];
+    sincos(orientation, &s, &c);
+    const double x2 = c * v[0] - s * y1;
+    const double y2 = s * v[0] + c * y1;
+    v[0] = x2;
+    v[1] = y2;
+    v[2] = z1;
+}
+
+/* One particle of generate_disk_galaxy (initial_conditions.cpp:28-33,
+ * 52-88) from its draws.  g: n, mass, radius, pos[3], vel[3], orientation,
+ * thickness, inclination, G, eps. */
+static void ref_disk_from_draws(const double* g, double ur, double ut, double nz, double n1,
+                                double n2, double n3, double* x, double* v, double* m) {
+    const double R = g[2], M = g[1], G = g[12], eps = g[13];
+    const double pm = M / g[0];
+    const double disp = 0.025 * sqrt(G * M / R);
+    const double scale = R / 3.0;
+    const double max_cdf = 1.0 - exp(-R / scale);
+    const double u = ur < 1.0e-12 ? 1.0e-12 : (1.0 - 1.0e-12 < ur ? 1.0 - 1.0e-12 : ur);
+    const double r = -scale * log(1.0 - u * max_cdf);
+    const double theta = 2.0 * kOrcPi * ut;
+    double st, ct;
+    sincos(theta, &st, &ct);
+    x[0] = r * ct;
+    x[1] = r * st;
+    x[2] = g[10] > 0.0 ? g[10] * nz : 0.0;
+    double frac = (r * r) / (R * R);
+    if (!(frac < 1.0)) frac = 1.0;
+    double menc = M * frac;
+    if (menc < pm) menc = pm;
+    const double s2 = r * r + eps * eps;
+    const double vc = sqrt(G * menc * r * r / pow(s2, 1.5));
+    v[0] = -st * vc + disp * n1;
+    v[1] = ct * vc + disp * n2;
+    v[2] = 0.0 * vc + 0.25 * disp * n3;
+    orient(g[9], g[11], x);
+    orient(g[9], g[11], v);
+    for (int d = 0; d < 3; ++d) {
+        x[d] = g[3 + d] + x[d];
+        v[d] = g[6 + d] + v[d];
+    }
+    *m = pm;
+}
+
+size_t orc_generate_galaxies(const double* gal, int count, double G, double eps, uint64_t seed,
+                             double* pos, double* vel, double* mass) {
+    orc_mt64 s;
+    mt64_seed(&s, seed);
+    size_t off = 0;
+    for (int k = 0; k < count; ++k) {
+        const double* c = gal + 12 * k; /* n, mass, radius, pos3, vel3, orient, thick, incl */
+        if (c[0] < 1.0 || c[1] <= 0.0 || c[2] <= 0.0) continue;
+        const double g[14] = {c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7],
+                              c[8], c[9], c[10], c[11], G, eps};
+        orc_polar normal = {0, 0.0};
+        for (size_t i = 0; i < (size_t)c[0]; ++i, ++off) {
+            const double ur = mt64_canonical(&s);
+            const double ut = mt64_canonical(&s);
+            const double nz = c[10] > 0.0 ? polar_normal(&normal, &s) : 0.0;
+            const double n1 = polar_normal(&normal, &s);
+            const double n2 = polar_normal(&normal, &s);
+            const double n3 = polar_normal(&normal, &s);
+            ref_disk_from_draws(g, ur, ut, nz, n1, n2, n3, pos + 3 * off, vel + 3 * off,
+                                mass + off);
+        }
+    }
+    return off;
+}
+
+/* Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]}, k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t a = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t b = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t o0 = (uint32_t)(b >> 32) ^ c[1] ^ k0;
+        const uint32_t o1 = (uint32_t)b;
+        const uint32_t o2 = (uint32_t)(a >> 32) ^ c[3] ^ k1;
+        const uint32_t o3 = (uint32_t)a;
+        c[0] = o0;
+        c[1] = o1;
+        c[2] = o2;
+        c[3] = o3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    memcpy(out, c, sizeof(c));
+}
+
+/* Draw block b of particle i: two uniforms of 53 bits in [0, 1). */
+static void philox_block(uint64_t seed, uint64_t i, uint32_t stream, uint32_t b, double u[2]) {
+    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), stream, b};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    u[0] = ldexp((double)(((uint64_t)o[0] << 21) ^ (o[1] >> 11)), -53);
+    u[1] = ldexp((double)(((uint64_t)o[2] << 21) ^ (o[3] >> 11)), -53);
+}
+
+static void gauss_pair(const double u[2], double* a, double* b) {
+    const double r = sqrt(-2.0 * log(1.0 - u[0]));
+    *a = r * cos(2.0 * kOrcPi * u[1]);
+    *b = r * sin(2.0 * kOrcPi * u[1]);
+}
+
+/* The device generators' models on the counter-based stream (blocks 0-2:
+ * (u_r, u_angle), normals (z, vx), normals (vy, vz); Plummer: blocks 0-2
+ * (u, cos theta, phi, cos theta', phi'), rejection pairs from block 3). */
+int orc_ic_generate(int model, const double* p, uint64_t seed, uint32_t stream, size_t n,
+                    double* pos, double* vel, double* mass) {
+    for (size_t i = 0; i < n; ++i) {
+        double b0[2], b1[2], b2[2], n0, n1, n2, n3;
+        philox_block(seed, i, stream, 0, b0);
+        philox_block(seed, i, stream, 1, b1);
+        philox_block(seed, i, stream, 2, b2);
+        double* x = pos + 3 * i;
+        double* v = vel + 3 * i;
+        if (model == 0) { /* reference disk: p as ref_disk_from_draws' g */
+            gauss_pair(b1, &n0, &n1);
+            gauss_pair(b2, &n2, &n3);
+            ref_disk_from_draws(p, b0[0], b0[1], n0, n1, n2, n3, x, v, mass + i);
+        } else if (model == 1) {
+            /* two-disk galaxy: n, central, md, disk total, h, r_min, r_max,
+             * z_sigma, v_sigma, inclination, pos3, vel3, G, eps */
+            double px[3] = {0, 0, 0}, pv[3] = {0, 0, 0};
+            if (i == 0) {
+                mass[i] = p[1];
+            } else {
+                gauss_pair(b1, &n0, &n1);
+                gauss_pair(b2, &n2, &n3);
+                const double h = p[4];
+                const double ea = exp(-p[5] / h), eb = exp(-p[6] / h);
+                const double R = -h * log(ea - b0[0] * (ea - eb));
+                const double phi = 2.0 * kOrcPi * b0[1];
+                px[0] = R * cos(phi);
+                px[1] = R * sin(phi);
+                px[2] = p[7] * n0;
+                const double cdf = (ea - exp(-R / h)) / (ea - eb);
+                const double menc = p[1] + p[3] * cdf;
+                const double s2 = R * R + p[17] * p[17];
+                const double vc = sqrt(p[16] * menc * R * R / (s2 * sqrt(s2)));
+                pv[0] = -sin(phi) * vc + p[8] * n1;
+                pv[1] = cos(phi) * vc + p[8] * n2;
+                pv[2] = p[8] * n3;
+                mass[i] = p[2];
+            }
+            const double c = cos(p[9]), sn = sin(p[9]);
+            const double y0 = c * px[1] - sn * px[2], z0 = sn * px[1] + c * px[2];
+            const double y1 = c * pv[1] - sn * pv[2], z1 = sn * pv[1] + c * pv[2];
+            x[0] = p[10] + px[0];
+            x[1] = p[11] + y0;
+            x[2] = p[12] + z0;
+            v[0] = p[13] + pv[0];
+            v[1] = p[14] + y1;
+            v[2] = p[15] + z1;
+        } else if (model == 2) { /* Plummer: n, total mass, scale, G */
+            const double u = b0[0];
+            const double ct = -1.0 + 2.0 * b0[1], ph = 2.0 * kOrcPi * b1[0];
+            const double r = 1.0 / sqrt(pow(u, -2.0 / 3.0) - 1.0);
+            double st = 1.0 - ct * ct;
+            st = sqrt(st > 0.0 ? st : 0.0);
+            x[0] = p[2] * r * st * cos(ph);
+            x[1] = p[2] * r * st * sin(ph);
+            x[2] = p[2] * r * ct;
+            double q = 0.0;
+            for (uint32_t a = 0; a < 1000; ++a) {
+                double w[2];
+                philox_block(seed, i, stream, 3 + a, w);
+                q = w[0];
+                const double g = 0.1 * w[1];
+                if (!(g > q * q * pow(1.0 - q * q, 3.5))) break;
+            }
+            const double ct2 = -1.0 + 2.0 * b1[1], ph2 = 2.0 * kOrcPi * b2[0];
+            const double vscale = sqrt(p[3] * p[1] / p[2]);
+            const double speed = q * sqrt(2.0) * pow(1.0 + r * r, -0.25);
+            double st2 = 1.0 - ct2 * ct2;
+            st2 = sqrt(st2 > 0.0 ? st2 : 0.0);
+            v[0] = vscale * speed * st2 * cos(ph2);
+            v[1] = vscale * speed * st2 * sin(ph2);
+            v[2] = vscale * speed * ct2;
+            mass[i] = p[1] / p[0];
+        } else {
+            return 1;
+        }
+    }
     return 0;
 }

@@ -30,6 +30,7 @@
 #define GCORACLE_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -105,6 +106,25 @@ typedef struct {
 int orc_tree(size_t n, const double* pos, const double* mass, double G, double eps,
              double theta, size_t leaf_capacity, int max_depth, int expansion_order,
              size_t begin, size_t end, double* acc, orc_tree_stats* stats);
+
+/* ---- initial conditions -------------------------------------------- */
+/* n-th output (1-based) of mt19937_64(seed). */
+uint64_t orc_mt64_nth(uint64_t seed, size_t n);
+/* generate_galaxies (cpp/core/initial_conditions.cpp:94-114) with GNU
+ * libstdc++'s uniform and normal draws.  gal: count x 12 doubles (n, mass,
+ * radius, pos[3], vel[3], orientation, thickness, inclination).  Returns
+ * the particle count written. */
+size_t orc_generate_galaxies(const double* gal, int count, double G, double eps, uint64_t seed,
+                             double* pos, double* vel, double* mass);
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* The device generators' models on the Philox stream of (seed, stream):
+ * model 0 reference disk (p: n, mass, radius, pos3, vel3, orientation,
+ * thickness, inclination, G, eps), 1 two-disk galaxy (p: n, central mass,
+ * disk particle mass, disk total, scale length, r_min, r_max, z_sigma,
+ * v_sigma, inclination, pos3, vel3, G, eps), 2 Plummer (p: n, total mass,
+ * scale, G).  Returns 0, or 1 for an unknown model. */
+int orc_ic_generate(int model, const double* p, uint64_t seed, uint32_t stream, size_t n,
+                    double* pos, double* vel, double* mass);
 
 #ifdef __cplusplus
 }

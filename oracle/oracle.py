@@ -354,5 +354,51 @@ def ref_write_run(directory, meta, step, time, pos, vel, acc, mass, groups, diag
         raise RuntimeError("reference SnapshotWriter threw")
 
 
+# ---------------------------------------------------------------- initial conditions
+IC_REF_DISK, IC_TWO_DISK, IC_PLUMMER = 0, 1, 2
+
+
+def mt64_nth(seed, n):
+    lib().orc_mt64_nth.restype = C.c_uint64
+    return int(lib().orc_mt64_nth(C.c_uint64(seed), _size_t(n)))
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def _gal_rows(galaxies):
+    rows = [[g["n_particles"], g["mass"], g["radius"], *g["position"], *g["velocity"],
+             g["orientation"], g["thickness"], g["inclination"]] for g in galaxies]
+    return np.ascontiguousarray(np.array(rows, dtype=np.float64).reshape(-1, 12))
+
+
+def generate_galaxies(galaxies, G, eps, seed):
+    """generate_galaxies restated (MT19937-64 + libstdc++ draws); galaxies:
+    dicts with the GalaxyConfig fields."""
+    gal = _gal_rows(galaxies)
+    n = int(sum(g["n_particles"] for g in galaxies))
+    pos, vel, mass = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    lib().orc_generate_galaxies.restype = _size_t
+    got = lib().orc_generate_galaxies(_p(gal), C.c_int(len(galaxies)), C.c_double(G),
+                                      C.c_double(eps), C.c_uint64(seed), _p(pos), _p(vel), _p(mass))
+    return pos[:got], vel[:got], mass[:got]
+
+
+def ic_generate(model, params, seed, stream, n):
+    """The device generators' models on the Philox stream (seed, stream)."""
+    par = _c64(params)
+    pos, vel, mass = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros(n)
+    rc = lib().orc_ic_generate(C.c_int(model), _p(par), C.c_uint64(seed), C.c_uint32(stream),
+                               _size_t(n), _p(pos), _p(vel), _p(mass))
+    if rc:
+        raise ValueError(f"unknown model {model}")
+    return pos, vel, mass
+
+
 __all__ = [n for n in dir() if not n.startswith("_")] + ["HERE"]
 os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
