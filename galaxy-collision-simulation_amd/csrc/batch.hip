@@ -18,6 +18,7 @@
 // once into a hipGraph and replayed.  The (chunk, blocks) schedule is chosen
 // by timing candidates on the batch itself (gc_batch_autotune).
 #include "gc_internal.hpp"
+#include "abi_guard.hpp"
 #include "ic_models.hpp"
 #include "mixed_tiles.hpp"
 
@@ -391,6 +392,7 @@ const char* gc_batch_last_error(const gc_batch* b) { return b ? b->err.c_str() :
 
 gc_status gc_batch_create(gc_batch** out, int device, const gc_params* p, int runs,
                           const size_t* counts) {
+    GC_ABI_BEGIN
     if (!out || !p || runs <= 0 || !counts) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "batch_create: bad arguments");
     *out = nullptr;
     if (p->solver != GC_SOLVER_DIRECT || p->precision != GC_PRECISION_MIXED)
@@ -470,6 +472,7 @@ gc_status gc_batch_create(gc_batch** out, int device, const gc_params* p, int ru
     }
     *out = b;
     return GC_OK;
+    GC_ABI_END("gc_batch_create")
 }
 
 void gc_batch_destroy(gc_batch* b) {
@@ -495,6 +498,7 @@ size_t gc_batch_particle_count(const gc_batch* b) {
 }
 
 gc_status gc_batch_upload(gc_batch* b, const double* pos, const double* vel, const double* mass) {
+    GC_ABI_BEGIN
     if (!b || !pos || !mass) return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_upload: null argument");
     GB_HIP(b, hipSetDevice(b->device), "select device");
     gc_status st = transfer(b, pos, nullptr, b->s.x, b->s.y, b->s.z);
@@ -512,10 +516,12 @@ gc_status gc_batch_upload(gc_batch* b, const double* pos, const double* vel, con
     GB_HIP(b, hipMemcpy(b->s.m, m.data(), sizeof(double) * b->total, hipMemcpyHostToDevice), "upload masses");
     GB_HIP(b, hipMemset(b->s.ax, 0, sizeof(double) * 3 * b->total), "clear accelerations");
     return GC_OK;
+    GC_ABI_END("gc_batch_upload")
 }
 
 gc_status gc_batch_generate_disks(gc_batch* b, const gc_disk_galaxy* galaxies,
                                   int galaxies_per_run, const uint64_t* seeds) {
+    GC_ABI_BEGIN
     if (!b || !galaxies || !seeds || galaxies_per_run <= 0)
         return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_generate_disks: bad arguments");
     std::vector<IcSegment> segs;
@@ -559,9 +565,11 @@ gc_status gc_batch_generate_disks(gc_batch* b, const gc_disk_galaxy* galaxies,
     if (e == hipSuccess) e = e2;
     GB_HIP(b, e, "batch_generate_disks");
     return GC_OK;
+    GC_ABI_END("gc_batch_generate_disks")
 }
 
 gc_status gc_batch_download(gc_batch* b, double* pos, double* vel, double* acc) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "batch_download: null batch");
     GB_HIP(b, hipSetDevice(b->device), "select device");
     gc_status st = GC_OK;
@@ -569,18 +577,22 @@ gc_status gc_batch_download(gc_batch* b, double* pos, double* vel, double* acc) 
     if (vel && (st = transfer(b, nullptr, vel, b->s.vx, b->s.vy, b->s.vz)) != GC_OK) return st;
     if (acc && (st = transfer(b, nullptr, acc, b->s.ax, b->s.ay, b->s.az)) != GC_OK) return st;
     return GC_OK;
+    GC_ABI_END("gc_batch_download")
 }
 
 gc_status gc_batch_compute_accelerations(gc_batch* b) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     GB_HIP(b, hipSetDevice(b->device), "select device");
     launch_force(b);
     launch_merge(b, false, 0.0);
     GB_HIP(b, hipGetLastError(), "launch batch force");
     return GC_OK;
+    GC_ABI_END("gc_batch_compute_accelerations")
 }
 
 gc_status gc_batch_step(gc_batch* b, double dt, int nsteps) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     if (!(dt > 0.0) || nsteps < 0) return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_step: dt must be positive, steps non-negative");
     GB_HIP(b, hipSetDevice(b->device), "select device");
@@ -605,15 +617,19 @@ gc_status gc_batch_step(gc_batch* b, double dt, int nsteps) {
     for (; left > 0; --left) launch_step(b, dt);
     GB_HIP(b, hipGetLastError(), "launch batch step");
     return GC_OK;
+    GC_ABI_END("gc_batch_step")
 }
 
 gc_status gc_batch_synchronize(gc_batch* b) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     GB_HIP(b, hipStreamSynchronize(b->stream), "synchronize");
     return GC_OK;
+    GC_ABI_END("gc_batch_synchronize")
 }
 
 gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     const bool sym = batch_symmetric(b);
     if (chunk_tiles < (sym ? 0 : 1) || chunk_tiles > 4096 || blocks < (sym ? 0 : 1) ||
@@ -635,18 +651,22 @@ gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int gr
     }
     b->graph_steps = graph_steps;
     return GC_OK;
+    GC_ABI_END("gc_batch_set_schedule")
 }
 
 gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps) {
+    GC_ABI_BEGIN
     if (!b || !chunk_tiles || !blocks || !graph_steps) return bfail(b, GC_ERR_INVALID_ARGUMENT, "null argument");
     const bool sym = batch_symmetric(b);
     *chunk_tiles = sym ? b->sym_len : b->chunk;
     *blocks = sym ? b->sym_blocks : b->blocks;
     *graph_steps = b->graph_steps;
     return GC_OK;
+    GC_ABI_END("gc_batch_get_schedule")
 }
 
 gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms, double* interactions) {
+    GC_ABI_BEGIN
     if (!b || reps <= 0 || !ms) return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_time: bad arguments");
     GB_HIP(b, hipSetDevice(b->device), "select device");
     const float eps2 = static_cast<float>(b->params.softening * b->params.softening);
@@ -686,9 +706,11 @@ gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms, double* 
     launch_merge(b, false, 0.0);  // leave accelerations consistent with the state
     GB_HIP(b, hipGetLastError(), "merge after timing");
     return GC_OK;
+    GC_ABI_END("gc_batch_time_force_kernel")
 }
 
 gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     GB_HIP(b, hipSetDevice(b->device), "select device");
     long long items1 = 0;
@@ -781,9 +803,11 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
     if (chunk_out) *chunk_out = sym ? best_t.sym_len : best_t.chunk;
     if (blocks_out) *blocks_out = sym ? best_t.sym_blocks : best_t.blocks;
     return GC_OK;
+    GC_ABI_END("gc_batch_autotune")
 }
 
 gc_status gc_batch_set_pairs(gc_batch* b, int mode, int targets_per_lane) {
+    GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     if (mode != GC_PAIRS_AUTO && mode != GC_PAIRS_TARGETS && mode != GC_PAIRS_SYMMETRIC)
         return bfail(b, GC_ERR_INVALID_ARGUMENT, "pair mode must be 0, 1 or 2");
@@ -800,14 +824,17 @@ gc_status gc_batch_set_pairs(gc_batch* b, int mode, int targets_per_lane) {
     }
     drop_graph(b);
     return GC_OK;
+    GC_ABI_END("gc_batch_set_pairs")
 }
 
 gc_status gc_batch_get_pairs(gc_batch* b, int* mode, int* targets_per_lane) {
+    GC_ABI_BEGIN
     if (!b || !mode || !targets_per_lane) return bfail(b, GC_ERR_INVALID_ARGUMENT, "null argument");
     const bool sym = batch_symmetric(b);
     *mode = sym ? GC_PAIRS_SYMMETRIC : GC_PAIRS_TARGETS;
     *targets_per_lane = sym ? batch_sym_ipt(b) : 1;
     return GC_OK;
+    GC_ABI_END("gc_batch_get_pairs")
 }
 
 }  // extern "C"

@@ -3,6 +3,7 @@
 // mixed), KDK stepping and diagnostics.  No exception crosses the ABI; every
 // HIP error becomes GC_ERR_HIP with a message from gc_last_error.
 #include "gc_internal.hpp"
+#include "abi_guard.hpp"
 #include "gc_fmm.hpp"
 #include "ic_models.hpp"
 
@@ -358,6 +359,7 @@ const char* gc_status_string(gc_status s) {
 const char* gc_last_error(const gc_ctx* c) { return c ? c->err.c_str() : g_thread_err.c_str(); }
 
 gc_status gc_create(gc_ctx** out, int device, const gc_params* p) {
+    GC_ABI_BEGIN
     if (!out) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "gc_create: null output");
     *out = nullptr;
     gc_status st = check_params(nullptr, p);
@@ -385,6 +387,7 @@ gc_status gc_create(gc_ctx** out, int device, const gc_params* p) {
     c->stream = c->own_stream;
     *out = c;
     return GC_OK;
+    GC_ABI_END("gc_create")
 }
 
 void gc_destroy(gc_ctx* c) {
@@ -403,29 +406,36 @@ void gc_destroy(gc_ctx* c) {
 }
 
 gc_status gc_set_params(gc_ctx* c, const gc_params* p) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     gc_status st = check_params(c, p);
     if (st != GC_OK) return st;
     c->params = *p;
     return GC_OK;
+    GC_ABI_END("gc_set_params")
 }
 
 gc_status gc_set_stream(gc_ctx* c, void* stream) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     c->stream = static_cast<hipStream_t>(stream);
     return GC_OK;
+    GC_ABI_END("gc_set_stream")
 }
 
 gc_status gc_use_own_stream(gc_ctx* c) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     c->stream = c->own_stream;
     return GC_OK;
+    GC_ABI_END("gc_use_own_stream")
 }
 
 void* gc_get_stream(gc_ctx* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
 
 gc_status gc_upload(gc_ctx* c, size_t n, const double* pos, const double* vel,
                     const double* mass) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (n > 0 && (!pos || !mass)) return fail(c, GC_ERR_INVALID_ARGUMENT, "upload: null positions or masses");
@@ -448,9 +458,11 @@ gc_status gc_upload(gc_ctx* c, size_t n, const double* pos, const double* vel,
     GC_HIP(c, hipMemsetAsync(c->s.ax, 0, sizeof(double) * 3 * c->s.cap, c->stream), "clear accelerations");
     GC_HIP(c, hipStreamSynchronize(c->stream), "upload");
     return launch_status(c, "upload kernels");
+    GC_ABI_END("gc_upload")
 }
 
 gc_status gc_set_accelerations(gc_ctx* c, const double* acc) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (!acc) return fail(c, GC_ERR_INVALID_ARGUMENT, "set_accelerations: null array");
@@ -461,9 +473,11 @@ gc_status gc_set_accelerations(gc_ctx* c, const double* acc) {
     launch_scatter_xyz(c->stage, c->s.ax, c->s.ay, c->s.az, n, c->stream);
     GC_HIP(c, hipStreamSynchronize(c->stream), "upload accelerations");
     return launch_status(c, "upload accelerations");
+    GC_ABI_END("gc_set_accelerations")
 }
 
 gc_status gc_download(gc_ctx* c, double* pos, double* vel, double* acc) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     const size_t n = c->s.n;
@@ -481,11 +495,13 @@ gc_status gc_download(gc_ctx* c, double* pos, double* vel, double* acc) {
         GC_HIP(c, hipStreamSynchronize(c->stream), "download");
     }
     return launch_status(c, "download kernels");
+    GC_ABI_END("gc_download")
 }
 
 size_t gc_particle_count(const gc_ctx* c) { return c ? c->s.n : 0; }
 
 gc_status gc_device_state(gc_ctx* c, gc_device_view* v) {
+    GC_ABI_BEGIN
     if (!c || !v) return fail(c, GC_ERR_INVALID_ARGUMENT, "device_state: null argument");
     v->n = c->s.n;
     v->x = c->s.x;
@@ -500,24 +516,30 @@ gc_status gc_device_state(gc_ctx* c, gc_device_view* v) {
     v->mass = c->s.m;
     v->stream = c->stream;
     return GC_OK;
+    GC_ABI_END("gc_device_state")
 }
 
 gc_status gc_compute_accelerations(gc_ctx* c) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (c->s.n == 0) return GC_OK;
     return force(c, 0, c->s.n, false, 0.0);
+    GC_ABI_END("gc_compute_accelerations")
 }
 
 gc_status gc_compute_accelerations_for_targets(gc_ctx* c, size_t begin, size_t end) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     end = std::min(end, c->s.n);
     if (begin >= end) return GC_OK;
     return force(c, begin, end, false, 0.0);
+    GC_ABI_END("gc_compute_accelerations_for_targets")
 }
 
 gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (!(dt > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "dt must be positive");
@@ -530,10 +552,12 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
         if (st != GC_OK) return st;
     }
     return launch_status(c, "launch step kernels");
+    GC_ABI_END("gc_step")
 }
 
 gc_status gc_generate_galaxies_device(gc_ctx* c, const gc_galaxy_config* galaxies, int count,
                                       uint64_t seed) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (count < 0 || (count > 0 && !galaxies))
         return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_galaxies_device: bad galaxy list");
@@ -552,10 +576,12 @@ gc_status gc_generate_galaxies_device(gc_ctx* c, const gc_galaxy_config* galaxie
         n += galaxies[k].n_particles;
     }
     return generate_state(c, segs, n, "generate_galaxies_device");
+    GC_ABI_END("gc_generate_galaxies_device")
 }
 
 gc_status gc_generate_disks_device(gc_ctx* c, const gc_disk_galaxy* galaxies, int count,
                                    uint64_t seed) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (count < 0 || (count > 0 && !galaxies))
         return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_disks_device: bad galaxy list");
@@ -574,10 +600,12 @@ gc_status gc_generate_disks_device(gc_ctx* c, const gc_disk_galaxy* galaxies, in
         n += galaxies[k].n_particles;
     }
     return generate_state(c, segs, n, "generate_disks_device");
+    GC_ABI_END("gc_generate_disks_device")
 }
 
 gc_status gc_generate_plummer_device(gc_ctx* c, size_t n, double total_mass, double scale,
                                      uint64_t seed) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (!(scale > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_plummer_device: scale must be positive");
     std::vector<IcSegment> segs;
@@ -593,15 +621,19 @@ gc_status gc_generate_plummer_device(gc_ctx* c, size_t n, double total_mass, dou
         segs.push_back(sg);
     }
     return generate_state(c, segs, n, "generate_plummer_device");
+    GC_ABI_END("gc_generate_plummer_device")
 }
 
 gc_status gc_synchronize(gc_ctx* c) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize");
     return GC_OK;
+    GC_ABI_END("gc_synchronize")
 }
 
 gc_status gc_diagnostics_compute(gc_ctx* c, gc_diagnostics* out) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (!out) return fail(c, GC_ERR_INVALID_ARGUMENT, "diagnostics: null output");
@@ -624,9 +656,11 @@ gc_status gc_diagnostics_compute(gc_ctx* c, gc_diagnostics* out) {
         out->angular_momentum[k] = host[10 + k];
     }
     return GC_OK;
+    GC_ABI_END("gc_diagnostics_compute")
 }
 
 gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (reps <= 0 || !ms) return fail(c, GC_ERR_INVALID_ARGUMENT, "time_force_kernel: bad arguments");
@@ -680,10 +714,12 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
         launch_merge_mixed(c->s, c->mixed, js, 0, n, false, 0.0, c->stream);
     }
     return launch_status(c, "timed force kernels");
+    GC_ABI_END("gc_time_force_kernel")
 }
 
 gc_status gc_set_state_device(gc_ctx* c, size_t n, const double* pos, const double* vel,
                               const double* mass, const double* acc) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (n > 0 && (!pos || !mass)) return fail(c, GC_ERR_INVALID_ARGUMENT, "set_state_device: null positions or masses");
@@ -706,9 +742,11 @@ gc_status gc_set_state_device(gc_ctx* c, size_t n, const double* pos, const doub
     }
     GC_HIP(c, hipMemcpyAsync(c->s.m, mass, b, hipMemcpyDeviceToDevice, c->stream), "copy masses in");
     return GC_OK;
+    GC_ABI_END("gc_set_state_device")
 }
 
 gc_status gc_get_state_device(gc_ctx* c, double* pos, double* vel, double* acc) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     const size_t n = c->s.n;
@@ -723,9 +761,11 @@ gc_status gc_get_state_device(gc_ctx* c, double* pos, double* vel, double* acc) 
                    "copy device state out");
     }
     return GC_OK;
+    GC_ABI_END("gc_get_state_device")
 }
 
 gc_status gc_step_reverse(gc_ctx* c, double dt, int nsteps) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (!(dt > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "dt must be positive");
@@ -738,10 +778,12 @@ gc_status gc_step_reverse(gc_ctx* c, double dt, int nsteps) {
         if (st != GC_OK) return st;
     }
     return launch_status(c, "launch reverse step kernels");
+    GC_ABI_END("gc_step_reverse")
 }
 
 gc_status gc_force_vjp(gc_ctx* c, const double* lambda, double* xbar, double* mbar,
                        int accumulate) {
+    GC_ABI_BEGIN
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
     if (!lambda || !xbar || !mbar) return fail(c, GC_ERR_INVALID_ARGUMENT, "force_vjp: null array");
@@ -773,9 +815,11 @@ gc_status gc_force_vjp(gc_ctx* c, const double* lambda, double* xbar, double* mb
                      c->params.softening, lambda, stride, xbar, mbar, stride, accumulate != 0,
                      c->stream);
     return launch_status(c, "launch mixed force VJP");
+    GC_ABI_END("gc_force_vjp")
 }
 
 gc_status gc_fmm_last_stats(gc_ctx* c, gc_fmm_stats* out) {
+    GC_ABI_BEGIN
     if (!c || !out) return fail(c, GC_ERR_INVALID_ARGUMENT, "fmm_last_stats: null argument");
     if (!c->fmm) return fail(c, GC_ERR_STATE, "fmm_last_stats: no hip-fmm evaluation yet");
     *out = c->fmm->stats();
@@ -785,9 +829,11 @@ gc_status gc_fmm_last_stats(gc_ctx* c, gc_fmm_stats* out) {
         out->eval_ms = e;
     }
     return GC_OK;
+    GC_ABI_END("gc_fmm_last_stats")
 }
 
 gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms) {
+    GC_ABI_BEGIN
     if (!ms || bytes == 0 || bytes % 16 != 0 || reps <= 0)
         return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "calibration_copy: bad arguments");
     if (device < 0 || device >= gc_device_count())
@@ -797,9 +843,11 @@ gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms) {
         return fail(nullptr, e == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
                     std::string("calibration_copy: ") + hipGetErrorString(e));
     return GC_OK;
+    GC_ABI_END("gc_calibration_copy")
 }
 
 gc_status gc_set_launch_shape(gc_ctx* c, int ipt, int jsplit) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (ipt != 0 && ipt != 1 && ipt != 2 && ipt != 4)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "targets per thread must be 0, 1, 2 or 4");
@@ -807,9 +855,11 @@ gc_status gc_set_launch_shape(gc_ctx* c, int ipt, int jsplit) {
     c->shape.ipt = ipt;
     c->shape.jsplit = jsplit;
     return GC_OK;
+    GC_ABI_END("gc_set_launch_shape")
 }
 
 gc_status gc_get_launch_shape(gc_ctx* c, int* ipt, int* jsplit) {
+    GC_ABI_BEGIN
     if (!c || !ipt || !jsplit) return fail(c, GC_ERR_INVALID_ARGUMENT, "get_launch_shape: null argument");
     if (c->s.n > 0 && symmetric_pairs(c, 0, c->s.n)) {
         gc_status st = ensure_sym(c);
@@ -820,14 +870,17 @@ gc_status gc_get_launch_shape(gc_ctx* c, int* ipt, int* jsplit) {
     }
     mixed_shape(c, c->s.n, ipt, jsplit);
     return GC_OK;
+    GC_ABI_END("gc_get_launch_shape")
 }
 
 gc_status gc_set_sources(gc_ctx* c, int mode) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (mode != GC_SOURCES_AUTO && mode != GC_SOURCES_LDS && mode != GC_SOURCES_SCALAR)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "source mode must be 0, 1 or 2");
     c->shape.sources = mode;
     return GC_OK;
+    GC_ABI_END("gc_set_sources")
 }
 
 int gc_get_sources(const gc_ctx* c) {
@@ -836,11 +889,13 @@ int gc_get_sources(const gc_ctx* c) {
 }
 
 gc_status gc_set_pairs(gc_ctx* c, int mode) {
+    GC_ABI_BEGIN
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     if (mode != GC_PAIRS_AUTO && mode != GC_PAIRS_TARGETS && mode != GC_PAIRS_SYMMETRIC)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "pair mode must be 0, 1 or 2");
     c->shape.pairs = mode;
     return GC_OK;
+    GC_ABI_END("gc_set_pairs")
 }
 
 int gc_get_pairs(const gc_ctx* c) {

@@ -27,6 +27,7 @@
 #include <thread>
 #include <vector>
 
+#include "abi_guard.hpp"
 #include "gcsim.h"
 
 namespace gc {
@@ -39,7 +40,6 @@ gc_status stage_state(gc_ctx* c, double* dst, size_t dst_doubles, size_t* n, siz
 // in gc_diagnostics field order, into dst on the context's stream; records
 // ev after the copy.
 gc_status stage_diagnostics(gc_ctx* c, double* dst, hipEvent_t ev, size_t* n);
-gc_status set_error(const std::string& msg, gc_status code);
 }  // namespace gc
 
 namespace {
@@ -153,18 +153,29 @@ bool write_snapshot_file(const std::filesystem::path& dir, const Job& j, int thr
     const size_t nchunks = (j.n + kChunkRows - 1) / kChunkRows;
     const size_t width = std::max<size_t>(1, std::min<size_t>(threads, nchunks));
     std::vector<std::vector<char>> bufs(width);
+    std::vector<char> formatted(width);
     for (size_t c0 = 0; ok && c0 < nchunks; c0 += width) {
         const size_t cw = std::min(width, nchunks - c0);
         auto task = [&](size_t t) {
-            const size_t r0 = (c0 + t) * kChunkRows;
-            format_rows(j, r0, std::min(j.n, r0 + kChunkRows), bufs[t]);
+            try {
+                const size_t r0 = (c0 + t) * kChunkRows;
+                format_rows(j, r0, std::min(j.n, r0 + kChunkRows), bufs[t]);
+                formatted[t] = 1;
+            } catch (...) {
+                formatted[t] = 0;  // out of host memory: reported below
+            }
         };
         std::vector<std::thread> pool;
-        for (size_t t = 1; t < cw; ++t) pool.emplace_back(task, t);
+        size_t started = 1;
+        try {
+            for (; started < cw; ++started) pool.emplace_back(task, started);
+        } catch (...) {  // fewer threads: this thread formats the rest
+        }
+        for (size_t t = started; t < cw; ++t) task(t);
         task(0);
         for (auto& th : pool) th.join();
         for (size_t t = 0; ok && t < cw; ++t)
-            ok = std::fwrite(bufs[t].data(), 1, bufs[t].size(), f) == bufs[t].size();
+            ok = formatted[t] && std::fwrite(bufs[t].data(), 1, bufs[t].size(), f) == bufs[t].size();
     }
     ok = (std::fclose(f) == 0) && ok;
     if (!ok) *err = "Could not write snapshot output in " + dir.string();
@@ -219,12 +230,16 @@ struct gc_writer {
             std::string e;
             if (j.pool && hipEventSynchronize(j.pool->ev[j.slot]) != hipSuccess)
                 e = "device staging for step " + std::to_string(j.step) + " failed";
-            if (e.empty() && j.kind == Job::kSnapshot) {
-                write_snapshot_file(dir, j, threads, &e);
-            } else if (e.empty()) {
-                if (j.staged) std::memcpy(&j.diag, j.staged, sizeof(gc_diagnostics));
-                if (!write_diagnostics_row(diag, j))
-                    e = "Could not write diagnostics output in " + dir.string();
+            try {
+                if (e.empty() && j.kind == Job::kSnapshot) {
+                    write_snapshot_file(dir, j, threads, &e);
+                } else if (e.empty()) {
+                    if (j.staged) std::memcpy(&j.diag, j.staged, sizeof(gc_diagnostics));
+                    if (!write_diagnostics_row(diag, j))
+                        e = "Could not write diagnostics output in " + dir.string();
+                }
+            } catch (const std::exception& x) {  // host memory or threads ran out
+                e = std::string("snapshot for step ") + std::to_string(j.step) + ": " + x.what();
             }
             {
                 std::lock_guard<std::mutex> lock(mu);
@@ -293,6 +308,7 @@ struct gc_writer {
 extern "C" {
 
 gc_status gc_writer_open(gc_writer** out, const char* directory) {
+    GC_ABI_BEGIN
     if (!out || !directory) return gc::set_error("writer_open: null argument", GC_ERR_INVALID_ARGUMENT);
     *out = nullptr;
     gc_writer* w = new (std::nothrow) gc_writer();
@@ -315,9 +331,11 @@ gc_status gc_writer_open(gc_writer** out, const char* directory) {
     w->worker = std::thread([w] { w->run(); });
     *out = w;
     return GC_OK;
+    GC_ABI_END("gc_writer_open")
 }
 
 gc_status gc_writer_metadata(gc_writer* w, const gc_run_info* info) {
+    GC_ABI_BEGIN
     if (!w || !info) return gc::set_error("writer_metadata: null argument", GC_ERR_INVALID_ARGUMENT);
     // snapshot_writer.cpp:50-70, at iostream's default precision (%g)
     FILE* m = std::fopen((w->dir / "metadata.json").string().c_str(), "wb");
@@ -334,11 +352,13 @@ gc_status gc_writer_metadata(gc_writer* w, const gc_run_info* info) {
     if (std::fclose(m) != 0)
         return gc::set_error("Could not write metadata output in " + w->dir.string(), GC_ERR_IO);
     return GC_OK;
+    GC_ABI_END("gc_writer_metadata")
 }
 
 gc_status gc_writer_snapshot_host(gc_writer* w, int step, double time, size_t n, const double* pos,
                                   const double* vel, const double* acc, const double* mass,
                                   const std::uint32_t* group_ids) {
+    GC_ABI_BEGIN
     if (!w || (n > 0 && (!pos || !vel || !acc || !mass)))
         return gc::set_error("writer_snapshot_host: null argument", GC_ERR_INVALID_ARGUMENT);
     Job j;
@@ -358,10 +378,12 @@ gc_status gc_writer_snapshot_host(gc_writer* w, int step, double time, size_t n,
     if (group_ids) j.groups.assign(group_ids, group_ids + n);
     w->push(std::move(j));
     return GC_OK;
+    GC_ABI_END("gc_writer_snapshot_host")
 }
 
 gc_status gc_writer_snapshot(gc_writer* w, gc_ctx* ctx, int step, double time,
                              const std::uint32_t* group_ids) {
+    GC_ABI_BEGIN
     if (!w || !ctx) return gc::set_error("writer_snapshot: null argument", GC_ERR_INVALID_ARGUMENT);
     size_t n = 0, cap = 0;
     gc_status st = gc::stage_state(ctx, nullptr, 0, &n, &cap, nullptr);  // sizes only
@@ -384,10 +406,12 @@ gc_status gc_writer_snapshot(gc_writer* w, gc_ctx* ctx, int step, double time,
     if (group_ids) j.groups.assign(group_ids, group_ids + n);
     w->push(std::move(j));
     return GC_OK;
+    GC_ABI_END("gc_writer_snapshot")
 }
 
 gc_status gc_writer_diagnostics_values(gc_writer* w, int step, double time, size_t n,
                                        const gc_diagnostics* d) {
+    GC_ABI_BEGIN
     if (!w || !d) return gc::set_error("writer_diagnostics: null argument", GC_ERR_INVALID_ARGUMENT);
     Job j;
     j.kind = Job::kDiagnostics;
@@ -397,9 +421,11 @@ gc_status gc_writer_diagnostics_values(gc_writer* w, int step, double time, size
     j.diag = *d;
     w->push(std::move(j));
     return GC_OK;
+    GC_ABI_END("gc_writer_diagnostics_values")
 }
 
 gc_status gc_writer_diagnostics(gc_writer* w, gc_ctx* ctx, int step, double time) {
+    GC_ABI_BEGIN
     if (!w || !ctx) return gc::set_error("writer_diagnostics: null argument", GC_ERR_INVALID_ARGUMENT);
     gc_status st = w->ensure(w->diags, 16, "writer_diagnostics");
     if (st != GC_OK) return st;
@@ -420,17 +446,21 @@ gc_status gc_writer_diagnostics(gc_writer* w, gc_ctx* ctx, int step, double time
     j.slot = k;
     w->push(std::move(j));
     return GC_OK;
+    GC_ABI_END("gc_writer_diagnostics")
 }
 
 gc_status gc_writer_set_threads(gc_writer* w, int threads) {
+    GC_ABI_BEGIN
     if (!w || threads < 0 || threads > 256)
         return gc::set_error("writer_set_threads: threads must be in [0, 256]", GC_ERR_INVALID_ARGUMENT);
     w->drain();  // the worker reads the count per snapshot
     w->threads = threads ? threads : default_threads();
     return GC_OK;
+    GC_ABI_END("gc_writer_set_threads")
 }
 
 gc_status gc_writer_flush(gc_writer* w) {
+    GC_ABI_BEGIN
     if (!w) return gc::set_error("writer_flush: null argument", GC_ERR_INVALID_ARGUMENT);
     w->drain();
     if (std::fflush(w->diag) != 0)
@@ -438,9 +468,11 @@ gc_status gc_writer_flush(gc_writer* w) {
     std::lock_guard<std::mutex> lock(w->mu);
     if (!w->err.empty()) return gc::set_error(w->err, GC_ERR_IO);
     return GC_OK;
+    GC_ABI_END("gc_writer_flush")
 }
 
 gc_status gc_writer_close(gc_writer* w) {
+    GC_ABI_BEGIN
     if (!w) return GC_OK;
     w->drain();
     {
@@ -457,6 +489,7 @@ gc_status gc_writer_close(gc_writer* w) {
     w->diags.release();
     delete w;
     return st;
+    GC_ABI_END("gc_writer_close")
 }
 
 }  // extern "C"

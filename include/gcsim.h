@@ -108,7 +108,9 @@ int gc_device_count(void);
 int gc_solver_available(void);
 
 const char* gc_status_string(gc_status s);
-/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+/* Last error message of ctx (or of the calling thread when ctx is NULL).
+ * No C++ exception leaves an entry point: a failed host allocation returns
+ * GC_ERR_OUT_OF_MEMORY with its message in gc_last_error(NULL). */
 const char* gc_last_error(const gc_ctx* ctx);
 
 gc_status gc_create(gc_ctx** out, int device, const gc_params* p);
