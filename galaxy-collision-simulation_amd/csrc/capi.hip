@@ -546,6 +546,24 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
     if (nsteps < 0) return fail(c, GC_ERR_INVALID_ARGUMENT, "steps must be non-negative");
     if (c->s.n == 0) return GC_OK;
     const double half = 0.5 * dt;
+    if (c->params.solver == GC_SOLVER_DIRECT && c->params.precision == GC_PRECISION_MIXED &&
+        symmetric_pairs(c, 0, c->s.n) && nsteps > 1) {
+        // each step's merge also opens the next step and packs its sources
+        st = ensure_sym(c);
+        if (st != GC_OK) return st;
+        const double G = c->params.gravitational_constant;
+        const int ipt = sym_ipt(c);
+        launch_open_step(c->s, half, dt, c->stream);
+        launch_pack_mixed(c->s, c->mixed, G, c->stream);
+        for (int k = 0; k < nsteps; ++k) {
+            launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, ipt, c->stream);
+            if (k + 1 < nsteps)
+                launch_merge_sym_open(c->s, c->sym, c->mixed, G, half, dt, c->stream);
+            else
+                launch_merge_sym(c->s, c->sym, nullptr, true, half, c->stream);
+        }
+        return launch_status(c, "launch step kernels");
+    }
     for (int k = 0; k < nsteps; ++k) {
         launch_open_step(c->s, half, dt, c->stream);
         st = force(c, 0, c->s.n, true, half);

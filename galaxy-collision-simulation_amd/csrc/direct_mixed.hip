@@ -34,20 +34,6 @@ namespace gc {
 namespace {
 
 
-template <typename T>
-__device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 // One block per source tile: FP64 bounding-box centre, then FP32 offsets.
 __global__ __launch_bounds__(kTile) void pack_mixed_kernel(
     int n, const double* __restrict__ x, const double* __restrict__ y,

@@ -189,15 +189,14 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
 constexpr int kMergeGroups = 16;
 constexpr int kMergeLanes = kTile / 4;  // 64
 
-__global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
-    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
-    const long long* __restrict__ tile_pbase, const int* __restrict__ tile_valid,
-    const float* __restrict__ ipart, const float* __restrict__ jpart, double* __restrict__ ax,
-    double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
-    double* __restrict__ vy, double* __restrict__ vz, int close_kick, double half_dt) {
+// Row and column partials of target tile T summed per group into part
+// (group g adds its share of the row items, then its share of the column
+// pairs, in order).
+__device__ __forceinline__ void merge_sym_groups(
+    int T, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
+    const float* __restrict__ jpart, double (*part)[3][kTile]) {
 #pragma clang fp contract(off)
-    __shared__ double part[kMergeGroups][3][kTile];
-    const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
     const int lane = threadIdx.x % kMergeLanes;
     const int g = threadIdx.x / kMergeLanes;
     double a[3][4];
@@ -237,6 +236,18 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int k = 0; k < 4; ++k) part[g][c][4 * lane + k] = a[c][k];
+}
+
+__global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
+    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    const long long* __restrict__ tile_pbase, const int* __restrict__ tile_valid,
+    const float* __restrict__ ipart, const float* __restrict__ jpart, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
+    double* __restrict__ vy, double* __restrict__ vz, int close_kick, double half_dt) {
+#pragma clang fp contract(off)
+    __shared__ double part[kMergeGroups][3][kTile];
+    const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
+    merge_sym_groups(T, row_start, tile_t0, tile_pbase, ipart, jpart, part);
     __syncthreads();
     // thread (c, e) of the first 3 * kTile adds target e's component c
     const int t = threadIdx.x;
@@ -252,6 +263,79 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
     double* vel = c == 0 ? vx : (c == 1 ? vy : vz);
     acc[i] = sum;
     if (close_kick) vel[i] = vel[i] + sum * half_dt;
+}
+
+// The merge of one KDK step fused with the next step's opening: closing
+// kick, then the next opening kick and drift (open_step_kernel's
+// arithmetic), then tile T's source entries for the next force evaluation
+// (pack_mixed_kernel's: bounding-box centre, FP32 offsets, G m).  Every
+// operation and rounding is the separate kernels', so the trajectory is
+// unchanged; the state and the tiles are touched once instead of three
+// times.
+__global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_kernel(
+    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
+    const float* __restrict__ jpart, double* __restrict__ x, double* __restrict__ y,
+    double* __restrict__ z, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, const double* __restrict__ m, double half_dt, double dt, double G,
+    float4* __restrict__ jp, double* __restrict__ centre) {
+#pragma clang fp contract(off)
+    __shared__ double part[kMergeGroups][3][kTile];
+    __shared__ double box[2][3][4];   // per component: wave minima, maxima
+    __shared__ float offs[3][kTile];
+    const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
+    merge_sym_groups(T, row_start, tile_t0, tile_pbase, ipart, jpart, part);
+    __syncthreads();
+    const int t = threadIdx.x;
+    const bool active = t < 3 * kTile;  // thread (c, e): component c of target e
+    const int c = active ? t / kTile : 0, e = t % kTile;
+    const int valid = n - T * kTile;
+    const bool own = active && e < valid;
+    const int i = T * kTile + e;
+    double pos = 0.0;
+    if (own) {
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < kMergeGroups; ++k) sum += part[k][c][e];
+        double* acc = c == 0 ? ax : (c == 1 ? ay : az);
+        double* vel = c == 0 ? vx : (c == 1 ? vy : vz);
+        double* xc = c == 0 ? x : (c == 1 ? y : z);
+        acc[i] = sum;
+        const double closed = vel[i] + sum * half_dt;   // closing kick
+        const double opened = closed + sum * half_dt;   // next opening kick
+        vel[i] = opened;
+        pos = xc[i] + opened * dt;                       // drift
+        xc[i] = pos;
+    }
+    // tile bounding box per component (4 waves per component)
+    double lo = own ? pos : INFINITY, hi = own ? pos : -INFINITY;
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    const int w = (t >> 6) & 3;
+    if (active && (t & 63) == 0) {
+        box[0][c][w] = lo;
+        box[1][c][w] = hi;
+    }
+    __syncthreads();
+    if (active) {
+        double l = box[0][c][0], h = box[1][c][0];
+#pragma unroll
+        for (int k = 1; k < 4; ++k) {
+            l = fmin(l, box[0][c][k]);
+            h = fmax(h, box[1][c][k]);
+        }
+        const double mid = (l + h) * 0.5;
+        offs[c][e] = own ? static_cast<float>(pos - mid) : 0.0f;
+        if (e == 0) centre[c * ntiles + T] = mid;
+    }
+    __syncthreads();
+    if (t < kTile) {
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (t < valid) o = make_float4(offs[0][t], offs[1][t], offs[2][t],
+                                       static_cast<float>(G * m[i]));
+        jp[i] = o;
+    }
 }
 
 }  // namespace
@@ -354,6 +438,14 @@ void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile
                        static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
                        plan.tile_pbase, tile_valid, plan.ipart, plan.jpart, s.ax, s.ay, s.az, s.vx,
                        s.vy, s.vz, close_kick ? 1 : 0, half_dt);
+}
+
+void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,
+                           double half_dt, double dt, hipStream_t st) {
+    hipLaunchKernelGGL(merge_sym_open_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups),
+                       0, st, static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
+                       plan.tile_pbase, plan.ipart, plan.jpart, s.x, s.y, s.z, s.vx, s.vy, s.vz,
+                       s.ax, s.ay, s.az, s.m, half_dt, dt, G, L.jp, L.centre);
 }
 
 }  // namespace gc

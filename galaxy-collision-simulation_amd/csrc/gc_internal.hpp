@@ -106,6 +106,10 @@ void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const doubl
 // tile_valid: valid targets per tile, or null for the first s.n slots
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
                       bool close_kick, double half_dt, hipStream_t st);
+// merge with the closing kick, then the next step's opening kick and drift
+// and the source tiles of the next force evaluation (first s.n slots)
+void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,
+                           double half_dt, double dt, hipStream_t st);
 // integrate.hip
 void launch_open_step(const DeviceState& s, double half_dt, double dt, hipStream_t st);
 void launch_kick(const DeviceState& s, double dt, hipStream_t st);

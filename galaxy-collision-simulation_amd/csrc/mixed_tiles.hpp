@@ -9,6 +9,21 @@ namespace gc {
 
 constexpr float kTinySeparation2 = 1e-30f;
 
+// Wavefront minimum / maximum (butterfly over the 64 lanes).
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 __device__ __forceinline__ void two_sum_acc(float& hi, float& lo, float v) {
     const float s = hi + v;
     const float bb = s - hi;

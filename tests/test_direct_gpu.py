@@ -367,3 +367,25 @@ def test_cpp_interface(gpu):
     res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "hip_solver_test passed" in res.stdout and "no device" not in res.stdout
+
+
+@pytest.mark.parametrize("n", [65536, 1000, 257])
+def test_multi_step_fusion_is_bit_identical(gpu, n):
+    # gc_step over several steps fuses each merge with the next opening kick,
+    # drift and source packing; one step at a time runs them as separate
+    # kernels.  The trajectories must agree bit for bit.
+    pos, vel, m = gpu.generators.config1(n=65536)
+    pos, vel, m = pos[:n], vel[:n], m[:n]
+    out = []
+    for fused in (True, False):
+        with _engine(gpu, "mixed", 0.02) as eng:
+            eng.upload(pos, vel, m)
+            eng.compute_accelerations()
+            if fused:
+                eng.step(5e-4, 6)
+            else:
+                for _ in range(6):
+                    eng.step(5e-4, 1)
+            out.append(eng.download())
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
