@@ -623,6 +623,12 @@ def main() -> int:
         t0 = time.perf_counter()
         a_cpu = oracle.direct_targets(pos, mass, 1.0, eps, threads=threads)
         cpu_s = time.perf_counter() - t0
+        # the reference's own mode: one core, on the first 2048 targets
+        sample1 = min(n, 2048)
+        t0 = time.perf_counter()
+        oracle.direct_targets(pos, mass, 1.0, eps, 0, sample1, threads=1)
+        one_s = time.perf_counter() - t0
+        pairs1 = sample1 * (n - 1)
         out["cpu_baseline"] = {
             "value": round(pairs / cpu_s / 1e9, 4),
             "unit": "Gpair-interactions/s",
@@ -630,6 +636,9 @@ def main() -> int:
             "kind": "port",
             "sample": f"one full FP64 force evaluation at N={n} (targets form, "
                       f"{pairs:.3e} pairs) on {threads} OpenMP threads, {cpu_s:.2f} s",
+            "one_core": {"value": round(pairs1 / one_s / 1e9, 4), "cores": 1,
+                         "sample": f"targets [0, {sample1}) against all {n} sources, "
+                                   f"{pairs1:.3e} pairs, {one_s:.2f} s"},
         }
         if "accuracy_vs_fp64" in out:
             err = (np.linalg.norm(a_step0 - a_cpu, axis=1)
