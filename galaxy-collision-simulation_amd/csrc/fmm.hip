@@ -1027,6 +1027,7 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
 #pragma clang fp contract(off)
     __shared__ int stack_node[kTreeStack];
     __shared__ unsigned long long stack_mask[kTreeStack];
+    __shared__ NearSrc<F> leaf_src[kEvalLanes];
     const int leaf = blockIdx.x;  // the group's root node
     if (leaf >= nnodes || !group[leaf]) return;
     const int tb = nb[leaf], te = ne[leaf];
@@ -1061,26 +1062,42 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
             if (!(mass > 0.0)) continue;
             const int nc = nchild[node];
             if (nc == 0) {
-                // leaf: pair forces for the lanes still carrying it
+                // leaf: pair forces for the lanes still carrying it; the
+                // leaf's sources are converted once (one lane each) into
+                // LDS, 64 at a time, and read back by every lane
                 const int sb = nb[node], se = ne[node];
                 F px = 0, py = 0, pz = 0;
-                for (int j = sb; j < se; ++j) {
-                    const F qx = static_cast<F>(sx[j] - cx), qy = static_cast<F>(sy[j] - cy),
-                            qz = static_cast<F>(sz[j] - cz);
-                    const F gm = static_cast<F>(sm[j]);  // G is applied to the whole sum
-                    const F dx = qx - tx, dy = qy - ty, dz = qz - tz;
-                    const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
-                    F r;
-                    if (SAFE) {
-                        r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
-                    } else {
-                        r = gc_rsq(s2);
+                for (int j0 = sb; j0 < se; j0 += kEvalLanes) {
+                    const int cnt = min(kEvalLanes, se - j0);
+                    if (lane < cnt) {
+                        const int j = j0 + lane;
+                        NearSrc<F> q;
+                        q.x = static_cast<F>(sx[j] - cx);
+                        q.y = static_cast<F>(sy[j] - cy);
+                        q.z = static_cast<F>(sz[j] - cz);
+                        q.gm = static_cast<F>(sm[j]);  // G is applied to the whole sum
+                        leaf_src[lane] = q;
                     }
-                    r = (j == me || !mine) ? F(0) : r;
-                    const F kk = (gm * r) * (r * r);
-                    px = gc_fma(dx, kk, px);
-                    py = gc_fma(dy, kk, py);
-                    pz = gc_fma(dz, kk, pz);
+                    __syncthreads();
+                    if (mine) {
+                        for (int k = 0; k < cnt; ++k) {
+                            const NearSrc<F> q = leaf_src[k];
+                            const F dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+                            const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+                            F r;
+                            if (SAFE) {
+                                r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
+                            } else {
+                                r = gc_rsq(s2);
+                            }
+                            r = j0 + k == me ? F(0) : r;
+                            const F kk = (q.gm * r) * (r * r);
+                            px = gc_fma(dx, kk, px);
+                            py = gc_fma(dy, kk, py);
+                            pz = gc_fma(dz, kk, pz);
+                        }
+                    }
+                    __syncthreads();  // leaf_src is refilled next
                 }
                 a[0] += static_cast<double>(px);
                 a[1] += static_cast<double>(py);
