@@ -8,7 +8,10 @@
 //   mbar_k = -G sum_{j != k} f(s_kj) lambda_j . d_kj
 //
 // Both are one all-pairs sweep per target with 7 source values
-// (position, mass, lambda).  FP64 form: one target per lane, ascending j,
+// (position, mass, lambda).  The pair term is odd under i <-> j, so the
+// mixed VJP of a full-range state normally runs the pair-once form in
+// direct_sym.hip (vjp_sym_kernel); the per-target form here serves target
+// ranges, the per-target pair mode and states past the pair-once limit.  FP64 form: one target per lane, ascending j,
 // sources staged in LDS.  Mixed form: sources as FP32 offsets from FP64 tile
 // centres (the force kernel's packed tiles) plus FP32 lambda, FP32 pair
 // terms, per-tile FP32 partials folded into FP64, source ranges split over
@@ -200,6 +203,14 @@ void launch_vjp_fp64(const DeviceState& s, double G, double eps, const double* l
     hipLaunchKernelGGL(vjp_fp64_kernel, dim3(blocks), dim3(kTile), 0, st, static_cast<int>(s.n),
                        s.x, s.y, s.z, s.m, lam, lstride, G, eps * eps, xbar, mbar, ostride,
                        accumulate ? 1 : 0);
+}
+
+void launch_pack_lambda(size_t n, size_t padded, const double* lam, int lstride, float4* out,
+                        hipStream_t st) {
+    if (padded == 0) return;
+    hipLaunchKernelGGL(pack_lambda_kernel, dim3(static_cast<unsigned>((padded + 255) / 256)),
+                       dim3(256), 0, st, static_cast<int>(n), static_cast<int>(padded), lam,
+                       lstride, out);
 }
 
 void launch_vjp_mixed(const DeviceState& s, MixedLayout& L, float4* lpack, double* part,

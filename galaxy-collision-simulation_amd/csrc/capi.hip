@@ -34,6 +34,8 @@ struct gc_ctx {
     float4* vjp_lpack = nullptr;  // FP32 cotangents, cap entries
     double* vjp_part = nullptr;   // per-split VJP partials
     size_t vjp_lpack_n = 0, vjp_part_n = 0;
+    float* vjp_sym_part = nullptr;  // pair-once VJP: item partials, then pair partials
+    size_t vjp_sym_part_n = 0;
     std::string err;
 };
 
@@ -173,6 +175,9 @@ int sym_ipt(const gc_ctx* c) {
     if (2 * waves4 >= 8192) return 2;
     return 1;
 }
+
+// Targets per lane of the pair-once VJP unless set (gc_set_launch_shape).
+constexpr int kVjpSymIpt = 2;
 
 gc_status ensure_sym(gc_ctx* c) {
     std::string msg;
@@ -398,6 +403,7 @@ void gc_destroy(gc_ctx* c) {
     free_state(c);
     (void)hipFree(c->vjp_lpack);
     (void)hipFree(c->vjp_part);
+    (void)hipFree(c->vjp_sym_part);
     (void)hipFree(c->diag_out);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -814,6 +820,34 @@ gc_status gc_force_vjp(gc_ctx* c, const double* lambda, double* xbar, double* mb
         launch_vjp_fp64(c->s, c->params.gravitational_constant, c->params.softening, lambda, stride,
                         xbar, mbar, stride, accumulate != 0, c->stream);
         return launch_status(c, "launch FP64 force VJP");
+    }
+    if (symmetric_pairs(c, 0, n)) {
+        // each unordered pair once, on the force's work list
+        st = ensure_sym(c);
+        if (st != GC_OK) return st;
+        const size_t ifl = static_cast<size_t>(c->sym.nitems) * 8 * kTile;
+        const size_t jfl = c->sym.jpart_floats / 3 * 4;  // 4 sums per source, not 3
+        if (c->vjp_lpack_n < c->s.cap || c->vjp_sym_part_n < ifl + jfl) {
+            GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize before VJP scratch");
+            (void)hipFree(c->vjp_lpack);
+            (void)hipFree(c->vjp_sym_part);
+            c->vjp_lpack = nullptr;
+            c->vjp_sym_part = nullptr;
+            c->vjp_lpack_n = c->vjp_sym_part_n = 0;
+            GC_HIP(c, hipMalloc(&c->vjp_lpack, sizeof(float4) * c->s.cap), "allocate VJP scratch");
+            GC_HIP(c, hipMalloc(&c->vjp_sym_part, sizeof(float) * (ifl + jfl)),
+                   "allocate VJP scratch");
+            c->vjp_lpack_n = c->s.cap;
+            c->vjp_sym_part_n = ifl + jfl;
+        }
+        launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
+        launch_pack_lambda(n, c->mixed.ntiles * kTile, lambda, stride, c->vjp_lpack, c->stream);
+        const int ipt = (c->shape.ipt == 1 || c->shape.ipt == 2 || c->shape.ipt == 4) ? c->shape.ipt
+                                                                                        : kVjpSymIpt;
+        launch_vjp_sym(c->s, c->mixed, c->sym, c->vjp_lpack, c->vjp_sym_part,
+                       c->vjp_sym_part + ifl, c->params.gravitational_constant, c->params.softening,
+                       ipt, xbar, mbar, stride, accumulate != 0, c->stream);
+        return launch_status(c, "launch pair-once force VJP");
     }
     const int js = vjp_mixed_splits(n);
     const size_t need_part = static_cast<size_t>(js) * 4 * c->s.cap;

@@ -338,6 +338,251 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_ker
     }
 }
 
+// ---- force VJP, each unordered pair once ---------------------------------
+// With w = G m_i lambda_j - G m_j lambda_i and J(d) = r3 I - 3 r5 d d^T
+// (d = x_j - x_i, even in d), the pair gives xbar_i += J w and
+// xbar_j -= J w, and the mass terms mbar_i -= r3 lambda_j . d,
+// mbar_j += r3 lambda_i . d (G applied at the merge).  The schedule is the
+// force kernel's: items of row I, sources of tile J rotating through the
+// wave with their four running sums; the diagonal tile runs the one-sided
+// form with the self pair dropped by index.
+template <int IPT, bool SAFE>
+__global__ __launch_bounds__(kTile / IPT) void vjp_sym_kernel(
+    const SymItem* __restrict__ items, int nitems, int n, int ntiles,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const float4* __restrict__ jp, const float4* __restrict__ lp,
+    const double* __restrict__ centre, float eps2, float* __restrict__ ipart,
+    float* __restrict__ jpart) {
+    constexpr int kThreads = kTile / IPT;
+    constexpr int kWaves = kThreads / 64;
+    __shared__ float4 src2[kChunks][2 * kChunk];
+    __shared__ float4 lam2[kChunks][2 * kChunk];
+    __shared__ float red[kWaves > 1 ? kWaves - 1 : 1][4][kChunk];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const SymItem item = items[it];
+        const int I = item.I;
+        double xi[IPT], yi[IPT], zi[IPT];
+        float mi[IPT], li0[IPT], li1[IPT], li2[IPT];
+        float h[4][IPT], l[4][IPT];  // (hi, lo) of xbar x, y, z and mbar
+        const int valid = n - I * kTile;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int i = I * kTile + k * kThreads + t;
+            const bool active = k * kThreads + t < valid;
+            xi[k] = active ? x[i] : 0.0;
+            yi[k] = active ? y[i] : 0.0;
+            zi[k] = active ? z[i] : 0.0;
+            mi[k] = jp[i].w;  // G m_i; zero past n
+            const float4 ml = lp[i];  // zero past n
+            li0[k] = ml.x;
+            li1[k] = ml.y;
+            li2[k] = ml.z;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) h[c][k] = l[c][k] = 0.f;
+        }
+        for (int J = item.jb; J < item.je; ++J) {
+            float tx[IPT], ty[IPT], tz[IPT], p[4][IPT];
+            tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
+#pragma unroll
+            for (int k = 0; k < IPT; ++k) p[0][k] = p[1][k] = p[2][k] = p[3][k] = 0.f;
+            const float4* src = jp + static_cast<size_t>(J) * kTile;
+            const float4* lsrc = lp + static_cast<size_t>(J) * kTile;
+            if (J == I) {
+                // one-sided over the tile's own sources, self pair excluded
+#pragma unroll 4
+                for (int jj = 0; jj < kTile; ++jj) {
+                    const float4 q = src[jj];
+                    const float4 lj = lsrc[jj];
+#pragma unroll
+                    for (int k = 0; k < IPT; ++k) {
+                        const float dx = q.x - tx[k], dy = q.y - ty[k], dz = q.z - tz[k];
+                        const float s2 = __builtin_fmaf(
+                            dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+                        float r;
+                        if (SAFE) {
+                            r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
+                        } else {
+                            r = __builtin_amdgcn_rsqf(s2);
+                        }
+                        r = jj == k * kThreads + t ? 0.f : r;
+                        const float r2 = r * r, r3 = r2 * r;
+                        const float w0 = __builtin_fmaf(mi[k], lj.x, -q.w * li0[k]);
+                        const float w1 = __builtin_fmaf(mi[k], lj.y, -q.w * li1[k]);
+                        const float w2 = __builtin_fmaf(mi[k], lj.z, -q.w * li2[k]);
+                        const float dw = __builtin_fmaf(dz, w2, __builtin_fmaf(dy, w1, dx * w0));
+                        const float cc = 3.f * r3 * r2 * dw;
+                        p[0][k] = __builtin_fmaf(r3, w0, __builtin_fmaf(-cc, dx, p[0][k]));
+                        p[1][k] = __builtin_fmaf(r3, w1, __builtin_fmaf(-cc, dy, p[1][k]));
+                        p[2][k] = __builtin_fmaf(r3, w2, __builtin_fmaf(-cc, dz, p[2][k]));
+                        const float ldj = __builtin_fmaf(lj.z, dz, __builtin_fmaf(lj.y, dy, lj.x * dx));
+                        p[3][k] = __builtin_fmaf(-r3, ldj, p[3][k]);
+                    }
+                }
+            } else {
+                for (int e = t; e < kTile; e += kThreads) {
+                    const float4 v = src[e], w = lsrc[e];
+                    src2[e / kChunk][e % kChunk] = v;
+                    src2[e / kChunk][kChunk + e % kChunk] = v;
+                    lam2[e / kChunk][e % kChunk] = w;
+                    lam2[e / kChunk][kChunk + e % kChunk] = w;
+                }
+                __syncthreads();
+                for (int c = 0; c < kChunks; ++c) {
+                    // lane l holds the running sums of source c*64 + (l + s) % 64
+                    float b0 = 0.f, b1 = 0.f, b2 = 0.f, bm = 0.f;
+                    const float4* row = &src2[c][lane];
+                    const float4* lrow = &lam2[c][lane];
+#pragma unroll 4
+                    for (int s = 0; s < kChunk; ++s) {
+                        const float4 q = row[s];
+                        const float4 lj = lrow[s];
+#pragma unroll
+                        for (int k = 0; k < IPT; ++k) {
+                            const float dx = q.x - tx[k], dy = q.y - ty[k], dz = q.z - tz[k];
+                            const float s2 = __builtin_fmaf(
+                                dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+                            float r;
+                            if (SAFE) {
+                                r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
+                            } else {
+                                r = __builtin_amdgcn_rsqf(s2);
+                            }
+                            const float r2 = r * r, r3 = r2 * r;
+                            const float w0 = __builtin_fmaf(mi[k], lj.x, -q.w * li0[k]);
+                            const float w1 = __builtin_fmaf(mi[k], lj.y, -q.w * li1[k]);
+                            const float w2 = __builtin_fmaf(mi[k], lj.z, -q.w * li2[k]);
+                            const float dw = __builtin_fmaf(dz, w2, __builtin_fmaf(dy, w1, dx * w0));
+                            const float cc = 3.f * r3 * r2 * dw;
+                            p[0][k] = __builtin_fmaf(r3, w0, __builtin_fmaf(-cc, dx, p[0][k]));
+                            p[1][k] = __builtin_fmaf(r3, w1, __builtin_fmaf(-cc, dy, p[1][k]));
+                            p[2][k] = __builtin_fmaf(r3, w2, __builtin_fmaf(-cc, dz, p[2][k]));
+                            b0 = __builtin_fmaf(-r3, w0, __builtin_fmaf(cc, dx, b0));
+                            b1 = __builtin_fmaf(-r3, w1, __builtin_fmaf(cc, dy, b1));
+                            b2 = __builtin_fmaf(-r3, w2, __builtin_fmaf(cc, dz, b2));
+                            const float ldj = __builtin_fmaf(lj.z, dz, __builtin_fmaf(lj.y, dy, lj.x * dx));
+                            const float ldi = __builtin_fmaf(li2[k], dz, __builtin_fmaf(li1[k], dy, li0[k] * dx));
+                            p[3][k] = __builtin_fmaf(-r3, ldj, p[3][k]);
+                            bm = __builtin_fmaf(r3, ldi, bm);
+                        }
+                        b0 = wave_rol1(b0);
+                        b1 = wave_rol1(b1);
+                        b2 = wave_rol1(b2);
+                        bm = wave_rol1(bm);
+                    }
+                    float* out = jpart + (item.pbase + pair_slot(I - item.t0, J - item.t0)) * 4 * kTile + c * kChunk + lane;
+                    if (kWaves > 1) {
+                        if (wave > 0) {
+                            red[wave - 1][0][lane] = b0;
+                            red[wave - 1][1][lane] = b1;
+                            red[wave - 1][2][lane] = b2;
+                            red[wave - 1][3][lane] = bm;
+                        }
+                        __syncthreads();
+                        if (wave == 0) {
+#pragma unroll
+                            for (int w = 0; w < kWaves - 1; ++w) {
+                                b0 += red[w][0][lane];
+                                b1 += red[w][1][lane];
+                                b2 += red[w][2][lane];
+                                bm += red[w][3][lane];
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (wave == 0) {
+                        out[0] = b0;
+                        out[kTile] = b1;
+                        out[2 * kTile] = b2;
+                        out[3 * kTile] = bm;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < IPT; ++k)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) two_sum_acc(h[c][k], l[c][k], p[c][k]);
+        }
+        float* base = ipart + static_cast<size_t>(item.islot) * 8 * kTile;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            const int e = k * kThreads + t;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                base[c * kTile + e] = h[c][k];
+                base[(4 + c) * kTile + e] = l[c][k];
+            }
+        }
+        __syncthreads();  // LDS reuse by the next item
+    }
+}
+
+// xbar_T, mbar_T = the row's item partials (hi + lo, J order) + the
+// column's pair partials (I order), in FP64, by 8 groups of 64 lanes (4
+// targets per lane) whose sums are added in group order.
+constexpr int kVjpMergeGroups = 8;
+
+__global__ __launch_bounds__(kMergeLanes * kVjpMergeGroups) void vjp_sym_merge_kernel(
+    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
+    const float* __restrict__ jpart, double G, double* __restrict__ xbar,
+    double* __restrict__ mbar, int ostride, int accumulate) {
+#pragma clang fp contract(off)
+    __shared__ double part[kVjpMergeGroups][4][kTile];
+    const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
+    const int lane = threadIdx.x % kMergeLanes;
+    const int g = threadIdx.x / kMergeLanes;
+    double a[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c][0] = a[c][1] = a[c][2] = a[c][3] = 0.0;
+    const int r0 = row_start[T], nr = row_start[T + 1] - r0;
+    const int s1 = r0 + (g + 1) * nr / kVjpMergeGroups;
+    for (int s = r0 + g * nr / kVjpMergeGroups; s < s1; ++s) {
+        const float* b = ipart + static_cast<size_t>(s) * 8 * kTile + 4 * lane;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float4 hi = *reinterpret_cast<const float4*>(b + c * kTile);
+            const float4 lo = *reinterpret_cast<const float4*>(b + (c + 4) * kTile);
+            a[c][0] += static_cast<double>(hi.x) + static_cast<double>(lo.x);
+            a[c][1] += static_cast<double>(hi.y) + static_cast<double>(lo.y);
+            a[c][2] += static_cast<double>(hi.z) + static_cast<double>(lo.z);
+            a[c][3] += static_cast<double>(hi.w) + static_cast<double>(lo.w);
+        }
+    }
+    const int t0 = tile_t0[T], cols = T - t0;
+    const float* col = jpart + (tile_pbase[T] + pair_slot(0, cols)) * 4 * kTile + 4 * lane;
+    const int i1 = (g + 1) * cols / kVjpMergeGroups;
+    for (int I = g * cols / kVjpMergeGroups; I < i1; ++I) {
+        const float* b = col + static_cast<size_t>(I) * 4 * kTile;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float4 v = *reinterpret_cast<const float4*>(b + c * kTile);
+            a[c][0] += static_cast<double>(v.x);
+            a[c][1] += static_cast<double>(v.y);
+            a[c][2] += static_cast<double>(v.z);
+            a[c][3] += static_cast<double>(v.w);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[g][c][4 * lane + k] = a[c][k];
+    __syncthreads();
+    const int valid = n - T * kTile;
+    for (int q = threadIdx.x; q < 4 * kTile; q += kMergeLanes * kVjpMergeGroups) {
+        const int c = q / kTile, e = q % kTile;
+        if (e >= valid) continue;
+        double sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < kVjpMergeGroups; ++k) sum += part[k][c][e];
+        const int i = T * kTile + e;
+        double* out = c < 3 ? xbar + static_cast<size_t>(c) * ostride + i : mbar + i;
+        const double v = c < 3 ? sum : G * sum;
+        *out = accumulate ? *out + v : v;
+    }
+}
+
 }  // namespace
 
 int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns, int len,
@@ -438,6 +683,34 @@ void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile
                        static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
                        plan.tile_pbase, tile_valid, plan.ipart, plan.jpart, s.ax, s.ay, s.az, s.vx,
                        s.vy, s.vz, close_kick ? 1 : 0, half_dt);
+}
+
+void launch_vjp_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan,
+                    const float4* lpack, float* vipart, float* vjpart, double G, double eps,
+                    int ipt, double* xbar, double* mbar, int ostride, bool accumulate,
+                    hipStream_t st) {
+    if (s.n == 0 || plan.nitems == 0) return;
+    const float eps2 = static_cast<float>(eps * eps);
+    const bool safe = !(eps2 >= kTinySeparation2);
+    const int n = static_cast<int>(s.n), nt = plan.ntiles;
+#define GC_VJP_SYM(IPT)                                                                          \
+    do {                                                                                         \
+        if (safe)                                                                                \
+            hipLaunchKernelGGL((vjp_sym_kernel<IPT, true>), dim3(plan.nitems), dim3(kTile / IPT), \
+                               0, st, plan.items, plan.nitems, n, nt, s.x, s.y, s.z, L.jp, lpack, \
+                               L.centre, eps2, vipart, vjpart);                                  \
+        else                                                                                     \
+            hipLaunchKernelGGL((vjp_sym_kernel<IPT, false>), dim3(plan.nitems),                   \
+                               dim3(kTile / IPT), 0, st, plan.items, plan.nitems, n, nt, s.x,    \
+                               s.y, s.z, L.jp, lpack, L.centre, eps2, vipart, vjpart);           \
+    } while (0)
+    if (ipt == 4) GC_VJP_SYM(4);
+    else if (ipt == 1) GC_VJP_SYM(1);
+    else GC_VJP_SYM(2);
+#undef GC_VJP_SYM
+    hipLaunchKernelGGL(vjp_sym_merge_kernel, dim3(nt), dim3(kMergeLanes * kVjpMergeGroups), 0, st,
+                       n, nt, plan.row_start, plan.tile_t0, plan.tile_pbase, vipart, vjpart, G,
+                       xbar, mbar, ostride, accumulate ? 1 : 0);
 }
 
 void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,

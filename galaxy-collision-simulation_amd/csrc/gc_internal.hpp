@@ -128,6 +128,17 @@ void launch_vjp_mixed(const DeviceState& s, MixedLayout& L, float4* lpack, doubl
                       int jsplit, double G, double eps, const double* lam, int lstride,
                       double* xbar, double* mbar, int ostride, bool accumulate, hipStream_t st);
 int vjp_mixed_splits(size_t n);
+// FP32 cotangents (lam SoA, stride lstride) packed as float4 over `padded`
+// slots, zero past n
+void launch_pack_lambda(size_t n, size_t padded, const double* lam, int lstride, float4* out,
+                        hipStream_t st);
+// direct_sym.hip: the VJP with each unordered pair once over plan's items
+// (L and lpack packed for the current state); vipart: nitems * 8 * kTile
+// floats, vjpart: 4 * kTile floats per tile pair; ipt 1, 2 or 4
+void launch_vjp_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan,
+                    const float4* lpack, float* vipart, float* vjpart, double G, double eps,
+                    int ipt, double* xbar, double* mbar, int ostride, bool accumulate,
+                    hipStream_t st);
 // diagnostics.hip: out[13] in gc_diagnostics field order (KE, PE, E, M, P3, C3, L3)
 void launch_diagnostics(const DeviceState& s, double G, double eps, double* scratch,
                         size_t scratch_doubles, double* out_dev, hipStream_t st);

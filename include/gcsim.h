@@ -175,7 +175,10 @@ gc_status gc_step_reverse(gc_ctx* ctx, double dt, int nsteps);
 /* Vector-Jacobian product of the direct force at the resident positions and
  * masses: given lambda = dL/da ([3][n]), writes (accumulate == 0) or adds
  * xbar = (da/dx)^T lambda ([3][n]) and mbar = (da/dm)^T lambda ([n]).
- * FP64 or mixed per the context's precision; solver must be direct. */
+ * FP64 or mixed per the context's precision; solver must be direct.  The
+ * mixed form evaluates each unordered pair once for full-range states up to
+ * 262,144 particles (the pair form gc_set_pairs selects; the per-target
+ * form otherwise), with targets per lane from gc_set_launch_shape. */
 gc_status gc_force_vjp(gc_ctx* ctx, const double* lambda, double* xbar, double* mbar,
                        int accumulate);
 

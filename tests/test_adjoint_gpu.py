@@ -130,3 +130,48 @@ def test_trajectory_gradient_against_finite_differences(gpu):
         ad = (va.grad * du).sum().item() + (ma.grad * dm).sum().item()
         assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9, (k, fd, ad)
     eng.close()
+
+
+def _vjp_mixed_form(gpu, x, m, lam, eps, pairs, ipt=0):
+    n = x.shape[0]
+    with gpu.Engine("hip-direct", "mixed", softening=eps) as eng:
+        eng.set_stream(torch.cuda.current_stream().cuda_stream)
+        eng.set_pairs(pairs)
+        if ipt:
+            eng.set_launch_shape(ipt, 0)
+        eng.set_state_device(n, x.t().contiguous().data_ptr(), None, m.data_ptr())
+        lam_s = lam.t().contiguous()
+        out = []
+        for _ in range(2):
+            xbar = torch.zeros(3, n, dtype=torch.float64, device=x.device)
+            mbar = torch.zeros(n, dtype=torch.float64, device=x.device)
+            eng.force_vjp(lam_s.data_ptr(), xbar.data_ptr(), mbar.data_ptr())
+            torch.cuda.synchronize()
+            out.append((xbar.t().clone(), mbar.clone()))
+    return out
+
+
+@pytest.mark.parametrize("n", [65536, 4097, 300])
+def test_pair_once_vjp_matches_per_target_vjp(gpu, n):
+    """The pair-once VJP (the default for full-range mixed states) against
+    the per-target one and the FP64 VJP: same 1e-5 bound, deterministic,
+    every targets-per-lane shape agreeing."""
+    pos, _, mass = gpu.generators.config1(n=65536)
+    idx = np.random.default_rng(n).choice(65536, n, replace=False)
+    x = torch.tensor(pos[idx], dtype=torch.float64, device=_dev())
+    m = torch.tensor(mass[idx], dtype=torch.float64, device=_dev())
+    g = torch.Generator().manual_seed(n)
+    lam = torch.randn(n, 3, generator=g, dtype=torch.float64).to(_dev())
+    eps = 0.02
+    ref_x, ref_m = _vjp_engine(gpu, "fp64", x, m, lam, eps)
+    sx, sm = ref_x.abs().max().item(), ref_m.abs().max().item()
+    (sym_x, sym_m), (sym_x2, sym_m2) = _vjp_mixed_form(gpu, x, m, lam, eps, "symmetric")
+    assert torch.equal(sym_x, sym_x2) and torch.equal(sym_m, sym_m2)  # deterministic
+    (tgt_x, tgt_m), _ = _vjp_mixed_form(gpu, x, m, lam, eps, "targets")
+    for gx, gm in ((sym_x, sym_m), (tgt_x, tgt_m)):
+        assert (gx - ref_x).abs().max().item() < 1e-5 * sx
+        assert (gm - ref_m).abs().max().item() < 1e-5 * sm
+    for ipt in (1, 4):
+        (ix, im), _ = _vjp_mixed_form(gpu, x, m, lam, eps, "symmetric", ipt)
+        assert (ix - ref_x).abs().max().item() < 1e-5 * sx
+        assert (im - ref_m).abs().max().item() < 1e-5 * sm
