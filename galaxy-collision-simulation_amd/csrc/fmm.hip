@@ -741,6 +741,35 @@ struct alignas(4 * sizeof(F)) NearSrc {
     F x, y, z, gm;
 };
 
+// Pair sums of one staged round for TPL targets per lane: group g takes
+// sources g, g + groups, ...; CHECK drops the self pair (source cb + jj is
+// the target's own particle when it equals selfpos).
+template <typename F, bool SAFE, int TPL, bool CHECK>
+__device__ __forceinline__ void near_round(const NearSrc<F>* tile, int g, int cnt, int groups,
+                                           int cb, F eps2, const F* tx, const F* ty, const F* tz,
+                                           const int* selfpos, F* px, F* py, F* pz) {
+#pragma clang fp contract(off)
+    for (int jj = g; jj < cnt; jj += groups) {
+        const NearSrc<F> sj = tile[jj];
+#pragma unroll
+        for (int q = 0; q < TPL; ++q) {
+            const F dx = sj.x - tx[q], dy = sj.y - ty[q], dz = sj.z - tz[q];
+            const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+            F r;
+            if (SAFE) {
+                r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
+            } else {
+                r = gc_rsq(s2);
+            }
+            if (CHECK) r = cb + jj == selfpos[q] ? F(0) : r;
+            const F kk = (sj.gm * r) * (r * r);
+            px[q] = gc_fma(dx, kk, px[q]);
+            py[q] = gc_fma(dy, kk, py[q]);
+            pz[q] = gc_fma(dz, kk, pz[q]);
+        }
+    }
+}
+
 // One leaf chunk of c targets of a near item at TPL targets per lane (lane
 // (g, l) holds targets l, l + h, .., h = ceil(c / TPL)); group g takes
 // sources g, g + groups, .. of each staged round.  The self pair
@@ -804,6 +833,8 @@ __device__ __forceinline__ void near_leaf_chunk(
         // the target leaf among this piece's near leaves, if it is there:
         // target ti is then source qpre[qself] + (base - tb) + ti
         const unsigned long long own = __ballot(lane < kq && qs[lane] == tb);
+        // the chunk's targets are sources [self_lo, self_lo + c) of the piece
+        const int self_lo = own ? qpre[__ffsll(static_cast<long long>(own)) - 1] + (base - tb) : -1;
         int selfpos[TPL];
 #pragma unroll
         for (int k = 0; k < TPL; ++k) {
@@ -850,25 +881,14 @@ __device__ __forceinline__ void near_leaf_chunk(
             F px[TPL], py[TPL], pz[TPL];
 #pragma unroll
             for (int q = 0; q < TPL; ++q) px[q] = py[q] = pz[q] = 0;
-            for (int jj = g; jj < cnt; jj += groups) {
-                const NearSrc<F> sj = tile[jj];
-#pragma unroll
-                for (int q = 0; q < TPL; ++q) {
-                    const F dx = sj.x - tx[q], dy = sj.y - ty[q], dz = sj.z - tz[q];
-                    const F s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
-                    F r;
-                    if (SAFE) {
-                        r = s2 > F(1e-30) ? gc_rsq(s2) : F(0);
-                    } else {
-                        r = gc_rsq(s2);
-                    }
-                    r = cb + jj == selfpos[q] ? F(0) : r;
-                    const F kk = (sj.gm * r) * (r * r);
-                    px[q] = gc_fma(dx, kk, px[q]);
-                    py[q] = gc_fma(dy, kk, py[q]);
-                    pz[q] = gc_fma(dz, kk, pz[q]);
-                }
-            }
+            // only a round holding the target leaf's own sources can meet a
+            // self pair (wave-uniform test); the others skip the check
+            if (own && cb < self_lo + c && self_lo < cb + cnt)
+                near_round<F, SAFE, TPL, true>(tile, g, cnt, groups, cb, eps2, tx, ty, tz, selfpos,
+                                               px, py, pz);
+            else
+                near_round<F, SAFE, TPL, false>(tile, g, cnt, groups, cb, eps2, tx, ty, tz, selfpos,
+                                                px, py, pz);
 #pragma unroll
             for (int q = 0; q < TPL; ++q) {
                 near_a[q][0] += static_cast<double>(px[q]);
