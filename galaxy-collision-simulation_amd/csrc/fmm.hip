@@ -103,18 +103,26 @@ __global__ __launch_bounds__(256) void bbox_partial_kernel(int n, const double* 
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Root cube of fmm_solver.cpp:76-85; node 0 of the node arrays.
-__global__ void bbox_final_kernel(int nblocks, const double* __restrict__ part, double eps, int n,
-                                  double* __restrict__ geom, int* __restrict__ nb,
-                                  int* __restrict__ ne) {
+// Root cube of fmm_solver.cpp:76-85; node 0 of the node arrays.  One wave:
+// lanes take the block partials in strides, then wavefront min / max (exact,
+// so the order does not matter).
+__global__ __launch_bounds__(64) void bbox_final_kernel(int nblocks, const double* __restrict__ part,
+                                                        double eps, int n, double* __restrict__ geom,
+                                                        int* __restrict__ nb, int* __restrict__ ne) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nblocks; ++b)
+    for (int b = threadIdx.x; b < nblocks; b += 64)
         for (int d = 0; d < 3; ++d) {
             lo[d] = fmin(lo[d], part[b * 6 + d]);
             hi[d] = fmax(hi[d], part[b * 6 + 3 + d]);
         }
+    for (int d = 0; d < 3; ++d) {
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    }
+    if (threadIdx.x != 0) return;
     double ext = hi[0] - lo[0];
     if (ext < hi[1] - lo[1]) ext = hi[1] - lo[1];
     if (ext < hi[2] - lo[2]) ext = hi[2] - lo[2];
