@@ -509,6 +509,9 @@ __global__ __launch_bounds__(256) void list_copy_kernel(
 
 // per-wave list totals of lists_kernel's leaf-to-wave assignment (li mod
 // lwaves), to size the stash regions
+// Persistent waves of the list traversals (and slots of the per-wave totals).
+constexpr int kListWaves = 4096;
+
 __global__ __launch_bounds__(256) void wave_totals_kernel(int nleaves, int lwaves,
                                                           const int* __restrict__ nfar,
                                                           const int* __restrict__ nnear,
@@ -1556,7 +1559,7 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     // regions, the lists are counted and stashed in one traversal and moved
     // to their places after the scan; a wave whose lists outgrow its region
     // falls back to the writing traversal.
-    const int lwaves = std::min(nleaves, 4096);
+    const int lwaves = std::min(nleaves, kListWaves);
     const bool stash = wcap_far > 0 && wcap_near > 0;
     if (stash) {
         const size_t fneed = static_cast<size_t>(lwaves) * wcap_far;
@@ -1659,7 +1662,7 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
                            overflow);
         // size the stash regions for the next builds from this build's
         // per-wave totals, while they stay within 8x the lists' size
-        if (!wtot) FMM_HIP(hipMalloc(&wtot, sizeof(long long) * 2 * 4096), "allocate wave totals");
+        if (!wtot) FMM_HIP(hipMalloc(&wtot, sizeof(long long) * 2 * kListWaves), "allocate wave totals");
         hipLaunchKernelGGL(wave_totals_kernel, dim3((lwaves + 255) / 256), dim3(256), 0, st, nleaves,
                            lwaves, nfar, nnear, wtot);
         std::vector<long long> wt(2 * static_cast<size_t>(lwaves));
