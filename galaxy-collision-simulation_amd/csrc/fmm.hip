@@ -1236,12 +1236,19 @@ __global__ __launch_bounds__(256) void stats_kernel(
     const int* __restrict__ ne, const int* __restrict__ nfar, const int* __restrict__ nnear,
     const int* __restrict__ nsrc, unsigned long long* __restrict__ out) {
     const int li = blockIdx.x * 256 + threadIdx.x;
-    if (li >= nleaves) return;
-    const long long cnt = ne[leaves[li]] - nb[leaves[li]];
-    atomicAdd(&out[0], static_cast<unsigned long long>(nfar[li]));
-    atomicAdd(&out[1], static_cast<unsigned long long>(nnear[li]));
-    atomicAdd(&out[2], static_cast<unsigned long long>(cnt * nfar[li]));
-    atomicAdd(&out[3], static_cast<unsigned long long>(cnt * nsrc[li] - cnt));
+    unsigned long long v[4] = {0ull, 0ull, 0ull, 0ull};
+    if (li < nleaves) {
+        const long long cnt = ne[leaves[li]] - nb[leaves[li]];
+        v[0] = static_cast<unsigned long long>(nfar[li]);
+        v[1] = static_cast<unsigned long long>(nnear[li]);
+        v[2] = static_cast<unsigned long long>(cnt * nfar[li]);
+        v[3] = static_cast<unsigned long long>(cnt * nsrc[li] - cnt);
+    }
+    // one atomic per wave and counter instead of one per leaf
+    for (int k = 0; k < 4; ++k) {
+        for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+        if ((threadIdx.x & 63) == 0 && v[k]) atomicAdd(&out[k], v[k]);
+    }
 }
 
 struct ToI64 {
