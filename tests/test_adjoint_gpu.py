@@ -175,3 +175,31 @@ def test_pair_once_vjp_matches_per_target_vjp(gpu, n):
         (ix, im), _ = _vjp_mixed_form(gpu, x, m, lam, eps, "symmetric", ipt)
         assert (ix - ref_x).abs().max().item() < 1e-5 * sx
         assert (im - ref_m).abs().max().item() < 1e-5 * sm
+
+
+def test_mixed_vjp_forms_without_softening(gpu):
+    """eps = 0 runs the guarded kernels (a pair closer than 1e-15 adds
+    nothing); both mixed forms against the FP64 VJP, and a coincident pair
+    contributes exactly nothing."""
+    n = 700
+    g = torch.Generator().manual_seed(71)
+    x = torch.randn(n, 3, generator=g, dtype=torch.float64).to(_dev())
+    m = (torch.rand(n, generator=g, dtype=torch.float64) + 0.5).to(_dev())
+    lam = torch.randn(n, 3, generator=g, dtype=torch.float64).to(_dev())
+    ref_x, ref_m = _vjp_engine(gpu, "fp64", x, m, lam, 0.0)
+    sx, sm = ref_x.abs().max().item(), ref_m.abs().max().item()
+    for pairs in ("symmetric", "targets"):
+        (gx, gm), _ = _vjp_mixed_form(gpu, x, m, lam, 0.0, pairs)
+        assert torch.isfinite(gx).all() and torch.isfinite(gm).all()
+        assert (gx - ref_x).abs().max().item() < 1e-5 * sx, pairs
+        assert (gm - ref_m).abs().max().item() < 1e-5 * sm, pairs
+    # two particles at one point and one apart: only the distant pairs count
+    x2 = torch.tensor([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [1.0, 0.5, -0.25]],
+                      dtype=torch.float64, device=_dev())
+    m2 = torch.tensor([1.0, 2.0, 0.5], dtype=torch.float64, device=_dev())
+    l2 = torch.tensor([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]],
+                      dtype=torch.float64, device=_dev())
+    r64 = _vjp_engine(gpu, "fp64", x2, m2, l2, 0.0)
+    (sx2, sm2), _ = _vjp_mixed_form(gpu, x2, m2, l2, 0.0, "symmetric")
+    assert torch.allclose(sx2, r64[0], rtol=1e-5, atol=1e-6)
+    assert torch.allclose(sm2, r64[1], rtol=1e-5, atol=1e-6)
