@@ -1048,7 +1048,7 @@ __device__ __forceinline__ bool tree_accepts(double dx, double dy, double dz, do
 
 template <typename F, int ORDER, bool SAFE>
 __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
-    int nnodes, const int* __restrict__ group, const double* __restrict__ geom,
+    int n, const double* __restrict__ geom,
     const double* __restrict__ mcom, const F* __restrict__ kcoef, const int* __restrict__ nb,
     const int* __restrict__ ne, const int* __restrict__ first_child,
     const int* __restrict__ nchild, const int* __restrict__ sidx, const double* __restrict__ sx, const double* __restrict__ sy,
@@ -1061,15 +1061,30 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
     __shared__ int stack_node[kTreeStack];
     __shared__ unsigned long long stack_mask[kTreeStack];
     __shared__ NearSrc<F> leaf_src[kEvalLanes];
-    const int leaf = blockIdx.x;  // the group's root node
-    if (leaf >= nnodes || !group[leaf]) return;
-    const int tb = nb[leaf], te = ne[leaf];
-    if (te <= tb) return;
+    // the wave's targets: 64 consecutive particles in key order
+    const int tb = static_cast<int>(blockIdx.x) * kEvalLanes;
+    if (tb >= n) return;
+    const int te = min(n, tb + kEvalLanes);
     const int lane = threadIdx.x;
     const F eps2 = static_cast<F>(eps * eps);
-    // offsets are taken from the target leaf's centre, so FP32 differences
-    // stay small for nearby cells and sources
-    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    // offsets are taken from the centre of the targets' bounding box, so
+    // FP32 differences stay small for nearby cells and sources
+    double cx, cy, cz;
+    {
+        const int q = tb + min(lane, te - tb - 1);
+        double lx = sx[q], ly = sy[q], lz = sz[q], hx = lx, hy = ly, hz = lz;
+        for (int o = 32; o > 0; o >>= 1) {
+            lx = fmin(lx, __shfl_xor(lx, o, 64));
+            ly = fmin(ly, __shfl_xor(ly, o, 64));
+            lz = fmin(lz, __shfl_xor(lz, o, 64));
+            hx = fmax(hx, __shfl_xor(hx, o, 64));
+            hy = fmax(hy, __shfl_xor(hy, o, 64));
+            hz = fmax(hz, __shfl_xor(hz, o, 64));
+        }
+        cx = (lx + hx) * 0.5;
+        cy = (ly + hy) * 0.5;
+        cz = (lz + hz) * 0.5;
+    }
     unsigned long long n_m2p = 0, n_p2p = 0;
     for (int base = tb; base < te; base += kEvalLanes) {
         const int c = min(kEvalLanes, te - base);
@@ -1198,27 +1213,15 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
     }
 }
 
-// Walk groups: the root if it holds <= 64 particles, else every child of a
-// node holding more than 64 that holds at most 64 (empty children are not
-// materialised).  Also counts the leaves holding particles.
-__global__ __launch_bounds__(256) void tree_group_kernel(int nnodes, const int* __restrict__ nb,
-                                                         const int* __restrict__ ne,
-                                                         const int* __restrict__ first_child,
-                                                         const int* __restrict__ nchild,
-                                                         int* __restrict__ group,
-                                                         unsigned long long* __restrict__ counters) {
+// Counts the leaves holding particles (the tree's leaf statistic).
+__global__ __launch_bounds__(256) void tree_leaf_count_kernel(int nnodes, const int* __restrict__ nb,
+                                                              const int* __restrict__ ne,
+                                                              const int* __restrict__ nchild,
+                                                              unsigned long long* __restrict__ counters) {
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= nnodes) return;
-    const int cnt = ne[k] - nb[k];
-    if (k == 0) group[0] = cnt <= kEvalLanes ? 1 : 0;
-    const int nc = nchild[k];
-    if (nc == 0) {
-        if (cnt > 0) atomicAdd(counters + 2, 1ull);
-        return;
-    }
-    const int fc = first_child[k];
-    for (int c = fc; c < fc + nc; ++c)
-        group[c] = (cnt > kEvalLanes && ne[c] - nb[c] <= kEvalLanes) ? 1 : 0;
+    const bool leaf = k < nnodes && nchild[k] == 0 && ne[k] > nb[k];
+    const unsigned long long m = __ballot(leaf);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(counters + 2, static_cast<unsigned long long>(__popcll(m)));
 }
 
 __global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, double* __restrict__ ax,
@@ -1780,13 +1783,13 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     const double G = p.gravitational_constant, eps = p.softening;
     if (p.solver == GC_SOLVER_TREE) {
         FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear counts");
-        FMM_HIP(grow_leaf_arrays(static_cast<size_t>(nnodes)), "allocate walk groups");
-        hipLaunchKernelGGL(tree_group_kernel, dim3((nnodes + 255) / 256), dim3(256), 0, st, nnodes,
-                           nb, ne, first_child, nchild, flag, stats_dev);
+        hipLaunchKernelGGL(tree_leaf_count_kernel, dim3((nnodes + 255) / 256), dim3(256), 0, st, nnodes,
+                           nb, ne, nchild, stats_dev);
         const double theta = p.theta;
+        const int n = static_cast<int>(s.n);
 #define GC_TREE(F, O, S, KC)                                                                     \
-    hipLaunchKernelGGL((tree_eval_kernel<F, O, S>), dim3(static_cast<unsigned>(nnodes)), block, 0,  \
-                       st, nnodes, flag, geom, mcom, KC, nb, ne, first_child, nchild, idx_sorted, sxs,    \
+    hipLaunchKernelGGL((tree_eval_kernel<F, O, S>), dim3(static_cast<unsigned>((n + kEvalLanes - 1) / kEvalLanes)), \
+                       block, 0, st, n, geom, mcom, KC, nb, ne, first_child, nchild, idx_sorted, sxs,    \
                        sys, szs, sms, G, eps, theta, b, e, s.ax, s.ay, s.az, s.vx, s.vy, s.vz,     \
                        close_kick ? 1 : 0, half_dt, stats_dev)
 #define GC_TREE_O(F, S, KC)                                                                      \
@@ -1795,12 +1798,14 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
         else if (order == 2) GC_TREE(F, 2, S, KC);                                               \
         else GC_TREE(F, 4, S, KC);                                                               \
     } while (0)
-        if (mixed) {
-            if (safe) GC_TREE_O(float, true, kc32);
-            else GC_TREE_O(float, false, kc32);
-        } else {
-            if (safe) GC_TREE_O(double, true, kc64);
-            else GC_TREE_O(double, false, kc64);
+        if (n > 0) {
+            if (mixed) {
+                if (safe) GC_TREE_O(float, true, kc32);
+                else GC_TREE_O(float, false, kc32);
+            } else {
+                if (safe) GC_TREE_O(double, true, kc64);
+                else GC_TREE_O(double, false, kc64);
+            }
         }
 #undef GC_TREE_O
 #undef GC_TREE
