@@ -1019,11 +1019,12 @@ __global__ __launch_bounds__(256) void gather_sorted_kernel(
 // non-zero distance from its centre of mass and width / distance < theta,
 // and otherwise its children's sums.
 //
-// Here one wave walks for the targets of a group, one per lane: the
-// largest subtrees holding at most 64 particles (a node of <= 64 whose
-// parent holds more, or the root), so that leaves of a few particles share
-// a wave and every target lies in the group's cube.  The walk is depth
-// first with a stack in LDS of (node, mask of lanes that still need it).  At an internal node each lane takes the reference's decision for
+// Here one wave walks for 64 consecutive targets in key order (spatially
+// compact), one per lane.  A lane's sums depend only on the nodes it visits
+// itself, in depth-first order, so grouping targets changes the work
+// shared by a wave, not any target's walk.  The walk is depth first with a
+// stack in LDS of (node, mask of lanes that still need it).  At an
+// internal node each lane takes the reference's decision for
 // its own target (in FP64, so decisions match it exactly); lanes that accept
 // evaluate the M2P from the node's coefficient table (far_m2p.hpp, read by
 // scalar loads since the node is uniform), the others carry the node's
