@@ -1219,10 +1219,16 @@ __global__ __launch_bounds__(256) void tree_leaf_count_kernel(int nnodes, const 
                                                               const int* __restrict__ ne,
                                                               const int* __restrict__ nchild,
                                                               unsigned long long* __restrict__ counters) {
+    __shared__ int wave_leaves[4];
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool leaf = k < nnodes && nchild[k] == 0 && ne[k] > nb[k];
     const unsigned long long m = __ballot(leaf);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(counters + 2, static_cast<unsigned long long>(__popcll(m)));
+    if ((threadIdx.x & 63) == 0) wave_leaves[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one atomic per block
+        const int c = wave_leaves[0] + wave_leaves[1] + wave_leaves[2] + wave_leaves[3];
+        if (c) atomicAdd(counters + 2, static_cast<unsigned long long>(c));
+    }
 }
 
 __global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, double* __restrict__ ax,
