@@ -251,10 +251,18 @@ struct gc_writer {
         }
     }
 
+    // queues j; if queueing throws, j's slot is freed before the exception
+    // leaves (a copy still in flight into it is ordered before any later
+    // copy into the same slot on that stream, and nothing reads it)
     void push(Job&& j) {
         {
             std::lock_guard<std::mutex> lock(mu);
-            jobs.push_back(std::move(j));
+            try {
+                jobs.push_back(std::move(j));
+            } catch (...) {
+                if (j.pool) j.pool->busy[j.slot] = 0;
+                throw;
+            }
         }
         cv.notify_all();
     }
@@ -328,7 +336,13 @@ gc_status gc_writer_open(gc_writer** out, const char* directory) {
                "center_of_mass_x,center_of_mass_y,center_of_mass_z,"
                "angular_momentum_x,angular_momentum_y,angular_momentum_z\n",
                w->diag);
-    w->worker = std::thread([w] { w->run(); });
+    try {
+        w->worker = std::thread([w] { w->run(); });
+    } catch (...) {
+        std::fclose(w->diag);
+        delete w;
+        throw;
+    }
     *out = w;
     return GC_OK;
     GC_ABI_END("gc_writer_open")
@@ -389,21 +403,21 @@ gc_status gc_writer_snapshot(gc_writer* w, gc_ctx* ctx, int step, double time,
     gc_status st = gc::stage_state(ctx, nullptr, 0, &n, &cap, nullptr);  // sizes only
     if (st == GC_OK) st = w->ensure(w->snaps, 10 * cap, "writer_snapshot");
     if (st != GC_OK) return st;
+    Job j;  // everything that allocates happens before a slot is taken
+    j.step = step;
+    j.time = time;
+    if (group_ids) j.groups.assign(group_ids, group_ids + n);
     const int k = w->take(w->snaps);
     st = gc::stage_state(ctx, w->snaps.slot(k), w->snaps.stride, &n, &cap, w->snaps.ev[k]);
     if (st != GC_OK) {
         w->give_back(w->snaps, k);
         return st;
     }
-    Job j;
-    j.step = step;
-    j.time = time;
     j.n = n;
     j.stride = cap;
     j.staged = w->snaps.slot(k);
     j.pool = &w->snaps;
     j.slot = k;
-    if (group_ids) j.groups.assign(group_ids, group_ids + n);
     w->push(std::move(j));
     return GC_OK;
     GC_ABI_END("gc_writer_snapshot")
