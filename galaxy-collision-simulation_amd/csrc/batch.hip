@@ -531,6 +531,11 @@ gc_status gc_batch_generate_disks(gc_batch* b, const gc_disk_galaxy* galaxies,
         for (int k = 0; k < galaxies_per_run; ++k) {
             const gc_disk_galaxy& g = galaxies[static_cast<size_t>(r) * galaxies_per_run + k];
             if (g.n_particles == 0) continue;
+            if (g.n_particles > b->counts[r] - in_run)  // also keeps in_run from wrapping
+                return bfail(b, GC_ERR_INVALID_ARGUMENT,
+                             "batch_generate_disks: run " + std::to_string(r) +
+                                 "'s galaxies hold more than its " +
+                                 std::to_string(b->counts[r]) + " particles");
             IcSegment sg;
             sg.model = kIcTwoDisk;
             sg.stream = static_cast<unsigned>(k);

@@ -250,12 +250,14 @@ gc_status need_particles(gc_ctx* c) {
 
 namespace {
 
+constexpr size_t kMaxIcParticles = static_cast<size_t>(1) << 30;
+
 // Replaces the resident state with n particles drawn on the device from
 // segs (begin/count cover [0, n), dst = begin).
 gc_status generate_state(gc_ctx* c, std::vector<IcSegment>& segs, size_t n, const char* what) {
     gc_status st = need_particles(c);
     if (st != GC_OK) return st;
-    if (n > static_cast<size_t>(1) << 30)
+    if (n > kMaxIcParticles)
         return fail(c, GC_ERR_INVALID_ARGUMENT, std::string(what) + ": particle count exceeds 2^30");
     st = ensure_capacity(c, n);
     if (st != GC_OK || n == 0) return st;
@@ -588,6 +590,8 @@ gc_status gc_generate_galaxies_device(gc_ctx* c, const gc_galaxy_config* galaxie
     std::vector<IcSegment> segs;
     size_t n = 0;
     for (int k = 0; k < count; ++k) {
+        if (galaxies[k].n_particles > kMaxIcParticles)  // bounds the sum below
+            return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_galaxies_device: particle count exceeds 2^30");
         if (ic::ref_disk_empty(galaxies[k])) continue;
         IcSegment sg;
         sg.model = kIcRefDisk;
@@ -612,6 +616,8 @@ gc_status gc_generate_disks_device(gc_ctx* c, const gc_disk_galaxy* galaxies, in
     std::vector<IcSegment> segs;
     size_t n = 0;
     for (int k = 0; k < count; ++k) {
+        if (galaxies[k].n_particles > kMaxIcParticles)  // bounds the sum below
+            return fail(c, GC_ERR_INVALID_ARGUMENT, "generate_disks_device: particle count exceeds 2^30");
         if (galaxies[k].n_particles == 0) continue;
         IcSegment sg;
         sg.model = kIcTwoDisk;

@@ -125,6 +125,11 @@ def test_device_generation_bad_arguments(gpu):
     with gpu.Engine("hip-direct", "mixed") as eng:
         with pytest.raises(gpu.GcError):
             eng.generate_plummer(10, scale=0.0)
+        # counts whose sum wraps to 0 in 64 bits are rejected before any launch
+        with pytest.raises(gpu.GcError):
+            eng.generate_reference_galaxies([dict(n_particles=2 ** 63), dict(n_particles=2 ** 63)])
+        with pytest.raises(gpu.GcError):
+            eng.generate_two_disk(n_per_galaxy=2 ** 31)
     runs = G.collisions(runs=2, seed=4, n_min=300, n_max=600)
     with Batch([r["n"] for r in runs]) as b:
         bad = [dict(runs[0]), dict(runs[1])]
