@@ -346,7 +346,9 @@ void gc_default_disk_galaxy(gc_disk_galaxy* g);
  * resident state (accelerations zeroed) with nothing crossing PCIe.  The
  * models are the host generators' (positions and velocities from the same
  * formulas); the particles differ from the host generators' because the
- * draws do.  G and eps come from the context's (batch's) parameters. */
+ * draws do.  G and eps come from the context's (batch's) parameters.  A
+ * context call takes at most 2^30 particles in all (GC_ERR_INVALID_ARGUMENT
+ * beyond, before any launch). */
 /* generate_disk_galaxy's model per gc_galaxy_config (empty galaxies skipped) */
 gc_status gc_generate_galaxies_device(gc_ctx* ctx, const gc_galaxy_config* galaxies, int count,
                                       uint64_t seed);
