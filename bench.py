@@ -557,6 +557,18 @@ def main() -> int:
     sources = eng.sources() if args.precision == "mixed" else None
     pair_form = eng.pairs() if args.precision == "mixed" else "targets"
     diag = eng.diagnostics()
+    # host-buffer use (the reference's compute_accelerations on a host
+    # particle vector): upload, one force evaluation, download, over PCIe
+    eng.upload(pos, vel, mass)
+    eng.compute_accelerations()
+    eng.download()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.upload(pos, vel, mass)
+        eng.compute_accelerations()
+        eng.download()
+    host_ms = (time.perf_counter() - t0) * 1e3 / reps
     eng.close()
 
     peak = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
@@ -599,6 +611,12 @@ def main() -> int:
             "flop_per_pair": FLOP_PER_PAIR,
         },
         "energy_after_steps": diag["total_energy"],
+        "host_buffers": {
+            "ms_per_evaluation": round(host_ms, 4),
+            "Gpair_per_s": round(pairs / (host_ms * 1e-3) / 1e9, 3),
+            "sample": f"upload of host arrays, one force evaluation and download of "
+                      f"positions, velocities and accelerations, mean of {reps}",
+        },
     }
 
     if not args.no_fp64 and args.precision == "mixed":
