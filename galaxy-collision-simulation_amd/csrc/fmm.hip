@@ -1091,13 +1091,20 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
         const int c = min(kEvalLanes, te - base);
         const bool active = lane < c;
         const int me = base + (active ? lane : 0);  // sorted index of this lane's target
+        const int id = sidx[me];
+        // only targets in [begin, end) walk; a chunk without one is skipped
+        // (the offsets' origin above stays that of the whole chunk, so a
+        // target's sums do not depend on the range)
+        const bool want = active && id >= begin && id < end;
+        const unsigned long long walkers = __ballot(want);
+        if (walkers == 0) continue;
         const double x = sx[me], y = sy[me], z = sz[me];
         const F tx = static_cast<F>(x - cx), ty = static_cast<F>(y - cy), tz = static_cast<F>(z - cz);
         double a[3] = {0.0, 0.0, 0.0};
         int sp = 0;
         if (lane == 0) {
             stack_node[0] = 0;
-            stack_mask[0] = c == 64 ? ~0ull : ((1ull << c) - 1ull);
+            stack_mask[0] = walkers;
         }
         __syncthreads();
         sp = 1;
@@ -1192,18 +1199,15 @@ __global__ __launch_bounds__(kEvalLanes) void tree_eval_kernel(
                 __syncthreads();
             }
         }
-        if (active) {
-            const int id = sidx[me];
-            if (id >= begin && id < end) {
-                const double gx = G * a[0], gy = G * a[1], gz = G * a[2];
-                ax[id] = gx;
-                ay[id] = gy;
-                az[id] = gz;
-                if (close_kick) {
-                    vx[id] = vx[id] + gx * half_dt;
-                    vy[id] = vy[id] + gy * half_dt;
-                    vz[id] = vz[id] + gz * half_dt;
-                }
+        if (want) {
+            const double gx = G * a[0], gy = G * a[1], gz = G * a[2];
+            ax[id] = gx;
+            ay[id] = gy;
+            az[id] = gz;
+            if (close_kick) {
+                vx[id] = vx[id] + gx * half_dt;
+                vy[id] = vy[id] + gy * half_dt;
+                vz[id] = vz[id] + gz * half_dt;
             }
         }
         __syncthreads();

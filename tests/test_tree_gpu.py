@@ -65,8 +65,12 @@ def test_massless_particles_still_feel_forces(gpu):
 def test_target_range_and_kdk_steps(gpu):
     pos, vel, m = gpu.generators.plummer(2500, seed=23)
     want, _ = O.tree(pos, m, 1.0, 0.02, 0.6, 16, 32, 4)
-    got, _ = _tree(gpu, "fp64", pos, m, 0.02, 0.6, 16, 4, 500, 1800)
+    got, st = _tree(gpu, "fp64", pos, m, 0.02, 0.6, 16, 4, 500, 1800)
     assert rel_err(got[500:1800], want[500:1800]).max() < FP64_TOL
+    # only the range's targets walk: the counts are the range's own
+    _, ost = O.tree(pos, m, 1.0, 0.02, 0.6, 16, 32, 4, begin=500, end=1800)
+    assert st["m2p_evaluations"] == ost["m2p_evaluations"]
+    assert st["p2p_pairs"] == ost["p2p_pairs"]
     solver = O.make_solver(O.SOLVER_TREE, 1.0, 0.02, 0.6, 16, 32, 4)
     rp, rv, ra = O.leapfrog(solver, pos, vel, want, m, 1e-3, 10)
     with gpu.Engine("hip-tree", "fp64", softening=0.02, theta=0.6, leaf_capacity=16) as eng:
@@ -76,3 +80,24 @@ def test_target_range_and_kdk_steps(gpu):
         gp, gv, ga = eng.download()
     assert np.abs(gp - rp).max() < 1e-11
     assert rel_err(ga, ra).max() < 1e-9
+
+
+def test_plummer_full_size_sampled(gpu):
+    # N = 2^20 Plummer sphere at the tree benchmark's settings (theta 0.6,
+    # leaves <= 16, p = 4, eps 0.01).  The FP32 walk over every particle, on
+    # its first 2048 targets (i.i.d. draws, so a random sample): against the
+    # oracle tree, and against FP64 direct sums for the method's own error.
+    # The same targets as a range give the oracle's exact walk counts.
+    pos, vel, m = gpu.generators.plummer(1 << 20, seed=1)
+    k = 2048
+    got, st = _tree(gpu, "mixed", pos, m, 0.01, 0.6, 16)
+    want, ost = O.tree(pos, m, 1.0, 0.01, 0.6, 16, 32, 4, begin=0, end=k)
+    assert st["node_count"] == ost["node_count"] and st["leaf_count"] == ost["leaf_count"]
+    assert rel_err(got[:k], want[:k]).max() < MIXED_TOL
+    part, pst = _tree(gpu, "mixed", pos, m, 0.01, 0.6, 16, 4, 0, k)
+    assert pst["m2p_evaluations"] == ost["m2p_evaluations"]
+    assert pst["p2p_pairs"] == ost["p2p_pairs"]
+    assert np.array_equal(part[:k], got[:k])
+    direct = O.direct_targets(pos, m, 1.0, 0.01, 0, k, threads=0)
+    err = rel_err(got[:k], direct[:k])
+    assert np.median(err) < 2e-4 and err.max() < 1e-2
