@@ -19,7 +19,7 @@ constexpr int kTile = 256;
 // many entries of padding past its last tile.
 constexpr int kStreamBatch = 8;
 
-inline size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
+constexpr size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
 // Resident SoA particle state in HBM (FP64), capacity padded to kTile.
 struct DeviceState {

@@ -116,6 +116,8 @@ SIGNATURES = {
                                       C.c_void_p]),
     "gc_get_state_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "gc_step_reverse": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "gc_direct_accelerations_noalloc": (C.c_int, [C.POINTER(Params), C.c_size_t, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p]),
     "gc_force_vjp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "gc_fmm_last_stats": (C.c_int, [C.c_void_p, C.POINTER(FmmStats)]),
     "gc_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(Params), C.c_int,

@@ -158,6 +158,31 @@ gc_status gc_compute_accelerations_for_targets(gc_ctx* ctx, size_t begin, size_t
 gc_status gc_time_force_kernel(gc_ctx* ctx, int reps, double* ms_per_launch,
                                double* interactions_per_launch);
 
+/* ---- no device allocation (no counterpart in the reference: for callers
+ * that cannot allocate device memory, such as code under HIP graph
+ * capture, tight serving loops and harnesses that forbid hipMalloc) ----
+ * Mixed-precision direct accelerations of n particles held in caller-owned
+ * device arrays: pos [3][n] (all x, then all y, then all z) and mass [n] in,
+ * acc [3][n] out.  G and softening come from p, which must select
+ * GC_SOLVER_DIRECT and GC_PRECISION_MIXED.  It runs the per-target kernels
+ * of gc_compute_accelerations (GC_PAIRS_TARGETS, one target per lane,
+ * scalar-load sources), asynchronously on `stream` (NULL: the null stream)
+ * of the device that owns that stream.
+ * The call performs no hipMalloc, hipFree or other allocation: its working
+ * buffers (source tiles, tile centres, per-split partial sums, 16.8 MB in
+ * all) are one static device array that each device holds until the process
+ * ends.  Every call on a device shares that device's array, so calls that
+ * overlap on one device race.  n above GC_NOALLOC_MAX_PARTICLES returns
+ * GC_ERR_INVALID_ARGUMENT before any launch.  Source splits: the context
+ * path's automatic count, min(64, ceil(16384 / T), T) for T = ceil(n / 256)
+ * tiles, capped at GC_NOALLOC_PARTIAL_FLOATS / (6 * 256 * T); a context set
+ * to the same split count (gc_set_launch_shape(ctx, 1, splits),
+ * GC_PAIRS_TARGETS) gives bit-identical accelerations. */
+#define GC_NOALLOC_MAX_PARTICLES 262144
+#define GC_NOALLOC_PARTIAL_FLOATS 3145728
+gc_status gc_direct_accelerations_noalloc(const gc_params* p, size_t n, const double* pos,
+                                          const double* mass, double* acc, void* stream);
+
 /* ---- differentiation (no reference counterpart: the reference has no
  * adjoint; these serve gradient-based callers such as torch.autograd) ----
  * Device arrays below are caller-owned, FP64, SoA "[3][n]" (all x, then all
