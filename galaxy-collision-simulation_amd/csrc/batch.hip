@@ -293,8 +293,12 @@ gc_status build_schedule(gc_batch* b, int chunk) {
     std::stable_sort(items.begin(), items.end(),
                      [](const int4& a, const int4& c) { return (a.z - a.y) > (c.z - c.y); });
     GB_HIP(b, hipStreamSynchronize(b->stream), "synchronize before schedule");
+    // a failure below leaves no schedule (nitems 0): the next launch
+    // rebuilds it (schedule_ready)
+    drop_graph(b);
     (void)hipFree(b->items);
     b->items = nullptr;
+    b->nitems = 0;
     GB_HIP(b, hipMalloc(&b->items, sizeof(int4) * std::max<size_t>(1, items.size())), "allocate work list");
     GB_HIP(b, hipMemcpy(b->items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice), "upload work list");
     GB_HIP(b, hipMemcpy(b->tile_slot0, slot0.data(), sizeof(int) * b->ntiles, hipMemcpyHostToDevice), "upload slots");
@@ -303,13 +307,20 @@ gc_status build_schedule(gc_batch* b, int chunk) {
     if (need > b->part_floats) {
         (void)hipFree(b->part);
         b->part = nullptr;
+        b->part_floats = 0;
         GB_HIP(b, hipMalloc(&b->part, sizeof(float) * need), "allocate batch partials");
         b->part_floats = need;
     }
     b->nitems = static_cast<int>(items.size());
     b->chunk = chunk;
-    drop_graph(b);
     return GC_OK;
+}
+
+// The per-target form needs its work list; one lost to a failed
+// reallocation is rebuilt for the requested chunk before any launch.
+gc_status schedule_ready(gc_batch* b) {
+    if (batch_symmetric(b) || b->nitems > 0 || b->ntiles == 0) return GC_OK;
+    return build_schedule(b, std::max(1, b->chunk));
 }
 
 void launch_force(gc_batch* b) {
@@ -589,6 +600,8 @@ gc_status gc_batch_compute_accelerations(gc_batch* b) {
     GC_ABI_BEGIN
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     GB_HIP(b, hipSetDevice(b->device), "select device");
+    gc_status st = schedule_ready(b);
+    if (st != GC_OK) return st;
     launch_force(b);
     launch_merge(b, false, 0.0);
     GB_HIP(b, hipGetLastError(), "launch batch force");
@@ -601,6 +614,8 @@ gc_status gc_batch_step(gc_batch* b, double dt, int nsteps) {
     if (!b) return bfail(nullptr, GC_ERR_INVALID_ARGUMENT, "null batch");
     if (!(dt > 0.0) || nsteps < 0) return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_step: dt must be positive, steps non-negative");
     GB_HIP(b, hipSetDevice(b->device), "select device");
+    gc_status st = schedule_ready(b);
+    if (st != GC_OK) return st;
     int left = nsteps;
     if (b->graph_steps > 1 && left >= b->graph_steps) {
         if (!b->graph || b->graph_dt != dt) {
@@ -674,6 +689,8 @@ gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms, double* 
     GC_ABI_BEGIN
     if (!b || reps <= 0 || !ms) return bfail(b, GC_ERR_INVALID_ARGUMENT, "batch_time: bad arguments");
     GB_HIP(b, hipSetDevice(b->device), "select device");
+    gc_status st0 = schedule_ready(b);
+    if (st0 != GC_OK) return st0;
     const float eps2 = static_cast<float>(b->params.softening * b->params.softening);
     const bool safe = !(eps2 >= kTinySeparation2);
     hipLaunchKernelGGL(batch_pack_kernel, dim3(b->ntiles), dim3(kTile), 0, b->stream, b->tile_valid,

@@ -98,7 +98,37 @@ gc_status ensure_capacity(gc_ctx* c, size_t n) {
     }
     GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize before reallocation");
     free_state(c);
-    GC_HIP(c, hipMalloc(&c->s.base, sizeof(double) * 10 * cap), "allocate particle state");
+    // every buffer, or none: a failure frees what was allocated and leaves
+    // cap at 0, so the next call allocates again
+    const size_t ntiles_cap = cap / kTile;
+    const size_t diag_doubles = diagnostics_scratch_doubles(cap);
+    const char* what = "allocate particle state";
+    hipError_t e = hipMalloc(&c->s.base, sizeof(double) * 10 * cap);
+    if (e == hipSuccess) {
+        what = "allocate mixed source tiles";
+        e = hipMalloc(&c->mixed.jp, sizeof(float4) * (cap + kStreamBatch));
+    }
+    if (e == hipSuccess) {
+        what = "allocate tile centres";
+        e = hipMalloc(&c->mixed.centre, sizeof(double) * 3 * ntiles_cap);
+    }
+    if (e == hipSuccess) {
+        what = "allocate staging buffer";
+        e = hipMalloc(&c->stage, sizeof(double) * 3 * cap);
+    }
+    if (e == hipSuccess) {
+        what = "allocate diagnostics scratch";
+        e = hipMalloc(&c->diag_scratch, sizeof(double) * diag_doubles);
+    }
+    if (e == hipSuccess) {
+        what = "clear state";
+        e = hipMemsetAsync(c->s.base, 0, sizeof(double) * 10 * cap, c->stream);
+    }
+    if (e != hipSuccess) {
+        free_state(c);
+        return fail(c, e == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
+                    std::string(what) + ": " + hipGetErrorString(e));
+    }
     double* b = c->s.base;
     c->s.x = b;
     c->s.y = b + cap;
@@ -110,18 +140,10 @@ gc_status ensure_capacity(gc_ctx* c, size_t n) {
     c->s.ay = b + 7 * cap;
     c->s.az = b + 8 * cap;
     c->s.m = b + 9 * cap;
+    c->diag_scratch_doubles = diag_doubles;
     c->s.cap = cap;
     c->s.n = n;
-    const size_t ntiles_cap = cap / kTile;
-    GC_HIP(c, hipMalloc(&c->mixed.jp, sizeof(float4) * (cap + kStreamBatch)),
-           "allocate mixed source tiles");
-    GC_HIP(c, hipMalloc(&c->mixed.centre, sizeof(double) * 3 * ntiles_cap), "allocate tile centres");
     c->mixed.ntiles = (n + kTile - 1) / kTile;
-    GC_HIP(c, hipMalloc(&c->stage, sizeof(double) * 3 * cap), "allocate staging buffer");
-    c->diag_scratch_doubles = diagnostics_scratch_doubles(cap);
-    GC_HIP(c, hipMalloc(&c->diag_scratch, sizeof(double) * c->diag_scratch_doubles),
-           "allocate diagnostics scratch");
-    GC_HIP(c, hipMemsetAsync(c->s.base, 0, sizeof(double) * 10 * cap, c->stream), "clear state");
     return GC_OK;
 }
 

@@ -1297,6 +1297,22 @@ hipError_t grow(T** p, size_t* cap, size_t need, size_t keep, hipStream_t st) {
     return hipSuccess;
 }
 
+// Frees *p and allocates `count` elements; *cap is `count` on success and 0
+// (with *p null) on failure, so a failed call is retried by the next one.
+template <typename T>
+hipError_t realloc_dev(T** p, size_t* cap, size_t count) {
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(p, sizeof(T) * std::max<size_t>(1, count));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    *cap = count;
+    return hipSuccess;
+}
+
 }  // namespace
 
 struct FmmEngine::Impl {
@@ -1429,29 +1445,39 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     const double theta = std::max(1.0e-6, p.theta);
     const int bblocks = std::min(1024, (n + 255) / 256);
 
-    // particle buffers
+    // particle buffers: all of them, or none (pcap stays 0 until the last
+    // allocation succeeds, so a failed build allocates again next time)
     if (static_cast<size_t>(n) > pcap) {
-        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part};
-        for (void* q : ptrs) (void)hipFree(q);
-        pcap = static_cast<size_t>(n);
-        FMM_HIP(hipMalloc(&sxs, sizeof(double) * pcap), "allocate sorted state");
-        FMM_HIP(hipMalloc(&sys, sizeof(double) * pcap), "allocate sorted state");
-        FMM_HIP(hipMalloc(&szs, sizeof(double) * pcap), "allocate sorted state");
-        FMM_HIP(hipMalloc(&sms, sizeof(double) * pcap), "allocate sorted state");
-        FMM_HIP(hipMalloc(&keys, sizeof(unsigned long long) * pcap), "allocate keys");
-        FMM_HIP(hipMalloc(&keys_sorted, sizeof(unsigned long long) * pcap), "allocate keys");
-        FMM_HIP(hipMalloc(&idx, sizeof(int) * pcap), "allocate indices");
-        FMM_HIP(hipMalloc(&idx_sorted, sizeof(int) * pcap), "allocate indices");
+        void** ptrs[] = {(void**)&keys, (void**)&keys_sorted, (void**)&idx, (void**)&idx_sorted,
+                         (void**)&sxs, (void**)&sys, (void**)&szs, (void**)&sms, &sort_tmp,
+                         (void**)&bbox_part};
+        for (void** q : ptrs) {
+            (void)hipFree(*q);
+            *q = nullptr;
+        }
+        pcap = 0;
+        const size_t np = static_cast<size_t>(n);
+        FMM_HIP(hipMalloc(&sxs, sizeof(double) * np), "allocate sorted state");
+        FMM_HIP(hipMalloc(&sys, sizeof(double) * np), "allocate sorted state");
+        FMM_HIP(hipMalloc(&szs, sizeof(double) * np), "allocate sorted state");
+        FMM_HIP(hipMalloc(&sms, sizeof(double) * np), "allocate sorted state");
+        FMM_HIP(hipMalloc(&keys, sizeof(unsigned long long) * np), "allocate keys");
+        FMM_HIP(hipMalloc(&keys_sorted, sizeof(unsigned long long) * np), "allocate keys");
+        FMM_HIP(hipMalloc(&idx, sizeof(int) * np), "allocate indices");
+        FMM_HIP(hipMalloc(&idx_sorted, sizeof(int) * np), "allocate indices");
         FMM_HIP(hipMalloc(&bbox_part, sizeof(double) * 6 * 1024), "allocate bbox partials");
         sort_tmp_bytes = 0;
-        FMM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp_bytes, keys, keys_sorted, idx,
+        size_t sort_bytes = 0;
+        FMM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, keys_sorted, idx,
                                                    idx_sorted, n, 0, 3 * kLevels, st),
                 "size radix sort");
-        FMM_HIP(hipMalloc(&sort_tmp, sort_tmp_bytes), "allocate radix sort storage");
+        FMM_HIP(hipMalloc(&sort_tmp, sort_bytes), "allocate radix sort storage");
+        sort_tmp_bytes = sort_bytes;
         if (!overflow) FMM_HIP(hipMalloc(&overflow, sizeof(int)), "allocate flag");
         if (!stats_dev) FMM_HIP(hipMalloc(&stats_dev, sizeof(unsigned long long) * 4), "allocate stats");
         if (!tree_counts)
             FMM_HIP(hipHostMalloc(&tree_counts, sizeof(unsigned long long) * 4), "allocate counts");
+        pcap = np;
     }
     const size_t need_nodes = std::max<size_t>(1024, 2 * static_cast<size_t>(n) / cap + 64);
     if (need_nodes > ncap) FMM_HIP(grow_nodes(need_nodes, 0, st), "allocate nodes");
@@ -1479,19 +1505,24 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         const int cnt = le - lb;
         if (cnt == 0) break;
         if (static_cast<size_t>(cnt) > lcap) {
-            (void)hipFree(bounds);
-            (void)hipFree(count);
-            (void)hipFree(offset);
-            (void)hipFree(scan_tmp);
-            lcap = static_cast<size_t>(cnt) * 2;
-            FMM_HIP(hipMalloc(&bounds, sizeof(int) * 9 * lcap), "allocate level scratch");
-            FMM_HIP(hipMalloc(&count, sizeof(int) * lcap), "allocate level scratch");
-            FMM_HIP(hipMalloc(&offset, sizeof(int) * lcap), "allocate level scratch");
+            void** ptrs[] = {(void**)&bounds, (void**)&count, (void**)&offset, &scan_tmp};
+            for (void** q : ptrs) {
+                (void)hipFree(*q);
+                *q = nullptr;
+            }
+            lcap = 0;
             scan_tmp_bytes = 0;
-            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp_bytes, count, offset,
-                                                     static_cast<int>(lcap), st),
+            const size_t want = static_cast<size_t>(cnt) * 2;
+            FMM_HIP(hipMalloc(&bounds, sizeof(int) * 9 * want), "allocate level scratch");
+            FMM_HIP(hipMalloc(&count, sizeof(int) * want), "allocate level scratch");
+            FMM_HIP(hipMalloc(&offset, sizeof(int) * want), "allocate level scratch");
+            size_t scan_bytes = 0;
+            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count, offset,
+                                                     static_cast<int>(want), st),
                     "size scan");
-            FMM_HIP(hipMalloc(&scan_tmp, scan_tmp_bytes), "allocate scan storage");
+            FMM_HIP(hipMalloc(&scan_tmp, scan_bytes), "allocate scan storage");
+            scan_tmp_bytes = scan_bytes;
+            lcap = want;
         }
         const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
         hipLaunchKernelGGL(split_count_kernel, dim3(blocks), dim3(256), 0, st, lb, le, level, cap,
@@ -1558,11 +1589,9 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         size_t need2 = 0;
         FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, nfar, far_off, nnodes, st), "size scan");
         need = std::max(need, need2);
-        if (need > scan_tmp_bytes) {
-            (void)hipFree(scan_tmp);
-            scan_tmp_bytes = need;
-            FMM_HIP(hipMalloc(&scan_tmp, scan_tmp_bytes), "allocate scan storage");
-        }
+        if (need > scan_tmp_bytes)
+            FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
+                    "allocate scan storage");
     }
     const unsigned nblk = static_cast<unsigned>((nnodes + 255) / 256);
     hipLaunchKernelGGL(leaf_flag_kernel, dim3(nblk), dim3(256), 0, st, nnodes, nchild, mcom, flag);
@@ -1587,37 +1616,22 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     if (stash) {
         const size_t fneed = static_cast<size_t>(lwaves) * wcap_far;
         const size_t nneed = static_cast<size_t>(lwaves) * wcap_near;
-        if (fneed > stash_far_cap) {
-            (void)hipFree(stash_far);
-            stash_far = nullptr;
-            stash_far_cap = fneed;
-            FMM_HIP(hipMalloc(&stash_far, sizeof(int) * stash_far_cap), "allocate list stash");
-        }
-        if (nneed > stash_near_cap) {
-            (void)hipFree(stash_near);
-            stash_near = nullptr;
-            stash_near_cap = nneed;
-            FMM_HIP(hipMalloc(&stash_near, sizeof(int) * stash_near_cap), "allocate list stash");
-        }
+        if (fneed > stash_far_cap)
+            FMM_HIP(realloc_dev(&stash_far, &stash_far_cap, fneed), "allocate list stash");
+        if (nneed > stash_near_cap)
+            FMM_HIP(realloc_dev(&stash_near, &stash_near_cap, nneed), "allocate list stash");
     }
     if (static_cast<size_t>(nleaves) > lpos_cap) {
-        (void)hipFree(lpos_far);
-        (void)hipFree(lpos_near);
-        lpos_far = lpos_near = nullptr;
-        lpos_cap = static_cast<size_t>(nleaves) * 2;
-        FMM_HIP(hipMalloc(&lpos_far, sizeof(long long) * lpos_cap), "allocate list stash");
-        FMM_HIP(hipMalloc(&lpos_near, sizeof(long long) * lpos_cap), "allocate list stash");
+        size_t c2 = 0;
+        lpos_cap = 0;  // set by the last allocation only
+        FMM_HIP(realloc_dev(&lpos_near, &c2, static_cast<size_t>(nleaves) * 2), "allocate list stash");
+        FMM_HIP(realloc_dev(&lpos_far, &lpos_cap, c2), "allocate list stash");
     }
     int lovf = 0;
     for (;;) {
         const size_t need = static_cast<size_t>(lwaves) * 2 * fcap;
-        if (need > frontier_ints) {
-            (void)hipFree(frontier);
-            frontier = nullptr;
-            frontier_ints = 0;
-            FMM_HIP(hipMalloc(&frontier, sizeof(int) * need), "allocate list frontiers");
-            frontier_ints = need;
-        }
+        if (need > frontier_ints)
+            FMM_HIP(realloc_dev(&frontier, &frontier_ints, need), "allocate list frontiers");
         FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
         if (stash)
             hipLaunchKernelGGL((lists_kernel<2>), dim3(lwaves), dim3(64), 0, st, nleaves, leaves,
@@ -1645,9 +1659,7 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, out, nleaves, st);
         if (e != hipSuccess) return e;
         if (need > scan_tmp_bytes) {
-            (void)hipFree(scan_tmp);
-            scan_tmp_bytes = need;
-            e = hipMalloc(&scan_tmp, scan_tmp_bytes);
+            e = realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need);
             if (e != hipSuccess) return e;
         }
         return hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, it, out, nleaves, st);
@@ -1662,18 +1674,14 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipMemcpyAsync(&lc[1], nnear + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
     FMM_HIP(hipStreamSynchronize(st), "list sizes");
     const long long total_far = lo[0] + lc[0], total_near = lo[1] + lc[1];
-    if (static_cast<size_t>(total_far) + 1 > farcap) {
-        (void)hipFree(far);
-        far = nullptr;
-        farcap = static_cast<size_t>(total_far) + 1 + static_cast<size_t>(total_far) / 4;
-        FMM_HIP(hipMalloc(&far, sizeof(int) * farcap), "allocate far lists");
-    }
-    if (static_cast<size_t>(total_near) + 1 > nearcap) {
-        (void)hipFree(near);
-        near = nullptr;
-        nearcap = static_cast<size_t>(total_near) + 1 + static_cast<size_t>(total_near) / 4;
-        FMM_HIP(hipMalloc(&near, sizeof(int) * nearcap), "allocate near lists");
-    }
+    if (static_cast<size_t>(total_far) + 1 > farcap)
+        FMM_HIP(realloc_dev(&far, &farcap,
+                            static_cast<size_t>(total_far) + 1 + static_cast<size_t>(total_far) / 4),
+                "allocate far lists");
+    if (static_cast<size_t>(total_near) + 1 > nearcap)
+        FMM_HIP(realloc_dev(&near, &nearcap,
+                            static_cast<size_t>(total_near) + 1 + static_cast<size_t>(total_near) / 4),
+                "allocate near lists");
     if (stash && !(lovf & 2)) {
         hipLaunchKernelGGL(list_copy_kernel, dim3(nleaves), dim3(256), 0, st, nleaves, lwaves, nfar,
                            nnear, far_off, near_off, lpos_far, lpos_near, wcap_far, wcap_near,
@@ -1706,13 +1714,11 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     // near-field work items: leaf li gets max(1, ceil(nnear / kNearItem))
     // items, each with one partial row per target of the leaf
     if (static_cast<size_t>(nleaves) > icap) {
-        (void)hipFree(icount);
-        (void)hipFree(ioff);
-        (void)hipFree(row_off);
-        icap = static_cast<size_t>(nleaves) * 2;
-        FMM_HIP(hipMalloc(&icount, sizeof(int) * icap), "allocate near items");
-        FMM_HIP(hipMalloc(&ioff, sizeof(int) * icap), "allocate near items");
-        FMM_HIP(hipMalloc(&row_off, sizeof(long long) * icap), "allocate near items");
+        size_t c2 = 0, c3 = 0;
+        icap = 0;  // set by the last allocation only
+        FMM_HIP(realloc_dev(&ioff, &c2, static_cast<size_t>(nleaves) * 2), "allocate near items");
+        FMM_HIP(realloc_dev(&row_off, &c3, c2), "allocate near items");
+        FMM_HIP(realloc_dev(&icount, &icap, c3), "allocate near items");
     }
     hipLaunchKernelGGL(near_item_count_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st,
                        nleaves, nnear, icount);
@@ -1725,11 +1731,9 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         size_t need2 = 0;
         FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, rit, row_off, nleaves, st), "size scan");
         need = std::max(need, need2);
-        if (need > scan_tmp_bytes) {
-            (void)hipFree(scan_tmp);
-            scan_tmp_bytes = need;
-            FMM_HIP(hipMalloc(&scan_tmp, scan_tmp_bytes), "allocate scan storage");
-        }
+        if (need > scan_tmp_bytes)
+            FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
+                    "allocate scan storage");
         FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, icount, ioff, nleaves, st),
                 "scan near items");
         FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, rit, row_off, nleaves, st),
@@ -1748,16 +1752,12 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipStreamSynchronize(st), "row count");
     nitems = ilast[0] + ilast[1];
     const size_t rows = static_cast<size_t>(rlast) + static_cast<size_t>(ilast[1]) * (ne_last - nb_last);
-    if (static_cast<size_t>(nitems) > itemcap) {
-        (void)hipFree(items);
-        itemcap = static_cast<size_t>(nitems) + static_cast<size_t>(nitems) / 4;
-        FMM_HIP(hipMalloc(&items, sizeof(NearItem) * itemcap), "allocate near items");
-    }
-    if (3 * rows > partcap) {
-        (void)hipFree(npart);
-        partcap = 3 * rows + 3 * rows / 4;
-        FMM_HIP(hipMalloc(&npart, sizeof(double) * partcap), "allocate near partials");
-    }
+    if (static_cast<size_t>(nitems) > itemcap)
+        FMM_HIP(realloc_dev(&items, &itemcap,
+                            static_cast<size_t>(nitems) + static_cast<size_t>(nitems) / 4),
+                "allocate near items");
+    if (3 * rows > partcap)
+        FMM_HIP(realloc_dev(&npart, &partcap, 3 * rows + 3 * rows / 4), "allocate near partials");
     hipLaunchKernelGGL(near_item_fill_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves,
                        leaves, nb, ne, nnear, icount, ioff, row_off, items);
 

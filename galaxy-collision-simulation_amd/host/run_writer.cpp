@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <filesystem>
 #include <mutex>
 #include <new>
@@ -251,29 +252,36 @@ struct gc_writer {
         }
     }
 
-    // queues j; if queueing throws, j's slot is freed before the exception
-    // leaves (a copy still in flight into it is ordered before any later
-    // copy into the same slot on that stream, and nothing reads it)
+    // queues j; if queueing throws, j's slot is freed (and waiters woken)
+    // before the exception leaves.  A copy may still be in flight into that
+    // slot: take() waits on the slot's event before handing it out again.
     void push(Job&& j) {
+        std::exception_ptr failed;
         {
             std::lock_guard<std::mutex> lock(mu);
             try {
                 jobs.push_back(std::move(j));
             } catch (...) {
                 if (j.pool) j.pool->busy[j.slot] = 0;
-                throw;
+                failed = std::current_exception();
             }
         }
         cv.notify_all();
+        if (failed) std::rethrow_exception(failed);
     }
 
-    // waits for a free slot of pool (the worker frees one per job)
+    // waits for a free slot of pool (the worker frees one per job), then
+    // for the last copy recorded into it
     int take(SlotPool& pool) {
-        std::unique_lock<std::mutex> lock(mu);
-        cv.wait(lock, [&] { return pool.any_free(); });
-        const int k = static_cast<int>(std::find(pool.busy.begin(), pool.busy.end(), 0) -
-                                       pool.busy.begin());
-        pool.busy[k] = 1;
+        int k = 0;
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            cv.wait(lock, [&] { return pool.any_free(); });
+            k = static_cast<int>(std::find(pool.busy.begin(), pool.busy.end(), 0) -
+                                 pool.busy.begin());
+            pool.busy[k] = 1;
+        }
+        if (pool.ev[k]) (void)hipEventSynchronize(pool.ev[k]);
         return k;
     }
 
