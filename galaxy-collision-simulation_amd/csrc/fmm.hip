@@ -3,14 +3,21 @@
 //
 // Per force evaluation, all on the context's stream:
 //  1. bounding box -> root cube exactly as fmm_solver.cpp:60-85 (FP64);
-//  2. per particle a 63-bit octant key: 21 levels of the reference's
+//  2. per particle a 96-bit octant key: 32 levels of the reference's
 //     ">=" tests against child centres recursed as c +- hw/2 in FP64
 //     (fmm_solver.cpp:106-134), so the key path equals the insertion path;
-//  3. stable radix sort of (key, index) (hipCUB);
+//     levels 1-21 in one 63-bit word, levels 22-32 in a 33-bit word;
+//  3. stable radix sort of (key, index) (hipCUB) on the first word; only
+//     when more than leaf_capacity particles share all 21 levels of it (and
+//     max_depth > 21) are both words sorted;
 //  4. level-by-level split: a node at depth L is internal iff it holds more
 //     than leaf_capacity particles and L < max_depth — the tree the
-//     reference's insertion builds (fmm_solver.cpp:136-155).  Empty children
-//     are not materialised (they have zero mass and never enter a list);
+//     reference's insertion builds (fmm_solver.cpp:136-155), to its depth
+//     cap of 32.  Empty children are not materialised (they have zero mass
+//     and never enter a list).  The deepest internal level is known before
+//     the split (a node at level L holds more than c particles iff sorted
+//     particles i and i + c share L levels), and level sizes stay on the
+//     device: each level's kernels run over a bound on its node count;
 //  5. upward pass, deepest level first: mass, centre of mass, 35 raw
 //     moments about the COM in FP64 (P2M at leaves, binomial M2M shift);
 //     then per node the far-field coefficient table of far_m2p.hpp (FP64
@@ -30,6 +37,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -37,9 +45,15 @@
 namespace gc {
 namespace {
 
-constexpr int kLevels = 21;       // 3 key bits per level, 63 bits
+constexpr int kLevels = 21;       // levels in the first key word: 3 bits each, 63 bits
+constexpr int kDeepLevels = 11;   // levels 22-32 in the second word: 33 bits
+constexpr int kMaxLevels = kLevels + kDeepLevels;  // the reference's depth cap
 constexpr int kMom = 35;          // raw Cartesian moments, total degree <= 4
 constexpr int kEvalLanes = 64;
+// device statistics: far, near, M2P, pairs (FMM) or M2P, pairs, leaves
+// (tree), then internal nodes at kStatInternal
+constexpr int kStats = 8;
+constexpr int kStatInternal = 4;
 
 __device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double gc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -103,12 +117,13 @@ __global__ __launch_bounds__(256) void bbox_partial_kernel(int n, const double* 
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Root cube of fmm_solver.cpp:76-85; node 0 of the node arrays.  One wave:
-// lanes take the block partials in strides, then wavefront min / max (exact,
-// so the order does not matter).
+// Root cube of fmm_solver.cpp:76-85 into root[4] (centre, half width), and
+// the first level offsets (level 0 is node 0).  One wave: lanes take the
+// block partials in strides, then wavefront min / max (exact, so the order
+// does not matter).
 __global__ __launch_bounds__(64) void bbox_final_kernel(int nblocks, const double* __restrict__ part,
-                                                        double eps, int n, double* __restrict__ geom,
-                                                        int* __restrict__ nb, int* __restrict__ ne) {
+                                                        double eps, double* __restrict__ root,
+                                                        int* __restrict__ lvl) {
 #pragma clang fp contract(off)
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int b = threadIdx.x; b < nblocks; b += 64)
@@ -129,59 +144,118 @@ __global__ __launch_bounds__(64) void bbox_final_kernel(int nblocks, const doubl
     double hw = 0.5 * ext;
     if (hw < eps + 1.0e-6) hw = eps + 1.0e-6;
     hw *= 1.0001;
-    for (int d = 0; d < 3; ++d) geom[d] = (lo[d] + hi[d]) * 0.5;
-    geom[3] = hw;
+    for (int d = 0; d < 3; ++d) root[d] = (lo[d] + hi[d]) * 0.5;
+    root[3] = hw;
+    lvl[0] = 0;
+    lvl[1] = 1;
+}
+
+// Node 0 of the node arrays: the root cube over all n particles.
+__global__ void root_kernel(int n, const double* __restrict__ root, double* __restrict__ geom,
+                            int* __restrict__ nb, int* __restrict__ ne) {
+    for (int d = 0; d < 4; ++d) geom[d] = root[d];
     nb[0] = 0;
     ne[0] = n;
 }
 
 // ---- 2. octant keys --------------------------------------------------------
+// Levels 1-21 into keys (digit of level l at bits 3 (21 - l)), levels 22-32
+// into keys_lo (digit of level l at bits 3 (32 - l)) when keys_lo is given.
 __global__ __launch_bounds__(256) void keys_kernel(int n, const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
                                                    const double* __restrict__ root,
                                                    unsigned long long* __restrict__ keys,
+                                                   unsigned long long* __restrict__ keys_lo,
                                                    int* __restrict__ idx) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     double c0 = root[0], c1 = root[1], c2 = root[2], hw = root[3];
     const double px = x[i], py = y[i], pz = z[i];
-    unsigned long long key = 0;
-    for (int l = 0; l < kLevels; ++l) {
+    unsigned long long key[2] = {0, 0};
+    const int levels = keys_lo ? kMaxLevels : kLevels;
+    for (int l = 0; l < levels; ++l) {
         const int ox = px >= c0 ? 1 : 0, oy = py >= c1 ? 1 : 0, oz = pz >= c2 ? 1 : 0;
-        key = (key << 3) | static_cast<unsigned long long>(ox + 2 * oy + 4 * oz);
+        const int w = l < kLevels ? 0 : 1;
+        key[w] = (key[w] << 3) | static_cast<unsigned long long>(ox + 2 * oy + 4 * oz);
         const double chw = hw * 0.5;
         c0 = c0 + (ox ? 1.0 : -1.0) * chw;
         c1 = c1 + (oy ? 1.0 : -1.0) * chw;
         c2 = c2 + (oz ? 1.0 : -1.0) * chw;
         hw = chw;
     }
-    keys[i] = key;
+    keys[i] = key[0];
+    if (keys_lo) keys_lo[i] = key[1];
     idx[i] = i;
 }
 
-// ---- 4. level split -------------------------------------------------------
-__device__ __forceinline__ int digit_at(unsigned long long key, int level /*1..kLevels*/) {
-    return static_cast<int>((key >> (3 * (kLevels - level))) & 7ull);
+// out[k] = in[perm[k]]
+template <typename T>
+__global__ __launch_bounds__(256) void permute_kernel(int n, const int* __restrict__ perm,
+                                                      const T* __restrict__ in, T* __restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) out[k] = in[perm[k]];
 }
 
-// Children of node k at level L: bounds[9] in sorted order, nonempty count.
+// Levels two keys share (0..21 on the first word; 21 + 0..11 with the second).
+__device__ __forceinline__ int shared_levels(unsigned long long a, unsigned long long b,
+                                             const unsigned long long* lo, int i, int j) {
+    const unsigned long long d = a ^ b;
+    if (d != 0) return kLevels - 1 - (63 - __clzll(static_cast<long long>(d))) / 3;
+    if (!lo) return kLevels;
+    const unsigned long long e = lo[i] ^ lo[j];
+    if (e != 0) return kMaxLevels - 1 - (63 - __clzll(static_cast<long long>(e))) / 3;
+    return kMaxLevels;
+}
+
+// The deepest level at which some node holds more than cap particles: the
+// most levels that sorted particles i and i + cap share (atomic max into
+// *out, which starts at -1).
+__global__ __launch_bounds__(256) void deepest_kernel(int n, int cap,
+                                                      const unsigned long long* __restrict__ keys,
+                                                      const unsigned long long* __restrict__ lo,
+                                                      int* __restrict__ out) {
+    int best = -1;
+    for (int i = blockIdx.x * 256 + threadIdx.x; static_cast<long long>(i) + cap < n;
+         i += gridDim.x * 256)
+        best = max(best, shared_levels(keys[i], keys[i + cap], lo, i, i + cap));
+    for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o, 64));
+    if ((threadIdx.x & 63) == 0 && best >= 0) atomicMax(out, best);
+}
+
+// ---- 4. level split -------------------------------------------------------
+// Digit of level 1..32 from the two key words.
+__device__ __forceinline__ int digit_at(const unsigned long long* keys,
+                                        const unsigned long long* lo, int i, int level) {
+    if (level <= kLevels) return static_cast<int>((keys[i] >> (3 * (kLevels - level))) & 7ull);
+    return static_cast<int>((lo[i] >> (3 * (kMaxLevels - level))) & 7ull);
+}
+
+// Children of the nodes of level L, nodes [lvl[L], lvl[L + 1]): bounds[9]
+// in sorted order and the non-empty count.  Launched over `bound` >= the
+// level's node count; entries past it get count 0, so the scan over
+// `bound` entries is the scan over the level.
 __global__ __launch_bounds__(256) void split_count_kernel(
-    int lb, int le, int level, int cap, int max_depth, const unsigned long long* __restrict__ keys,
+    int level, int bound, const int* __restrict__ lvl, int cap, int max_depth,
+    const unsigned long long* __restrict__ keys, const unsigned long long* __restrict__ keys_lo,
     const int* __restrict__ nb, const int* __restrict__ ne, int* __restrict__ bounds,
     int* __restrict__ count, int* __restrict__ overflow) {
-    const int k = lb + blockIdx.x * 256 + threadIdx.x;
-    if (k >= le) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= bound) return;
+    const int k = lvl[level] + j;
+    if (k >= lvl[level + 1]) {
+        count[j] = 0;
+        return;
+    }
     const int b = nb[k], e = ne[k];
-    const int j = k - lb;
     int* bd = bounds + 9 * j;
     const bool internal = (e - b) > cap && level < max_depth;
     if (!internal) {
         count[j] = 0;
         return;
     }
-    if (level >= kLevels) {
+    if (level >= kMaxLevels || (level >= kLevels && !keys_lo)) {  // excluded by the host
         atomicOr(overflow, 1);
         count[j] = 0;
         return;
@@ -192,7 +266,7 @@ __global__ __launch_bounds__(256) void split_count_kernel(
         int lo = b, hi = e;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (digit_at(keys[mid], level + 1) < d) lo = mid + 1;
+            if (digit_at(keys, keys_lo, mid, level + 1) < d) lo = mid + 1;
             else hi = mid;
         }
         bd[d] = lo;
@@ -202,15 +276,25 @@ __global__ __launch_bounds__(256) void split_count_kernel(
     count[j] = c;
 }
 
+// Emits level L + 1 (children in digit order, parents in order) at
+// lvl[L + 1] + offset, and its end lvl[L + 2].  Node slots beyond node_cap
+// (excluded by the host's bound) set overflow bit 2 instead of writing.
 __global__ __launch_bounds__(256) void split_emit_kernel(
-    int lb, int le, const int* __restrict__ bounds, const int* __restrict__ count,
-    const int* __restrict__ offset, double* __restrict__ geom, int* __restrict__ nb,
-    int* __restrict__ ne, int* __restrict__ first_child, int* __restrict__ nchild) {
+    int level, int bound, int* __restrict__ lvl, int node_cap, const int* __restrict__ bounds,
+    const int* __restrict__ count, const int* __restrict__ offset, double* __restrict__ geom,
+    int* __restrict__ nb, int* __restrict__ ne, int* __restrict__ first_child,
+    int* __restrict__ nchild, int* __restrict__ overflow) {
 #pragma clang fp contract(off)
-    const int k = lb + blockIdx.x * 256 + threadIdx.x;
-    if (k >= le) return;
-    const int j = k - lb;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= bound) return;
+    const int lb = lvl[level], le = lvl[level + 1];
+    const int k = lb + j;
+    if (k >= le) {
+        if (j == 0) lvl[level + 2] = le;  // empty level
+        return;
+    }
     const int c = count[j];
+    if (k == le - 1) lvl[level + 2] = le + offset[j] + c;
     nchild[k] = c;
     if (c == 0) {
         first_child[k] = -1;
@@ -218,6 +302,12 @@ __global__ __launch_bounds__(256) void split_emit_kernel(
     }
     const int* bd = bounds + 9 * j;
     int pos = le + offset[j];
+    if (pos + c > node_cap) {
+        atomicOr(overflow, 2);
+        nchild[k] = 0;
+        first_child[k] = -1;
+        return;
+    }
     first_child[k] = pos;
     const double chw = geom[4 * k + 3] * 0.5;
     for (int d = 0; d < 8; ++d) {
@@ -230,6 +320,15 @@ __global__ __launch_bounds__(256) void split_emit_kernel(
         ne[pos] = bd[d + 1];
         ++pos;
     }
+}
+
+// Internal nodes (the reference's node_count is 1 + 8 x this): one atomic
+// per wave into *out.
+__global__ __launch_bounds__(256) void internal_count_kernel(int nnodes, const int* __restrict__ nchild,
+                                                             unsigned long long* __restrict__ out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long m = __ballot(k < nnodes && nchild[k] > 0);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(out, static_cast<unsigned long long>(__popcll(m)));
 }
 
 // ---- 5. upward pass -------------------------------------------------------
@@ -1031,7 +1130,7 @@ __global__ __launch_bounds__(256) void gather_sorted_kernel(
 // children.  A leaf gives the pair forces to the lanes still carrying it.
 // Every target therefore sees exactly the reference's cells and leaves;
 // only the order of the sums differs.
-constexpr int kTreeStack = 256;  // > 21 levels x 7 siblings + 1
+constexpr int kTreeStack = 256;  // >= 31 levels x 7 siblings + 8 children (depth 32)
 
 // width / distance < theta on the reference's roundings (quadtree.cpp:
 // 222-228): decided from width^2 against theta^2 d2 when they differ by
@@ -1319,13 +1418,15 @@ struct FmmEngine::Impl {
     // particles
     size_t pcap = 0;
     unsigned long long *keys = nullptr, *keys_sorted = nullptr;
+    unsigned long long *keys_lo = nullptr, *keys_lo_sorted = nullptr;  // levels 22-32
     int *idx = nullptr, *idx_sorted = nullptr;
     double *sxs = nullptr, *sys = nullptr, *szs = nullptr, *sms = nullptr;  // key order
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     double* bbox_part = nullptr;
-    // nodes (one growth unit per array)
-    size_t ncap = 0;
+    // nodes: topology (geom, nb, ne, first_child, nchild) sized by the
+    // split's bound, moments (mcom, mom, kc64, kc32) by the node count
+    size_t ncap = 0, mcap = 0;
     double* geom = nullptr;  // 4 per node: cx, cy, cz, hw
     double* mcom = nullptr;  // 4 per node: mass, com
     double* mom = nullptr;   // 35 per node
@@ -1338,6 +1439,9 @@ struct FmmEngine::Impl {
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
     int* overflow = nullptr;
+    int* lvl = nullptr;       // kMaxLevels + 4 level offsets (device)
+    int* deepest = nullptr;   // deepest level holding more than leaf_capacity particles
+    double* geom_root = nullptr;  // root cube (centre, half width)
     // leaves and lists
     size_t leafcap = 0;
     int *flag = nullptr, *flag_pos = nullptr, *leaves = nullptr;
@@ -1374,9 +1478,9 @@ struct FmmEngine::Impl {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
     ~Impl() {
-        void* ptrs[] = {keys, keys_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
+        void* ptrs[] = {keys, keys_sorted, keys_lo, keys_lo_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
                         kc64, kc32, nb, ne, first_child, nchild, bounds, count, offset,
-                        scan_tmp, overflow, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
+                        scan_tmp, overflow, lvl, deepest, geom_root, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
                         near_off, frontier, far, near, stash_far, stash_near, lpos_far, lpos_near, wtot, icount, ioff, row_off, items, npart, stats_dev};
         for (void* p : ptrs) (void)hipFree(p);
         if (tree_counts) (void)hipHostFree(tree_counts);
@@ -1405,21 +1509,32 @@ struct FmmEngine::Impl {
         return e;
     }
 
-    // Grows every node array to `nodes` entries keeping the first `keep`
-    // (geometry and topology; mass/moments are rewritten by the upward pass).
-    hipError_t grow_nodes(size_t nodes, size_t keep, hipStream_t st) {
+    // Topology arrays for `nodes` nodes (contents not kept: every build
+    // writes them from the root down).
+    hipError_t grow_topology(size_t nodes, hipStream_t st) {
+        if (nodes <= ncap) return hipSuccess;
         size_t c;
         hipError_t e;
-        c = 4 * ncap; if ((e = grow(&geom, &c, 4 * nodes, 4 * keep, st)) != hipSuccess) return e;
-        c = 4 * ncap; if ((e = grow(&mcom, &c, 4 * nodes, 0, st)) != hipSuccess) return e;
-        c = kMom * ncap; if ((e = grow(&mom, &c, kMom * nodes, 0, st)) != hipSuccess) return e;
-        c = kFarCoef * ncap; if ((e = grow(&kc64, &c, kFarCoef * nodes, 0, st)) != hipSuccess) return e;
-        c = kFarCoef * ncap; if ((e = grow(&kc32, &c, kFarCoef * nodes, 0, st)) != hipSuccess) return e;
-        c = ncap; if ((e = grow(&nb, &c, nodes, keep, st)) != hipSuccess) return e;
-        c = ncap; if ((e = grow(&ne, &c, nodes, keep, st)) != hipSuccess) return e;
-        c = ncap; if ((e = grow(&first_child, &c, nodes, keep, st)) != hipSuccess) return e;
-        c = ncap; if ((e = grow(&nchild, &c, nodes, keep, st)) != hipSuccess) return e;
-        ncap = nodes;
+        c = 4 * ncap; if ((e = grow(&geom, &c, 4 * nodes, 0, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&nb, &c, nodes, 0, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&ne, &c, nodes, 0, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&first_child, &c, nodes, 0, st)) != hipSuccess) return e;
+        c = ncap; if ((e = grow(&nchild, &c, nodes, 0, st)) != hipSuccess) return e;
+        ncap = std::max(nodes, 2 * ncap);  // grow() doubles the same way
+        return hipSuccess;
+    }
+
+    // Mass, moment and coefficient arrays for `nodes` nodes (rewritten by
+    // every upward pass).
+    hipError_t grow_moments(size_t nodes, hipStream_t st) {
+        if (nodes <= mcap) return hipSuccess;
+        size_t c;
+        hipError_t e;
+        c = 4 * mcap; if ((e = grow(&mcom, &c, 4 * nodes, 0, st)) != hipSuccess) return e;
+        c = kMom * mcap; if ((e = grow(&mom, &c, kMom * nodes, 0, st)) != hipSuccess) return e;
+        c = kFarCoef * mcap; if ((e = grow(&kc64, &c, kFarCoef * nodes, 0, st)) != hipSuccess) return e;
+        c = kFarCoef * mcap; if ((e = grow(&kc32, &c, kFarCoef * nodes, 0, st)) != hipSuccess) return e;
+        mcap = std::max(nodes, 2 * mcap);
         return hipSuccess;
     }
 
@@ -1441,6 +1556,10 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
                            std::string* err) {
     const int n = static_cast<int>(s.n);
     const int cap = std::max(1, p.leaf_capacity);
+    if (p.max_depth > kMaxLevels) {
+        if (err) *err = "max_depth above 32 (the reference's depth cap; keys hold 32 levels)";
+        return GC_ERR_UNSUPPORTED;
+    }
     const int max_depth = std::max(1, p.max_depth);
     const double theta = std::max(1.0e-6, p.theta);
     const int bblocks = std::min(1024, (n + 255) / 256);
@@ -1448,7 +1567,8 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     // particle buffers: all of them, or none (pcap stays 0 until the last
     // allocation succeeds, so a failed build allocates again next time)
     if (static_cast<size_t>(n) > pcap) {
-        void** ptrs[] = {(void**)&keys, (void**)&keys_sorted, (void**)&idx, (void**)&idx_sorted,
+        void** ptrs[] = {(void**)&keys, (void**)&keys_sorted, (void**)&keys_lo,
+                         (void**)&keys_lo_sorted, (void**)&idx, (void**)&idx_sorted,
                          (void**)&sxs, (void**)&sys, (void**)&szs, (void**)&sms, &sort_tmp,
                          (void**)&bbox_part};
         for (void** q : ptrs) {
@@ -1463,97 +1583,144 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         FMM_HIP(hipMalloc(&sms, sizeof(double) * np), "allocate sorted state");
         FMM_HIP(hipMalloc(&keys, sizeof(unsigned long long) * np), "allocate keys");
         FMM_HIP(hipMalloc(&keys_sorted, sizeof(unsigned long long) * np), "allocate keys");
+        FMM_HIP(hipMalloc(&keys_lo, sizeof(unsigned long long) * np), "allocate keys");
+        FMM_HIP(hipMalloc(&keys_lo_sorted, sizeof(unsigned long long) * np), "allocate keys");
         FMM_HIP(hipMalloc(&idx, sizeof(int) * np), "allocate indices");
         FMM_HIP(hipMalloc(&idx_sorted, sizeof(int) * np), "allocate indices");
         FMM_HIP(hipMalloc(&bbox_part, sizeof(double) * 6 * 1024), "allocate bbox partials");
         sort_tmp_bytes = 0;
-        size_t sort_bytes = 0;
+        size_t sort_bytes = 0, sort_bytes_lo = 0;
         FMM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys, keys_sorted, idx,
                                                    idx_sorted, n, 0, 3 * kLevels, st),
                 "size radix sort");
+        FMM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes_lo, keys_lo, keys_lo_sorted,
+                                                   idx, idx_sorted, n, 0, 3 * kDeepLevels, st),
+                "size radix sort");
+        sort_bytes = std::max(sort_bytes, sort_bytes_lo);
         FMM_HIP(hipMalloc(&sort_tmp, sort_bytes), "allocate radix sort storage");
         sort_tmp_bytes = sort_bytes;
         if (!overflow) FMM_HIP(hipMalloc(&overflow, sizeof(int)), "allocate flag");
-        if (!stats_dev) FMM_HIP(hipMalloc(&stats_dev, sizeof(unsigned long long) * 4), "allocate stats");
+        if (!lvl) FMM_HIP(hipMalloc(&lvl, sizeof(int) * (kMaxLevels + 4)), "allocate level offsets");
+        if (!deepest) FMM_HIP(hipMalloc(&deepest, sizeof(int)), "allocate depth");
+        if (!geom_root) FMM_HIP(hipMalloc(&geom_root, sizeof(double) * 4), "allocate root");
+        if (!stats_dev) FMM_HIP(hipMalloc(&stats_dev, sizeof(unsigned long long) * kStats), "allocate stats");
         if (!tree_counts)
-            FMM_HIP(hipHostMalloc(&tree_counts, sizeof(unsigned long long) * 4), "allocate counts");
+            FMM_HIP(hipHostMalloc(&tree_counts, sizeof(unsigned long long) * kStats), "allocate counts");
         pcap = np;
     }
-    const size_t need_nodes = std::max<size_t>(1024, 2 * static_cast<size_t>(n) / cap + 64);
-    if (need_nodes > ncap) FMM_HIP(grow_nodes(need_nodes, 0, st), "allocate nodes");
-
-    // 1-3: root, keys, sort
+    // 1-3: root, keys, sort on the first key word
+    const bool deep_keys = max_depth > kLevels;
     hipLaunchKernelGGL(bbox_partial_kernel, dim3(bblocks), dim3(256), 0, st, n, s.x, s.y, s.z,
                        bbox_part);
     hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(64), 0, st, bblocks, bbox_part,
-                       p.softening, n, geom, nb, ne);
+                       p.softening, geom_root, lvl);
     hipLaunchKernelGGL(keys_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, s.x, s.y, s.z,
-                       geom, keys, idx);
+                       geom_root, keys, deep_keys ? keys_lo : nullptr, idx);
     FMM_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_tmp_bytes, keys, keys_sorted, idx,
                                                idx_sorted, n, 0, 3 * kLevels, st),
             "radix sort keys");
+    const unsigned dblocks = static_cast<unsigned>(std::min(1024, (n + 255) / 256));
+    FMM_HIP(hipMemsetAsync(deepest, 0xFF, sizeof(int), st), "clear depth");
+    hipLaunchKernelGGL(deepest_kernel, dim3(dblocks), dim3(256), 0, st, n, cap, keys_sorted,
+                       nullptr, deepest);
+    int D = -1;
+    FMM_HIP(hipMemcpyAsync(&D, deepest, sizeof(int), hipMemcpyDeviceToHost, st), "read depth");
+    FMM_HIP(hipStreamSynchronize(st), "tree depth");
+    const unsigned pblocks = static_cast<unsigned>((n + 255) / 256);
+    const bool deep = deep_keys && D >= kLevels;
+    if (deep) {
+        // more than leaf_capacity particles share all 21 levels of the first
+        // word: order by both words (stable sort on the second, then on the
+        // first), and take the depth from both
+        FMM_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_tmp_bytes, keys_lo, keys_lo_sorted,
+                                                   idx, idx_sorted, n, 0, 3 * kDeepLevels, st),
+                "radix sort deep keys");
+        hipLaunchKernelGGL(permute_kernel<unsigned long long>, dim3(pblocks), dim3(256), 0, st, n,
+                           idx_sorted, keys, keys_sorted);
+        FMM_HIP(hipMemcpyAsync(idx, idx_sorted, sizeof(int) * n, hipMemcpyDeviceToDevice, st),
+                "deep key order");
+        FMM_HIP(hipcub::DeviceRadixSort::SortPairs(sort_tmp, sort_tmp_bytes, keys_sorted, keys, idx,
+                                                   idx_sorted, n, 0, 3 * kLevels, st),
+                "radix sort keys");
+        std::swap(keys, keys_sorted);
+        hipLaunchKernelGGL(permute_kernel<unsigned long long>, dim3(pblocks), dim3(256), 0, st, n,
+                           idx_sorted, keys_lo, keys_lo_sorted);
+        FMM_HIP(hipMemsetAsync(deepest, 0xFF, sizeof(int), st), "clear depth");
+        hipLaunchKernelGGL(deepest_kernel, dim3(dblocks), dim3(256), 0, st, n, cap, keys_sorted,
+                           keys_lo_sorted, deepest);
+        FMM_HIP(hipMemcpyAsync(&D, deepest, sizeof(int), hipMemcpyDeviceToHost, st), "read depth");
+        FMM_HIP(hipStreamSynchronize(st), "tree depth");
+    }
     FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
-    hipLaunchKernelGGL(gather_sorted_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, idx_sorted,
+    hipLaunchKernelGGL(gather_sorted_kernel, dim3(pblocks), dim3(256), 0, st, n, idx_sorted,
                        s.x, s.y, s.z, s.m, sxs, sys, szs, sms);
 
-    // 4: levels
-    level_begin.assign(1, 0);
-    level_begin.push_back(1);
-    int internal = 0;
-    for (int level = 0;; ++level) {
-        const int lb = level_begin[level], le = level_begin[level + 1];
-        const int cnt = le - lb;
-        if (cnt == 0) break;
-        if (static_cast<size_t>(cnt) > lcap) {
-            void** ptrs[] = {(void**)&bounds, (void**)&count, (void**)&offset, &scan_tmp};
-            for (void** q : ptrs) {
-                (void)hipFree(*q);
-                *q = nullptr;
-            }
-            lcap = 0;
-            scan_tmp_bytes = 0;
-            const size_t want = static_cast<size_t>(cnt) * 2;
-            FMM_HIP(hipMalloc(&bounds, sizeof(int) * 9 * want), "allocate level scratch");
-            FMM_HIP(hipMalloc(&count, sizeof(int) * want), "allocate level scratch");
-            FMM_HIP(hipMalloc(&offset, sizeof(int) * want), "allocate level scratch");
-            size_t scan_bytes = 0;
-            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count, offset,
-                                                     static_cast<int>(want), st),
-                    "size scan");
-            FMM_HIP(hipMalloc(&scan_tmp, scan_bytes), "allocate scan storage");
-            scan_tmp_bytes = scan_bytes;
-            lcap = want;
-        }
-        const unsigned blocks = static_cast<unsigned>((cnt + 255) / 256);
-        hipLaunchKernelGGL(split_count_kernel, dim3(blocks), dim3(256), 0, st, lb, le, level, cap,
-                           max_depth, keys_sorted, nb, ne, bounds, count, overflow);
-        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, count, offset, cnt, st),
-                "scan children");
-        int last[2] = {0, 0};
-        FMM_HIP(hipMemcpyAsync(&last[0], offset + cnt - 1, sizeof(int), hipMemcpyDeviceToHost, st),
-                "read level size");
-        FMM_HIP(hipMemcpyAsync(&last[1], count + cnt - 1, sizeof(int), hipMemcpyDeviceToHost, st),
-                "read level size");
-        FMM_HIP(hipStreamSynchronize(st), "level size");
-        const int next = last[0] + last[1];
-        const size_t need = static_cast<size_t>(le) + next;
-        if (need > ncap)
-            FMM_HIP(grow_nodes(std::max(need, ncap * 2), static_cast<size_t>(le), st), "grow nodes");
-        hipLaunchKernelGGL(split_emit_kernel, dim3(blocks), dim3(256), 0, st, lb, le, bounds, count,
-                           offset, geom, nb, ne, first_child, nchild);
-        level_begin.push_back(le + next);
+    // 4: levels.  Levels 0..lmax hold internal nodes; level lmax + 1 only
+    // leaves (it is split too, to mark them).  A level's node count is at
+    // most 8 x its parents' internal nodes, each holding > cap particles,
+    // and at most n: the bound each level's launch covers.
+    const int lmax = D < 0 ? -1 : std::min(D, max_depth - 1);
+    const int nlev = lmax + 2;  // levels split: 0 .. lmax + 1
+    std::vector<long long> bound(static_cast<size_t>(nlev) + 1, 0);
+    bound[0] = 1;
+    const long long max_internal = static_cast<long long>(n) / (static_cast<long long>(cap) + 1);
+    long long total_bound = 1, level_bound = 1;
+    for (int L = 0; L + 1 < nlev; ++L) {
+        bound[L + 1] = std::min<long long>(n, 8 * std::min(max_internal, bound[L]));
+        total_bound += bound[L + 1];
+        level_bound = std::max(level_bound, bound[L + 1]);
     }
+    FMM_HIP(grow_topology(static_cast<size_t>(total_bound), st), "allocate nodes");
+    hipLaunchKernelGGL(root_kernel, dim3(1), dim3(1), 0, st, n, geom_root, geom, nb, ne);
+    if (static_cast<size_t>(level_bound) > lcap) {
+        void** ptrs[] = {(void**)&bounds, (void**)&count, (void**)&offset, &scan_tmp};
+        for (void** q : ptrs) {
+            (void)hipFree(*q);
+            *q = nullptr;
+        }
+        lcap = 0;
+        scan_tmp_bytes = 0;
+        const size_t want = static_cast<size_t>(level_bound);
+        FMM_HIP(hipMalloc(&bounds, sizeof(int) * 9 * want), "allocate level scratch");
+        FMM_HIP(hipMalloc(&count, sizeof(int) * want), "allocate level scratch");
+        FMM_HIP(hipMalloc(&offset, sizeof(int) * want), "allocate level scratch");
+        size_t scan_bytes = 0;
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count, offset,
+                                                 static_cast<int>(want), st),
+                "size scan");
+        FMM_HIP(hipMalloc(&scan_tmp, scan_bytes), "allocate scan storage");
+        scan_tmp_bytes = scan_bytes;
+        lcap = want;
+    }
+    const unsigned long long* klo = deep ? keys_lo_sorted : nullptr;
+    for (int L = 0; L < nlev; ++L) {
+        const int bl = static_cast<int>(bound[L]);
+        const unsigned blocks = static_cast<unsigned>((bl + 255) / 256);
+        hipLaunchKernelGGL(split_count_kernel, dim3(blocks), dim3(256), 0, st, L, bl, lvl, cap,
+                           max_depth, keys_sorted, klo, nb, ne, bounds, count, overflow);
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, count, offset, bl, st),
+                "scan children");
+        hipLaunchKernelGGL(split_emit_kernel, dim3(blocks), dim3(256), 0, st, L, bl, lvl,
+                           static_cast<int>(std::min<size_t>(ncap, INT_MAX)), bounds, count,
+                           offset, geom, nb, ne,
+                           first_child, nchild, overflow);
+    }
+    std::vector<int> lv(static_cast<size_t>(nlev) + 2, 0);
     int ovf = 0;
+    FMM_HIP(hipMemcpyAsync(lv.data(), lvl, sizeof(int) * lv.size(), hipMemcpyDeviceToHost, st),
+            "read level offsets");
     FMM_HIP(hipMemcpyAsync(&ovf, overflow, sizeof(int), hipMemcpyDeviceToHost, st), "read flag");
     FMM_HIP(hipStreamSynchronize(st), "tree build");
     if (ovf) {
-        if (err)
-            *err = "octree deeper than 21 levels (coincident or extremely clustered particles); "
-                   "lower max_depth or raise leaf_capacity";
-        return GC_ERR_UNSUPPORTED;
+        if (err) *err = "octree split exceeded its key levels or node bound (internal error)";
+        return GC_ERR_STATE;
     }
+    level_begin.assign(lv.begin(), lv.end());  // level L: [level_begin[L], level_begin[L + 1])
     nnodes = level_begin.back();
-    const int depth = static_cast<int>(level_begin.size()) - 3;  // deepest non-empty level
+    int depth = 0;  // deepest non-empty level
+    for (int L = 0; L + 1 < static_cast<int>(level_begin.size()); ++L)
+        if (level_begin[L + 1] > level_begin[L]) depth = L;
+    FMM_HIP(grow_moments(static_cast<size_t>(nnodes), st), "allocate node moments");
 
     // 5: upward, deepest level first
     for (int level = depth; level >= 0; --level) {
@@ -1564,17 +1731,15 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     }
     hipLaunchKernelGGL(far_coef_kernel, dim3((nnodes + 63) / 64), dim3(64), 0, st, nnodes, mcom, mom,
                        kc64, kc32);
+    // internal nodes, counted on the device (read with the other statistics)
+    FMM_HIP(hipMemsetAsync(stats_dev + kStatInternal, 0, sizeof(unsigned long long), st),
+            "clear counts");
+    hipLaunchKernelGGL(internal_count_kernel, dim3((nnodes + 255) / 256), dim3(256), 0, st, nnodes,
+                       nchild, stats_dev + kStatInternal);
     if (p.solver == GC_SOLVER_TREE) {
-        // the Barnes-Hut walk needs no lists; its work counts come from the
-        // evaluation (tree_stats)
-        std::vector<int> nch(static_cast<size_t>(nnodes));
-        FMM_HIP(hipMemcpyAsync(nch.data(), nchild, sizeof(int) * nnodes, hipMemcpyDeviceToHost, st),
-                "read tree");
-        FMM_HIP(hipStreamSynchronize(st), "tree");
-        internal = 0;
-        for (int v : nch) internal += v > 0 ? 1 : 0;
+        // the Barnes-Hut walk needs no lists; its work counts and the node
+        // count come with the evaluation (stats())
         last = gc_fmm_stats{};
-        last.node_count = 1 + 8 * static_cast<size_t>(internal);
         last.max_depth = depth;
         last.tree_nodes = static_cast<size_t>(nnodes);
         nleaves = nnodes;  // evaluation grid: every node, leaves holding particles act
@@ -1765,14 +1930,10 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
     hipLaunchKernelGGL(stats_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves, leaves,
                        nb, ne, nfar, nnear, nsrc, stats_dev);
-    unsigned long long sv[4];
+    unsigned long long sv[kStatInternal + 1];
     FMM_HIP(hipMemcpyAsync(sv, stats_dev, sizeof(sv), hipMemcpyDeviceToHost, st), "read stats");
-    std::vector<int> nch(static_cast<size_t>(nnodes));
-    FMM_HIP(hipMemcpyAsync(nch.data(), nchild, sizeof(int) * nnodes, hipMemcpyDeviceToHost, st), "read tree");
     FMM_HIP(hipStreamSynchronize(st), "stats");
-    internal = 0;
-    for (int v : nch) internal += v > 0 ? 1 : 0;
-    last.node_count = 1 + 8 * static_cast<size_t>(internal);
+    last.node_count = 1 + 8 * static_cast<size_t>(sv[kStatInternal]);
     last.leaf_count = static_cast<size_t>(nleaves);
     last.far_interactions = sv[0];
     last.near_interactions = sv[1];
@@ -1821,7 +1982,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
 #undef GC_TREE_O
 #undef GC_TREE
         FMM_HIP(hipGetLastError(), "launch tree evaluation");
-        FMM_HIP(hipMemcpyAsync(tree_counts, stats_dev, sizeof(unsigned long long) * 3,
+        FMM_HIP(hipMemcpyAsync(tree_counts, stats_dev, sizeof(unsigned long long) * (kStatInternal + 1),
                                hipMemcpyDeviceToHost, st), "tree counts");
         tree_counts_pending = true;
         return GC_OK;
@@ -1864,6 +2025,7 @@ const gc_fmm_stats& FmmEngine::stats() {
         impl_->last.m2p_evaluations = impl_->tree_counts[0];
         impl_->last.p2p_pairs = impl_->tree_counts[1];
         impl_->last.leaf_count = impl_->tree_counts[2];
+        impl_->last.node_count = 1 + 8 * static_cast<size_t>(impl_->tree_counts[kStatInternal]);
         impl_->tree_counts_pending = false;
     }
     return impl_->last;

@@ -87,21 +87,58 @@ def test_fmm_kdk_steps_track_golden(gpu):
     assert rel_err(acc, a20).max() < 1e-6
 
 
-def test_depth_beyond_key_is_reported(gpu):
+def _deep_cases():
+    """Inputs whose trees go below 21 levels, up to the depth cap of 32."""
+    rng = np.random.default_rng(6)
+    # the oracle's own depth-cap case: 40 coincident particles in a cloud
+    # with a dense core (cpp/fmm/fmm_solver.cpp:136-155 stops at depth 32)
+    pos = rng.normal(size=(200, 3))
+    pos[:50] *= 0.05
+    pos[:40] = pos[0]
+    yield "coincident40", pos, rng.uniform(0.1, 1.0, 200), 8
+    # two clumps of 50 coincident particles
     pos = np.zeros((100, 3))
-    pos[50:] = 1.0  # 50 coincident particles cannot be separated by any split
-    m = np.ones(100)
-    with pytest.raises(gpu.GcError) as e:
-        _fmm(gpu, "mixed", pos, m, 0.02, 0.5, 8)
-    assert "21 levels" in str(e.value)
-    # with a depth cap the reference keeps them in one leaf: supported
-    with gpu.Engine("hip-fmm", "fp64", softening=0.02, theta=0.5, leaf_capacity=8,
+    pos[50:] = 1.0
+    yield "coincident50x2", pos, np.ones(100), 8
+    # a cluster resolved only by levels 22-31 (spread 1e-8 in a unit cube):
+    # the second key word's digits decide the splits
+    pos = rng.normal(size=(600, 3))
+    pos[:120] = pos[0] + rng.normal(size=(120, 3)) * 1e-8
+    yield "cluster1e-8", pos, rng.uniform(0.1, 1.0, 600), 4
+
+
+@pytest.mark.parametrize("case", ["coincident40", "coincident50x2", "cluster1e-8"])
+def test_depth_up_to_32_levels(gpu, case):
+    name, pos, m, leaf = next(c for c in _deep_cases() if c[0] == case)
+    want, st = O.fmm(pos, m, 1.0, 0.02, 0.5, leaf, 32, 4)
+    assert st["max_depth_reached"] > 21
+    a64, s64 = _fmm(gpu, "fp64", pos, m, 0.02, 0.5, leaf)
+    for k in STAT_KEYS:
+        assert s64[k] == st[k], (name, k, s64[k], st[k])
+    assert s64["max_depth"] == st["max_depth_reached"]
+    assert rel_err(a64, want).max() < FP64_TOL
+    a32, s32 = _fmm(gpu, "mixed", pos, m, 0.02, 0.5, leaf)
+    assert s32["node_count"] == st["node_count"]
+    assert rel_err(a32, want).max() < MIXED_TOL
+    # a shallower cap keeps the clumps in one leaf, as the reference does
+    want6, st6 = O.fmm(pos, m, 1.0, 0.02, 0.5, leaf, 6, 4)
+    with gpu.Engine("hip-fmm", "fp64", softening=0.02, theta=0.5, leaf_capacity=leaf,
                     max_depth=6) as eng:
         eng.upload(pos, None, m)
         eng.compute_accelerations()
         got = eng.download()[2]
-    want, _ = O.fmm(pos, m, 1.0, 0.02, 0.5, 8, 6, 4)
-    assert rel_err(got, want).max() < FP64_TOL
+        s6 = eng.fmm_stats()
+    assert s6["node_count"] == st6["node_count"] and s6["max_depth"] == 6
+    assert rel_err(got, want6).max() < FP64_TOL
+
+
+def test_depth_cap_above_32_is_rejected(gpu):
+    pos, _, m = gpu.generators.plummer(500, seed=3)
+    with pytest.raises(gpu.GcError) as e:
+        with gpu.Engine("hip-fmm", "fp64", softening=0.02, max_depth=33) as eng:
+            eng.upload(pos, None, m)
+            eng.compute_accelerations()
+    assert "max_depth above 32" in str(e.value)
 
 
 def test_plummer_full_size_sampled(gpu):

@@ -101,3 +101,25 @@ def test_plummer_full_size_sampled(gpu):
     direct = O.direct_targets(pos, m, 1.0, 0.01, 0, k, threads=0)
     err = rel_err(got[:k], direct[:k])
     assert np.median(err) < 2e-4 and err.max() < 1e-2
+
+
+@pytest.mark.parametrize("leaf,eps", [(4, 0.0), (8, 0.02)])
+def test_duplicates_at_depth_cap(gpu, leaf, eps):
+    # the oracle's depth-cap case (pinned against the compiled reference):
+    # 40 coincident particles reach depth 32; with eps = 0 their pairs are
+    # skipped (s2 == 0) and every other pair counts
+    rng = np.random.default_rng(8)
+    pos = rng.normal(size=(300, 3))
+    pos[:75] *= 0.05
+    pos[:40] = pos[0]
+    m = rng.uniform(0.1, 1.0, 300)
+    want, st = O.tree(pos, m, 1.0, eps, 0.6, leaf, 32, 4)
+    assert st["max_depth_reached"] == 32
+    a64, s64 = _tree(gpu, "fp64", pos, m, eps, 0.6, leaf)
+    for k in ("node_count", "leaf_count", "m2p_evaluations", "p2p_pairs"):
+        assert s64[k] == st[k], (k, s64[k], st[k])
+    assert s64["max_depth"] == 32
+    assert rel_err(a64, want).max() < FP64_TOL
+    a32, s32 = _tree(gpu, "mixed", pos, m, eps, 0.6, leaf)
+    assert s32["p2p_pairs"] == st["p2p_pairs"]
+    assert rel_err(a32, want).max() < MIXED_TOL
