@@ -203,7 +203,8 @@ constexpr int kVjpSymIpt = 2;
 
 gc_status ensure_sym(gc_ctx* c) {
     std::string msg;
-    const int rc = sym_prepare(c->sym, static_cast<int>(c->mixed.ntiles), c->stream, &msg);
+    const int rc = sym_prepare(c->sym, static_cast<int>(c->mixed.ntiles), c->shape.items,
+                               c->stream, &msg);
     if (rc == 0) return GC_OK;
     return fail(c, rc == 2 ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP, msg);
 }
@@ -951,6 +952,16 @@ gc_status gc_get_launch_shape(gc_ctx* c, int* ipt, int* jsplit) {
     mixed_shape(c, c->s.n, ipt, jsplit);
     return GC_OK;
     GC_ABI_END("gc_get_launch_shape")
+}
+
+gc_status gc_set_pair_items(gc_ctx* c, int tiles) {
+    GC_ABI_BEGIN
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (tiles < 0 || tiles > kSymMaxTiles)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "source tiles per item out of range");
+    c->shape.items = tiles;
+    return GC_OK;
+    GC_ABI_END("gc_set_pair_items")
 }
 
 gc_status gc_set_sources(gc_ctx* c, int mode) {

@@ -622,14 +622,16 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
     return 0;
 }
 
-int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err) {
-    if (ntiles == plan.ntiles && plan.items) return 0;
+int sym_prepare(SymPlan& plan, int ntiles, int len, hipStream_t st, std::string* err) {
+    if (ntiles == plan.ntiles && len == plan.len && plan.items) return 0;
     if (ntiles < 1 || ntiles > kSymMaxTiles) {
         *err = "symmetric form: tile count out of range";
         return 1;
     }
     const int t0 = 0;
-    return sym_prepare_runs(plan, &t0, &ntiles, 1, 0, st, err);
+    const int rc = sym_prepare_runs(plan, &t0, &ntiles, 1, len, st, err);
+    if (rc == 0) plan.len = len;
+    return rc;
 }
 
 void sym_release(SymPlan& plan) {

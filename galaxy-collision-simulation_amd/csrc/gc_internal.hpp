@@ -46,6 +46,7 @@ struct MixedLayout {
 constexpr int kSymMaxTiles = 1024;  // N <= 262,144: pair partials <= 1.6 GB
 struct SymPlan {
     int ntiles = 0;             // plan built for this tile count
+    int len = 0;                // ... and this item length (0: the default)
     int nitems = 0;
     SymItem* items = nullptr;   // longest first
     int* row_start = nullptr;   // ntiles + 1 slot offsets; a row's slots in J order
@@ -61,6 +62,7 @@ struct LaunchShape {
     int jsplit = 0;   // source-range splits; 0 = automatic
     int sources = 0;  // GC_SOURCES_*: 0 automatic, 1 LDS tiles, 2 scalar loads
     int pairs = 0;    // GC_PAIRS_*: 0 automatic, 1 per target, 2 each pair once
+    int items = 0;    // symmetric form: source tiles per work item; 0 = automatic
 };
 
 // One model over a run of particles for the device generator (ic.hip).
@@ -88,7 +90,8 @@ void launch_direct_mixed(const DeviceState& s, MixedLayout& L, double eps, int i
 void launch_merge_mixed(const DeviceState& s, const MixedLayout& L, int jsplit, size_t begin,
                         size_t end, bool close_kick, double half_dt, hipStream_t st);
 // direct_sym.hip: 0 on success, 2 out of memory, 1/3 other errors (err set)
-int sym_prepare(SymPlan& plan, int ntiles, hipStream_t st, std::string* err);
+// len: source tiles per work item (0: the default of sym_plan.hpp)
+int sym_prepare(SymPlan& plan, int ntiles, int len, hipStream_t st, std::string* err);
 // independent runs of tiles [run_t0[r], run_t0[r] + run_nt[r]) (pairs only
 // within a run)
 // len: J tiles per item (0: the default of sym_plan.hpp)

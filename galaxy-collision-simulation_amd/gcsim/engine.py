@@ -148,6 +148,10 @@ class Engine:
         all sources) or "symmetric" (each unordered pair once)."""
         self._check(self._lib.gc_set_pairs(self._ctx, PAIRS[mode]), "set pairs")
 
+    def set_pair_items(self, tiles=0):
+        """Source tiles per work item of the symmetric form (0 = automatic)."""
+        self._check(self._lib.gc_set_pair_items(self._ctx, int(tiles)), "set pair items")
+
     def pairs(self) -> str:
         return {L.GC_PAIRS_TARGETS: "targets",
                 L.GC_PAIRS_SYMMETRIC: "symmetric"}[self._lib.gc_get_pairs(self._ctx)]

@@ -300,6 +300,13 @@ int gc_get_sources(const gc_ctx* ctx);
 enum { GC_PAIRS_AUTO = 0, GC_PAIRS_TARGETS = 1, GC_PAIRS_SYMMETRIC = 2 };
 gc_status gc_set_pairs(gc_ctx* ctx, int mode);
 int gc_get_pairs(const gc_ctx* ctx);
+/* Tuning: source tiles (of 256 particles) per work item of the SYMMETRIC
+ * form, 0..1024 (0, the default: about 16,384 items, at least one tile
+ * each).  The same quantity as a symmetric batch's chunk_tiles
+ * (gc_batch_set_schedule), so a run of a batch and the same run on a
+ * context give bit-identical sums when both use the same item length and
+ * targets per lane.  Sums of different item lengths differ by rounding. */
+gc_status gc_set_pair_items(gc_ctx* ctx, int tiles);
 
 /* The same choice for a batch: SYMMETRIC covers each unordered pair of a
  * run once (runs of up to 262,144 particles; AUTO is SYMMETRIC when every

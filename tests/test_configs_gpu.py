@@ -1,0 +1,207 @@
+"""The benchmark workloads at their configured shapes (configs/*.toml), on
+the GPU, against the FP64 oracle path.
+
+C1: two-disk N = 65,536, eps 0.02, dt 5e-4, 1000 KDK steps, FP64 and mixed.
+    The mixed run's relative energy drift stays within 2x of the FP64 run's
+    (the FP64 device path is bit-identical to the oracle's targets form),
+    and every 50 steps the mixed forces of the mixed trajectory's state are
+    within 1e-5 (max relative) of FP64 forces of the same state.
+C3: two-disk N = 16,384, eps 0.05, dt 1e-3, 200 KDK steps, checkpoints
+    every 20, loss = mean |x(T) - x_target|^2 with targets from a run whose
+    initial velocities carry N(0, 0.005): gradients in (v0, m) by the FP64
+    adjoint and the mixed (FP32 force) adjoint.  The FP64 adjoint matches
+    FP64 central differences (h = 1e-6) of the FP64 forward along 16 random
+    directions to 1e-5; the mixed gradients are within 1e-5 (relative norm)
+    of the FP64 ones and within 1e-3 of the differences per direction.  At
+    N = 1024 the same 200 steps against an FP64 torch autograd restatement
+    of the leapfrog.
+C4: 256 collisions, N log-uniform in [512, 8192], eps 0.05, dt 1e-3, under
+    the autotuned schedule with 10-step graphs: every run's forces within
+    1e-5 of the oracle's, and runs stepped 20 steps in the batch
+    bit-identical to the same runs stepped alone with the same pair form,
+    item length and targets per lane.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-5  # max relative force error, mixed vs FP64
+
+
+def test_c1_energy_drift_and_force_error_over_1000_steps(gpu):
+    n, eps, dt, steps, every = 65536, 0.02, 5e-4, 1000, 50
+    pos, vel, mass = gpu.generators.config1(n=n)
+    engines = {p: gpu.Engine("hip-direct", p, softening=eps) for p in ("fp64", "mixed")}
+    probe = gpu.Engine("hip-direct", "fp64", softening=eps)
+    energy = {p: [] for p in engines}
+    worst = 0.0
+    try:
+        for e in engines.values():
+            e.upload(pos, vel, mass)
+            e.compute_accelerations()
+        for k in range(0, steps + 1, every):
+            if k:
+                for e in engines.values():
+                    e.step(dt, every)
+            for p, e in engines.items():
+                energy[p].append(e.diagnostics()["total_energy"])
+            pm, vm, am = engines["mixed"].download()
+            probe.upload(pm, vm, mass)
+            probe.compute_accelerations()
+            err = rel_err(am, probe.download()[2]).max()
+            worst = max(worst, err)
+            assert err <= MIXED_TOL, (k, err)
+            print(f"C1 step {k}: mixed force err {err:.2e}", flush=True)
+    finally:
+        for e in list(engines.values()) + [probe]:
+            e.close()
+    drift = {}
+    for p, s in energy.items():
+        s = np.array(s)
+        drift[p] = (np.abs(s - s[0]) / abs(s[0])).max()
+    print(f"C1 drift fp64 {drift['fp64']:.3e} mixed {drift['mixed']:.3e} "
+          f"max force err {worst:.2e}", flush=True)
+    assert drift["fp64"] > 0
+    assert drift["mixed"] <= 2.0 * drift["fp64"]
+
+
+def _c3_inputs(gpu, n):
+    import torch
+
+    pos, vel, m = gpu.generators.two_disk(n // 2, softening=0.05, seed=42)
+    dev = torch.device("cuda:0")
+    x0, v0, mm = (torch.tensor(a, dtype=torch.float64, device=dev) for a in (pos, vel, m))
+    g = torch.Generator(device=dev).manual_seed(8)
+    vt = v0 + 0.005 * torch.randn(v0.shape, dtype=torch.float64, device=dev, generator=g)
+    return x0, v0, mm, vt
+
+
+def _loss(x, target):
+    return ((x - target) ** 2).sum(1).mean()
+
+
+def test_c3_adjoint_configured_shape(gpu):
+    import torch
+
+    from gcsim import adjoint
+
+    n, steps, dt, every, h = 16384, 200, 1e-3, 20, 1e-6
+    x0, v0, m, vt = _c3_inputs(gpu, n)
+    dev = x0.device
+    e64 = gpu.Engine("hip-direct", "fp64", softening=0.05)
+    e32 = gpu.Engine("hip-direct", "mixed", softening=0.05)
+    try:
+        with torch.no_grad():
+            target, _ = adjoint.trajectory(x0, vt, m, dt, steps, engine=e64)
+        grads = {}
+        for name, eng in (("mixed", e32), ("fp64", e64)):
+            va, ma = v0.clone().requires_grad_(), m.clone().requires_grad_()
+            xT, _ = adjoint.trajectory(x0, va, ma, dt, steps, engine=eng, checkpoint_every=every)
+            _loss(xT, target).backward()
+            grads[name] = (va.grad.detach().clone(), ma.grad.detach().clone())
+        gv, gm = grads["mixed"]
+        gv64, gm64 = grads["fp64"]
+        rel_v = ((gv - gv64).norm() / gv64.norm()).item()
+        rel_m = ((gm - gm64).norm() / gm64.norm()).item()
+        print(f"C3 mixed vs fp64 adjoint: v {rel_v:.2e} m {rel_m:.2e}", flush=True)
+        assert rel_v < 1e-5 and rel_m < 1e-5
+
+        def L(v, mm):
+            with torch.no_grad():
+                x, _ = adjoint.trajectory(x0, v, mm, dt, steps, engine=e64, checkpoint_every=every)
+                return _loss(x, target).item()
+
+        gen = torch.Generator(device=dev).manual_seed(9)
+        worst = 0.0
+        for k in range(16):
+            du = torch.randn(v0.shape, dtype=torch.float64, device=dev, generator=gen)
+            dm = torch.randn(m.shape, dtype=torch.float64, device=dev, generator=gen) * 1e-3
+            fd = (L(v0 + h * du, m + h * dm) - L(v0 - h * du, m - h * dm)) / (2 * h)
+            ad64 = (gv64 * du).sum().item() + (gm64 * dm).sum().item()
+            ad = (gv * du).sum().item() + (gm * dm).sum().item()
+            # FP64 adjoint vs FP64 differences: the discrete adjoint is exact,
+            # the difference quotient carries O(h^2) and rounding of L / h
+            assert abs(fd - ad64) <= 1e-5 * abs(fd) + 1e-12, (k, fd, ad64)
+            # mixed adjoint: its gradients are within 1e-5 of the FP64 ones in
+            # norm (above); a single projection can cancel to a small value,
+            # so per direction the bound is 1e-3 (measured worst: 4.2e-4)
+            assert abs(fd - ad) <= 1e-3 * abs(fd) + 1e-12, (k, fd, ad)
+            worst = max(worst, abs(fd - ad) / abs(fd))
+        print(f"C3 16 directions: worst mixed-vs-FD relative {worst:.2e}", flush=True)
+    finally:
+        e32.close()
+        e64.close()
+
+
+def test_c3_gradient_against_autograd_at_1024(gpu):
+    import torch
+
+    from gcsim import adjoint
+
+    steps, dt = 200, 1e-3
+    x0, v0, m, vt = _c3_inputs(gpu, 1024)
+    with torch.no_grad():
+        target, _ = adjoint.reference_trajectory(x0, vt, m, dt, steps, softening=0.05)
+    vr, mr = v0.clone().requires_grad_(), m.clone().requires_grad_()
+    xT, _ = adjoint.reference_trajectory(x0, vr, mr, dt, steps, softening=0.05)
+    _loss(xT, target).backward()
+    for precision, tol in (("fp64", 1e-8), ("mixed", 1e-5)):
+        eng = gpu.Engine("hip-direct", precision, softening=0.05)
+        va, ma = v0.clone().requires_grad_(), m.clone().requires_grad_()
+        xT2, _ = adjoint.trajectory(x0, va, ma, dt, steps, engine=eng, checkpoint_every=20)
+        _loss(xT2, target).backward()
+        for got, want in ((va.grad, vr.grad), (ma.grad, mr.grad)):
+            err = ((got - want).norm() / want.norm()).item()
+            assert err < tol, (precision, err)
+        eng.close()
+
+
+def test_c4_batch_configured_shape(gpu):
+    from gcsim.batch import Batch
+
+    specs = gpu.generators.collisions(256, seed=2026, n_min=512, n_max=8192)
+    parts = [gpu.generators.collision_particles(s, softening=0.05) for s in specs]
+    counts = [len(p[2]) for p in parts]
+    assert len(counts) == 256 and min(counts) >= 512 and max(counts) <= 8192
+    data = tuple(np.concatenate([p[i] for p in parts]) for i in range(3))
+    with Batch(counts, softening=0.05) as b:
+        b.upload(*data)
+        b.autotune()
+        chunk, blocks, _ = b.schedule()
+        b.set_schedule(chunk, blocks, graph_steps=10)
+        form, ipt = b.pairs()
+        print(f"C4 tuned: {form} ipt {ipt} chunk {chunk} blocks {blocks}", flush=True)
+        b.compute_accelerations()
+        a0 = b.split(b.download()[2])
+        b.step(1e-3, 20)  # two captured 10-step graphs
+        got = b.download()
+    threads = O.direct_targets_omp_threads(0)
+    worst = 0.0
+    for r, p in enumerate(parts):
+        ref = O.direct_targets(p[0], p[2], 1.0, 0.05, threads=threads)
+        err = rel_err(a0[r], ref).max()
+        worst = max(worst, err)
+        assert err <= MIXED_TOL, (r, counts[r], err)
+    print(f"C4 forces of 256 runs vs oracle: max rel err {worst:.2e}", flush=True)
+    order = np.argsort(counts)
+    for r in (order[0], order[85], order[170], order[-1]):
+        with gpu.Engine("hip-direct", "mixed", softening=0.05) as eng:
+            if form == "symmetric":
+                eng.set_pairs("symmetric")
+                eng.set_launch_shape(ipt, 0)
+                eng.set_pair_items(chunk)
+            else:
+                ntiles = (counts[r] + 255) // 256
+                eng.set_pairs("targets")
+                eng.set_sources("lds")
+                eng.set_launch_shape(1, (ntiles + chunk - 1) // chunk)
+            eng.upload(*parts[r])
+            eng.compute_accelerations()
+            eng.step(1e-3, 20)
+            alone = eng.download()
+        for g, w in zip(got, alone):
+            assert np.array_equal(b.split(g)[r], w), (r, counts[r])
