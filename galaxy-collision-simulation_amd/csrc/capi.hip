@@ -36,6 +36,9 @@ struct gc_ctx {
     size_t vjp_lpack_n = 0, vjp_part_n = 0;
     float* vjp_sym_part = nullptr;  // pair-once VJP: item partials, then pair partials
     size_t vjp_sym_part_n = 0;
+    Src64* jp64 = nullptr;       // FP64 throughput form: packed sources
+    double* part64 = nullptr;    // ... and its per-split partial sums
+    size_t jp64_n = 0, part64_n = 0;
     std::string err;
 };
 
@@ -69,7 +72,8 @@ gc_status check_params(gc_ctx* c, const gc_params* p) {
     if (!(p->softening >= 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "softening must be non-negative");
     if (p->solver != GC_SOLVER_DIRECT && p->solver != GC_SOLVER_FMM && p->solver != GC_SOLVER_TREE)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "unknown solver kind");
-    if (p->precision != GC_PRECISION_FP64 && p->precision != GC_PRECISION_MIXED)
+    if (p->precision != GC_PRECISION_FP64 && p->precision != GC_PRECISION_MIXED &&
+        p->precision != GC_PRECISION_FP64_FAST)
         return fail(c, GC_ERR_INVALID_ARGUMENT, "unknown precision");
     return GC_OK;
 }
@@ -226,6 +230,30 @@ gc_status launch_status(gc_ctx* c, const char* what) {
     return GC_OK;
 }
 
+// Buffers of the FP64 throughput form for the resident particles and
+// `jsplit` source splits.
+gc_status ensure_fp64_fast(gc_ctx* c, int jsplit) {
+    const size_t nsrc = fp64_fast_sources(c->s.n);
+    const size_t npart = static_cast<size_t>(jsplit) * 3 * c->s.cap;
+    if (nsrc <= c->jp64_n && npart <= c->part64_n) return GC_OK;
+    GC_HIP(c, hipStreamSynchronize(c->stream), "synchronize before FP64 buffers");
+    if (nsrc > c->jp64_n) {
+        (void)hipFree(c->jp64);
+        c->jp64 = nullptr;
+        c->jp64_n = 0;
+        GC_HIP(c, hipMalloc(&c->jp64, sizeof(Src64) * nsrc), "allocate FP64 sources");
+        c->jp64_n = nsrc;
+    }
+    if (npart > c->part64_n) {
+        (void)hipFree(c->part64);
+        c->part64 = nullptr;
+        c->part64_n = 0;
+        GC_HIP(c, hipMalloc(&c->part64, sizeof(double) * npart), "allocate FP64 partial sums");
+        c->part64_n = npart;
+    }
+    return GC_OK;
+}
+
 // a(x) for targets [begin, end); optionally the closing half kick.
 gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double half_dt) {
     const gc_params& p = c->params;
@@ -243,6 +271,17 @@ gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double hal
         launch_direct_fp64(c->s, p.gravitational_constant, p.softening, begin, end, close_kick,
                            half_dt, c->stream);
         return launch_status(c, "launch FP64 direct kernel");
+    }
+    if (p.precision == GC_PRECISION_FP64_FAST) {
+        int ipt = 2, js = 1, per = 8;
+        fp64_fast_shape(c->s.n, end - begin, c->shape.ipt, c->shape.jsplit, &ipt, &js, &per);
+        gc_status st = ensure_fp64_fast(c, js);
+        if (st != GC_OK) return st;
+        launch_pack_fp64(c->s, c->jp64, p.gravitational_constant, c->stream);
+        launch_direct_fp64_fast(c->s, c->jp64, c->part64, p.softening, ipt, js, per, begin, end,
+                                c->stream);
+        launch_merge_fp64(c->s, c->part64, js, begin, end, close_kick, half_dt, c->stream);
+        return launch_status(c, "launch FP64 throughput kernels");
     }
     if (symmetric_pairs(c, begin, end)) {
         gc_status st = ensure_sym(c);
@@ -429,6 +468,8 @@ void gc_destroy(gc_ctx* c) {
     (void)hipFree(c->vjp_lpack);
     (void)hipFree(c->vjp_part);
     (void)hipFree(c->vjp_sym_part);
+    (void)hipFree(c->jp64);
+    (void)hipFree(c->part64);
     (void)hipFree(c->diag_out);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -727,8 +768,15 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
         if (rc != GC_OK) return fail(c, static_cast<gc_status>(rc), "hip-fmm timing: " + msg);
         return GC_OK;
     }
-    int ipt = 1, js = 1;
+    int ipt = 1, js = 1, per = 8;
     const bool sym = c->params.precision == GC_PRECISION_MIXED && symmetric_pairs(c, 0, n);
+    const bool fast64 = c->params.precision == GC_PRECISION_FP64_FAST;
+    if (fast64) {
+        fp64_fast_shape(n, n, c->shape.ipt, c->shape.jsplit, &ipt, &js, &per);
+        st = ensure_fp64_fast(c, js);
+        if (st != GC_OK) return st;
+        launch_pack_fp64(c->s, c->jp64, c->params.gravitational_constant, c->stream);
+    }
     if (c->params.precision == GC_PRECISION_MIXED) {
         if (sym) {
             st = ensure_sym(c);
@@ -744,6 +792,9 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
         if (sym) {
             launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, sym_ipt(c),
                               c->stream);
+        } else if (fast64) {
+            launch_direct_fp64_fast(c->s, c->jp64, c->part64, c->params.softening, ipt, js, per, 0,
+                                    n, c->stream);
         } else if (c->params.precision == GC_PRECISION_MIXED) {
             launch_direct_mixed(c->s, c->mixed, c->params.softening, ipt, js, scalar_sources(c), 0, n,
                                 c->stream);
@@ -763,6 +814,8 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
     if (sym) {
         // restore accelerations consistent with the state
         launch_merge_sym(c->s, c->sym, nullptr, false, 0.0, c->stream);
+    } else if (fast64) {
+        launch_merge_fp64(c->s, c->part64, js, 0, n, false, 0.0, c->stream);
     } else if (c->params.precision == GC_PRECISION_MIXED) {
         launch_merge_mixed(c->s, c->mixed, js, 0, n, false, 0.0, c->stream);
     }
@@ -845,7 +898,7 @@ gc_status gc_force_vjp(gc_ctx* c, const double* lambda, double* xbar, double* mb
     const size_t n = c->s.n;
     if (n == 0) return GC_OK;
     const int stride = static_cast<int>(n);
-    if (c->params.precision == GC_PRECISION_FP64) {
+    if (c->params.precision != GC_PRECISION_MIXED) {
         launch_vjp_fp64(c->s, c->params.gravitational_constant, c->params.softening, lambda, stride,
                         xbar, mbar, stride, accumulate != 0, c->stream);
         return launch_status(c, "launch FP64 force VJP");
@@ -942,6 +995,16 @@ gc_status gc_set_launch_shape(gc_ctx* c, int ipt, int jsplit) {
 gc_status gc_get_launch_shape(gc_ctx* c, int* ipt, int* jsplit) {
     GC_ABI_BEGIN
     if (!c || !ipt || !jsplit) return fail(c, GC_ERR_INVALID_ARGUMENT, "get_launch_shape: null argument");
+    if (c->params.precision == GC_PRECISION_FP64_FAST) {
+        int per = 0;
+        fp64_fast_shape(c->s.n, c->s.n, c->shape.ipt, c->shape.jsplit, ipt, jsplit, &per);
+        return GC_OK;
+    }
+    if (c->params.precision == GC_PRECISION_FP64) {  // one target per lane, no splits
+        *ipt = 1;
+        *jsplit = 0;
+        return GC_OK;
+    }
     if (c->s.n > 0 && symmetric_pairs(c, 0, c->s.n)) {
         gc_status st = ensure_sym(c);
         if (st != GC_OK) return st;
@@ -991,6 +1054,7 @@ gc_status gc_set_pairs(gc_ctx* c, int mode) {
 
 int gc_get_pairs(const gc_ctx* c) {
     if (!c) return -1;
+    if (c->params.precision != GC_PRECISION_MIXED) return GC_PAIRS_TARGETS;
     return symmetric_pairs(c, 0, c->s.n) ? GC_PAIRS_SYMMETRIC : GC_PAIRS_TARGETS;
 }
 

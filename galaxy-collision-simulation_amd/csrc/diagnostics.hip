@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void diag_linear_kernel(
     }
 }
 
-struct alignas(32) Src64 {
+struct alignas(32) PotSrc {
     double x, y, z, m;
 };
 
@@ -110,12 +110,12 @@ __device__ __forceinline__ double inv_dist(double x) {
 // Sources of one LDS tile against this lane's target: sum of m_j / d_ij;
 // DIAG: the target's own tile, only j > i counts.
 template <bool DIAG>
-__device__ __forceinline__ void potential_tile(const Src64* tile, int cnt, int jl_min, double xi,
+__device__ __forceinline__ void potential_tile(const PotSrc* tile, int cnt, int jl_min, double xi,
                                                double yi, double zi, double eps2, double& s0,
                                                double& s1) {
     int jj = 0;
     for (; jj + 2 <= cnt; jj += 2) {
-        const Src64 a = tile[jj], b = tile[jj + 1];
+        const PotSrc a = tile[jj], b = tile[jj + 1];
         double dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
         const double ra = inv_dist(__builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2))));
         dx = b.x - xi;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void potential_tile(const Src64* tile, int cnt, int j
         s1 = __builtin_fma(DIAG && jj + 1 <= jl_min ? 0.0 : b.m, rb, s1);
     }
     if (jj < cnt) {
-        const Src64 a = tile[jj];
+        const PotSrc a = tile[jj];
         const double dx = a.x - xi, dy = a.y - yi, dz = a.z - zi;
         const double ra = inv_dist(__builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2))));
         s0 = __builtin_fma(DIAG && jj <= jl_min ? 0.0 : a.m, ra, s0);
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void diag_potential_kernel(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, const double* __restrict__ m, double G, double eps2,
     double* __restrict__ partial) {
-    __shared__ Src64 tile[256];
+    __shared__ PotSrc tile[256];
     __shared__ double red[4];
     const int t = threadIdx.x;
     const int I = blockIdx.x, c = blockIdx.y;
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void diag_potential_kernel(
         const int j0 = jt * 256;
         __syncthreads();
         if (j0 + t < n) {
-            Src64 q;
+            PotSrc q;
             q.x = x[j0 + t];
             q.y = y[j0 + t];
             q.z = z[j0 + t];

@@ -21,6 +21,11 @@ constexpr int kStreamBatch = 8;
 
 constexpr size_t round_up(size_t v, size_t m) { return (v + m - 1) / m * m; }
 
+// FP64 source record of the FP64 kernels: position and G * mass.
+struct alignas(32) Src64 {
+    double x, y, z, gm;
+};
+
 // Resident SoA particle state in HBM (FP64), capacity padded to kTile.
 struct DeviceState {
     size_t n = 0, cap = 0;
@@ -82,6 +87,19 @@ struct IcSegment {
 // direct_fp64.hip
 void launch_direct_fp64(const DeviceState& s, double G, double eps, size_t begin, size_t end,
                         bool close_kick, double half_dt, hipStream_t st);
+// direct_fp64.hip, throughput form: sources packed as Src64 (G m) with
+// zero-mass padding (fp64_fast_sources(n) entries); launch shape for
+// ntargets targets (ipt / jsplit 0: automatic): targets per lane, splits and
+// sources per split; partials jsplit * 3 * s.cap doubles
+size_t fp64_fast_sources(size_t n);
+void launch_pack_fp64(const DeviceState& s, Src64* jp, double G, hipStream_t st);
+void fp64_fast_shape(size_t n, size_t ntargets, int ipt, int jsplit, int* ipt_out,
+                     int* jsplit_out, int* per_split_out);
+void launch_direct_fp64_fast(const DeviceState& s, const Src64* jp, double* part, double eps,
+                             int ipt, int jsplit, int per_split, size_t begin, size_t end,
+                             hipStream_t st);
+void launch_merge_fp64(const DeviceState& s, const double* part, int jsplit, size_t begin,
+                       size_t end, bool close_kick, double half_dt, hipStream_t st);
 // direct_mixed.hip
 void launch_pack_mixed(const DeviceState& s, MixedLayout& L, double G, hipStream_t st);
 // stream: sources by wave-uniform (scalar) loads instead of LDS tiles

@@ -22,7 +22,7 @@ GC_ERR_STATE = 6
 GC_ERR_IO = 7
 
 GC_SOLVER_DIRECT, GC_SOLVER_FMM, GC_SOLVER_TREE = 0, 1, 2
-GC_PRECISION_FP64, GC_PRECISION_MIXED = 0, 1
+GC_PRECISION_FP64, GC_PRECISION_MIXED, GC_PRECISION_FP64_FAST = 0, 1, 2
 GC_SOURCES_AUTO, GC_SOURCES_LDS, GC_SOURCES_SCALAR = 0, 1, 2
 GC_PAIRS_AUTO, GC_PAIRS_TARGETS, GC_PAIRS_SYMMETRIC = 0, 1, 2
 

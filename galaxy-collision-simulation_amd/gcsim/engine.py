@@ -27,7 +27,8 @@ SOLVERS = {"hip-direct": L.GC_SOLVER_DIRECT, "hip": L.GC_SOLVER_DIRECT,
 PAIRS = {"auto": L.GC_PAIRS_AUTO, "targets": L.GC_PAIRS_TARGETS, "symmetric": L.GC_PAIRS_SYMMETRIC}
 SOURCES = {"auto": L.GC_SOURCES_AUTO, "lds": L.GC_SOURCES_LDS, "scalar": L.GC_SOURCES_SCALAR}
 PRECISIONS = {"fp64": L.GC_PRECISION_FP64, "double": L.GC_PRECISION_FP64,
-              "mixed": L.GC_PRECISION_MIXED, "fp32": L.GC_PRECISION_MIXED}
+              "mixed": L.GC_PRECISION_MIXED, "fp32": L.GC_PRECISION_MIXED,
+              "fp64-fast": L.GC_PRECISION_FP64_FAST}
 
 
 def _c64(a, shape=None):
@@ -56,7 +57,7 @@ def make_params(solver="hip-direct", precision="fp64", G=1.0, softening=0.02, th
         raise ValueError(f"Unknown solver '{solver}'. Use hip-direct, hip-fmm or hip-tree.")
     pkey = precision.lower()
     if pkey not in PRECISIONS:
-        raise ValueError(f"Unknown precision '{precision}'. Use fp64 or mixed.")
+        raise ValueError(f"Unknown precision '{precision}'. Use fp64, fp64-fast or mixed.")
     p = L.Params()
     L.load().gc_default_params(C.byref(p))
     p.gravitational_constant = G

@@ -25,7 +25,9 @@ gc_params make_params(const PhysicsParams& p, HipForceSolver::Kind kind, HipPrec
     g.solver = kind == HipForceSolver::Kind::fmm    ? GC_SOLVER_FMM
                : kind == HipForceSolver::Kind::tree ? GC_SOLVER_TREE
                                                     : GC_SOLVER_DIRECT;
-    g.precision = prec == HipPrecision::mixed ? GC_PRECISION_MIXED : GC_PRECISION_FP64;
+    g.precision = prec == HipPrecision::mixed       ? GC_PRECISION_MIXED
+                  : prec == HipPrecision::fp64_fast ? GC_PRECISION_FP64_FAST
+                                                    : GC_PRECISION_FP64;
     g.theta = o.theta;
     g.leaf_capacity = static_cast<int>(std::min<std::size_t>(o.leaf_capacity, 1u << 30));
     g.max_depth = o.max_depth;
@@ -83,7 +85,8 @@ HipPrecision parse_hip_precision(const std::string& value) {
                    [](unsigned char ch) { return static_cast<char>(std::tolower(ch)); });
     if (v == "fp64" || v == "double") return HipPrecision::fp64;
     if (v == "mixed" || v == "fp32") return HipPrecision::mixed;
-    throw std::runtime_error("Unknown precision '" + value + "'. Use fp64 or mixed.");
+    if (v == "fp64-fast") return HipPrecision::fp64_fast;
+    throw std::runtime_error("Unknown precision '" + value + "'. Use fp64, fp64-fast or mixed.");
 }
 
 bool hip_solver_available() { return gc_solver_available() != 0; }

@@ -76,13 +76,17 @@ struct Diagnostics {
 
 namespace fmmgalaxy {
 
-enum class HipPrecision { fp64, mixed };
+// fp64: bit-identical to the reference's FP64 targets form; mixed: FP32
+// pairs, FP64 state; fp64_fast: the FP64 throughput form (rounding-level
+// differences from fp64).
+enum class HipPrecision { fp64, mixed, fp64_fast };
 
 // Solver-name predicates in the style of cpp/core/main.cpp:62-73.
 bool uses_hip_direct_solver(const std::string& solver);
 bool uses_hip_fmm_solver(const std::string& solver);
 bool uses_hip_tree_solver(const std::string& solver);
-// "fp64" | "double" -> fp64, "mixed" | "fp32" -> mixed; throws otherwise.
+// "fp64" | "double" -> fp64, "mixed" | "fp32" -> mixed, "fp64-fast" ->
+// fp64_fast; throws otherwise.
 HipPrecision parse_hip_precision(const std::string& value);
 
 bool hip_solver_available();

@@ -55,10 +55,20 @@ typedef enum { GC_SOLVER_DIRECT = 0, GC_SOLVER_FMM = 1, GC_SOLVER_TREE = 2 } gc_
 
 /* GC_PRECISION_FP64: FP64 pairs, per target ascending-j sums, the same
  *   operation order as compute_direct_accelerations_for_targets
- *   (cpp/direct/direct_solver.cpp:59-83).
+ *   (cpp/direct/direct_solver.cpp:59-83): bit-identical to the reference's
+ *   targets form.
  * GC_PRECISION_MIXED: FP32 pair forces on FP32 offsets from FP64 tile
- *   centres, per-tile FP32 partials merged by two-sum, FP64 state. */
-typedef enum { GC_PRECISION_FP64 = 0, GC_PRECISION_MIXED = 1 } gc_precision;
+ *   centres, per-tile FP32 partials merged by two-sum, FP64 state.
+ * GC_PRECISION_FP64_FAST: the direct solver's FP64 throughput form: FP64
+ *   pairs with 1/sqrt from the hardware estimate plus one correction step,
+ *   sources split into ranges whose FP64 partial sums are added in range
+ *   order (deterministic); it differs from GC_PRECISION_FP64 by rounding
+ *   only.  The FMM and tree solvers and the adjoint treat it as FP64. */
+typedef enum {
+    GC_PRECISION_FP64 = 0,
+    GC_PRECISION_MIXED = 1,
+    GC_PRECISION_FP64_FAST = 2
+} gc_precision;
 
 /* PhysicsParams (cpp/core/simulation_params.hpp:5-8) + FmmOptions
  * (cpp/fmm/fmm_solver.hpp:13-18) + solver/precision selection. */
@@ -266,7 +276,8 @@ gc_status gc_fmm_last_stats(gc_ctx* ctx, gc_fmm_stats* out);
  * (calibration_copy_kernel) on `device`; returns ms per copy. */
 gc_status gc_calibration_copy(int device, size_t bytes, int reps, double* ms_per_copy);
 
-/* Tuning: mixed-kernel launch shape (targets per thread 1/2/4, j-splits).
+/* Tuning: mixed-kernel launch shape (targets per thread 1/2/4, j-splits),
+ * also that of the FP64 throughput form (default 2 targets per lane).
  * 0 = automatic.  gc_get_launch_shape reports the shape a full-range force
  * of the resident particles uses: for the symmetric form its targets per
  * lane and jsplit 0 (it has no source splits). */

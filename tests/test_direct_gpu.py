@@ -389,3 +389,63 @@ def test_multi_step_fusion_is_bit_identical(gpu, n):
             out.append(eng.download())
     for a, b in zip(*out):
         assert np.array_equal(a, b)
+
+
+# ---- FP64 throughput form (precision "fp64-fast") ----------------------------
+FAST64_TOL = 1e-13  # max relative force difference from the oracle's targets form
+
+
+def _fast64(gpu, pos, mass, eps, G=1.0, shape=None, begin=None, end=None):
+    with gpu.Engine("hip-direct", "fp64-fast", softening=eps, G=G) as eng:
+        if shape:
+            eng.set_launch_shape(*shape)
+        eng.upload(pos, None, mass)
+        eng.compute_accelerations(begin, end)
+        return eng.download()[2]
+
+
+@pytest.mark.parametrize("n,eps,G", [(1, 0.02, 1.0), (2, 0.0, 1.0), (257, 0.01, 2.5),
+                                     (3001, 0.05, 1.0)])
+def test_fp64_fast_against_oracle(gpu, n, eps, G):
+    rng = np.random.default_rng(n + 7)
+    pos, m = rng.normal(size=(n, 3)), rng.uniform(0.1, 1.0, n)
+    want = O.direct_targets(pos, m, G, eps)
+    got = _fast64(gpu, pos, m, eps, G)
+    assert rel_err(got, want).max() < FAST64_TOL
+
+
+def test_fp64_fast_coincident_without_softening(gpu):
+    # s2 == 0 pairs are skipped, as the reference's softened_acceleration does
+    pos = np.array([[0.0, 0, 0], [0.0, 0, 0], [1.0, 0, 0], [0.5, 0.5, 0.5], [1.0, 0, 0]])
+    m = np.array([1.0, 2.0, 3.0, 4.0, 0.5])
+    got = _fast64(gpu, pos, m, 0.0)
+    assert np.isfinite(got).all()
+    assert rel_err(got, O.direct_targets(pos, m, 1.0, 0.0)).max() < FAST64_TOL
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (2, 7), (4, 64), (2, 0), (1, 200)])
+def test_fp64_fast_launch_shapes_and_ranges(gpu, shape):
+    pos, vel, m = gpu.generators.config0()
+    want = O.direct_targets(pos, m, 1.0, 0.05)
+    got = _fast64(gpu, pos, m, 0.05, shape=shape)
+    assert rel_err(got, want).max() < FAST64_TOL
+    assert np.array_equal(got, _fast64(gpu, pos, m, 0.05, shape=shape))  # deterministic
+    part = _fast64(gpu, pos, m, 0.05, shape=shape, begin=123, end=777)
+    assert not part[:123].any() and not part[777:].any()
+    assert rel_err(part[123:777], want[123:777]).max() < FAST64_TOL
+
+
+def test_fp64_fast_full_size_and_steps(gpu):
+    pos, vel, m = gpu.generators.config1()
+    with gpu.Engine("hip-direct", "fp64", softening=0.02) as e64, \
+            gpu.Engine("hip-direct", "fp64-fast", softening=0.02) as ef:
+        for e in (e64, ef):
+            e.upload(pos, vel, m)
+            e.compute_accelerations()
+        a64, af = e64.download()[2], ef.download()[2]
+        assert rel_err(af, a64).max() < FAST64_TOL
+        for e in (e64, ef):
+            e.step(5e-4, 20)
+        (p64, v64, a64), (pf, vf, af) = e64.download(), ef.download()
+    assert np.abs(pf - p64).max() < 1e-12 and np.abs(vf - v64).max() < 1e-11
+    assert rel_err(af, a64).max() < 1e-12
