@@ -20,11 +20,10 @@
 // lane l holds source l's sum.  The four waves' sums are added in wave
 // order through LDS, and each (I, J) pair leaves one FP32 partial per
 // target of tile J.  The diagonal tile J == I runs the
-// one-sided form (tile_pairs of mixed_tiles.hpp).  The i-side sum of each J
-// tile is an FP32 partial folded into (hi, lo) by two-sum, as in the
-// one-sided kernel.  The merge adds, per target, its row's item partials in
-// J order and its column's pair partials in I order, in FP64: the result
-// does not depend on scheduling.
+// one-sided form (tile_pairs of mixed_tiles.hpp).  The i-side sum over each
+// J tile is an FP32 row partial of the pair (I, J).  The merge adds, per
+// target, its row's partials in J order and its column's pair partials in I
+// order, in FP64: the result does not depend on scheduling.
 #include "gc_internal.hpp"
 #include "mixed_tiles.hpp"
 
@@ -46,15 +45,93 @@ __host__ __device__ inline size_t pair_slot(int I, int J) {  // I < J
     return static_cast<size_t>(J) * static_cast<size_t>(J - 1) / 2 + static_cast<size_t>(I);
 }
 
+// Row-partial slot of tile pair (I, J), I <= J, within a run.
+__host__ __device__ inline size_t row_slot(int I, int J) {
+    return static_cast<size_t>(J) * static_cast<size_t>(J + 1) / 2 + static_cast<size_t>(I);
+}
+
+#ifndef GC_SYM_LDS_PAD
+#define GC_SYM_LDS_PAD 1024
+#endif
+// Extra (unused) LDS per block of the 4-targets-per-lane kernel: with its
+// 8,960 bytes of source tile and reduction space, 1 KB more holds it at 16
+// blocks per CU of 160 KB, 4 waves per SIMD: 4,096 resident items on 256
+// CUs, which the work list fills in whole rounds (its VGPRs alone would
+// allow 5 waves, its LDS 18 blocks).
+constexpr unsigned kSymLdsPad = GC_SYM_LDS_PAD;
+
 constexpr int kChunk = 64;                // sources per rotation (one wave)
 constexpr int kChunks = kTile / kChunk;   // 4
+
+// One target (offsets t, G m_i) and one source q of a rotation step: the
+// pair's force is added to the target's sum p and subtracted from the
+// source's running sum a.  live == false drops the pair (both sides).
+template <bool SAFE>
+__device__ __forceinline__ void sym_pair(float4 q, float tx, float ty, float tz, float mi,
+                                         float eps2, bool live, float& px, float& py, float& pz,
+                                         float& ax, float& ay, float& az) {
+    const float dx = q.x - tx, dy = q.y - ty, dz = q.z - tz;
+    const float s2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    float r;
+    if (SAFE) {
+        r = s2 > kTinySeparation2 ? __builtin_amdgcn_rsqf(s2) : 0.f;
+    } else {
+        r = __builtin_amdgcn_rsqf(s2);
+    }
+    r = live ? r : 0.f;
+    const float r3 = (r * r) * r;
+    const float kj = q.w * r3, ki = mi * r3;
+    px = __builtin_fmaf(dx, kj, px);
+    py = __builtin_fmaf(dy, kj, py);
+    pz = __builtin_fmaf(dz, kj, pz);
+    ax = __builtin_fmaf(-dx, ki, ax);
+    ay = __builtin_fmaf(-dy, ki, ay);
+    az = __builtin_fmaf(-dz, ki, az);
+}
+
+// Chunk C of the diagonal tile with 4 targets per lane, one wave: the
+// lane's target groups k (targets k * 64 + lane) are the tile's 64-particle
+// chunks.  Groups k < C meet every source of chunk C; group C meets the
+// sources of its own chunk at forward distance 1..32 (distance 32 from
+// lanes 0..31 only), so each unordered pair of the tile comes up exactly
+// once and the tile costs half a tile pair.  After the 64 rotation steps
+// lane l holds the source sum of particle C * 64 + l, the lane's own target
+// of group C, and adds it to that target's sum.
+template <int C, bool SAFE>
+__device__ __forceinline__ void diag_chunk(const float4* row, int lane, float eps2,
+                                           const float* tx, const float* ty, const float* tz,
+                                           const float* mi, float* px, float* py, float* pz) {
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    auto step = [&](int s, bool half, bool live) {
+        const float4 q = row[s];
+#pragma unroll
+        for (int k = 0; k < C; ++k)
+            sym_pair<SAFE>(q, tx[k], ty[k], tz[k], mi[k], eps2, true, px[k], py[k], pz[k], ax, ay,
+                           az);
+        if (half)
+            sym_pair<SAFE>(q, tx[C], ty[C], tz[C], mi[C], eps2, live, px[C], py[C], pz[C], ax, ay,
+                           az);
+        ax = wave_rol1(ax);
+        ay = wave_rol1(ay);
+        az = wave_rol1(az);
+    };
+    step(0, false, false);
+#pragma unroll 8
+    for (int s = 1; s < 32; ++s) step(s, true, true);
+    step(32, true, lane < 32);
+#pragma unroll 8
+    for (int s = 33; s < kChunk; ++s) step(s, false, false);
+    px[C] += ax;
+    py[C] += ay;
+    pz[C] += az;
+}
 
 // IPT targets per lane: a block of kTile / IPT threads holds the kTile
 // targets of tile I (thread t: targets t + k kTile / IPT).  The rotating
 // source sums are shared by a lane's IPT targets, so the three DPP moves of
 // a step serve 2 * IPT interactions.
 template <int IPT, bool SAFE>
-__global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
+__global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 4))) void direct_sym_kernel(
     const SymItem* __restrict__ items, int nitems, int n, const int* __restrict__ tile_valid,
     int ntiles, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ jp,
@@ -74,7 +151,6 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
         const int I = item.I;
         double xi[IPT], yi[IPT], zi[IPT];
         float mi[IPT];
-        float hx[IPT], hy[IPT], hz[IPT], lx[IPT], ly[IPT], lz[IPT];
         const int valid = tile_valid ? tile_valid[I] : n - I * kTile;
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
@@ -84,7 +160,49 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
             yi[k] = active ? y[i] : 0.0;
             zi[k] = active ? z[i] : 0.0;
             mi[k] = jp[i].w;  // G m_i; zero past n
-            hx[k] = hy[k] = hz[k] = lx[k] = ly[k] = lz[k] = 0.f;
+        }
+        // Each (I, J) row partial is stored after the next tile's sources
+        // are staged: vmcnt counts stores too, so stores issued before the
+        // staging loads would be waited for with them.
+        float qx[IPT], qy[IPT], qz[IPT];
+        int pending = -1;
+        auto store_row = [&](int Jp) {
+            float* base = ipart + (item.pbase + item.t0 + row_slot(I - item.t0, Jp - item.t0)) *
+                                      3 * kTile;
+#pragma unroll
+            for (int k = 0; k < IPT; ++k) {
+                const int e = k * kThreads + t;
+                base[e] = qx[k];
+                base[kTile + e] = qy[k];
+                base[2 * kTile + e] = qz[k];
+            }
+        };
+        if (item.jb == I) {
+            // the diagonal tile, always an item of its own
+            float tx[IPT], ty[IPT], tz[IPT];
+            tile_offsets<IPT>(centre, ntiles, I, xi, yi, zi, tx, ty, tz);
+#pragma unroll
+            for (int k = 0; k < IPT; ++k) qx[k] = qy[k] = qz[k] = 0.f;
+            const float4* src = jp + static_cast<size_t>(I) * kTile;
+            if constexpr (IPT == 4) {
+                // each unordered pair once
+                for (int e = t; e < kTile; e += kThreads) {
+                    const float4 v = src[e];
+                    src2[e / kChunk][e % kChunk] = v;
+                    src2[e / kChunk][kChunk + e % kChunk] = v;
+                }
+                __syncthreads();
+                diag_chunk<0, SAFE>(&src2[0][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+                diag_chunk<1, SAFE>(&src2[1][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+                diag_chunk<2, SAFE>(&src2[2][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+                diag_chunk<3, SAFE>(&src2[3][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+            } else {
+                // one-sided: every target over the tile's sources
+                tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, qx, qy, qz);
+            }
+            store_row(I);
+            __syncthreads();  // LDS reuse by the next item
+            continue;
         }
         for (int J = item.jb; J < item.je; ++J) {
             float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
@@ -92,15 +210,17 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
 #pragma unroll
             for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
             const float4* src = jp + static_cast<size_t>(J) * kTile;
-            if (J == I) {
-                tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, px, py, pz);
-            } else {
+            {
                 for (int e = t; e < kTile; e += kThreads) {
                     const float4 v = src[e];
                     src2[e / kChunk][e % kChunk] = v;
                     src2[e / kChunk][kChunk + e % kChunk] = v;
                 }
                 __syncthreads();
+                if (pending >= 0) {
+                    store_row(pending);
+                    pending = -1;
+                }
                 for (int c = 0; c < kChunks; ++c) {
                     // lane l holds the running sum of source c*64 + (l + s) % 64
                     // at step s; wave_rol1 after each step keeps that true.
@@ -160,40 +280,33 @@ __global__ __launch_bounds__(kTile / IPT) void direct_sym_kernel(
                     }
                 }
             }
-            fold_tile<IPT>(px, py, pz, hx, hy, hz, lx, ly, lz);
-        }
-        float* base = ipart + static_cast<size_t>(item.islot) * 6 * kTile;
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) {
-            const int e = k * kThreads + t;
-            base[e] = hx[k];
-            base[kTile + e] = hy[k];
-            base[2 * kTile + e] = hz[k];
-            base[3 * kTile + e] = lx[k];
-            base[4 * kTile + e] = ly[k];
-            base[5 * kTile + e] = lz[k];
+            for (int k = 0; k < IPT; ++k) {
+                qx[k] = px[k];
+                qy[k] = py[k];
+                qz[k] = pz[k];
+            }
+            pending = J;
         }
+        if (pending >= 0) store_row(pending);
         __syncthreads();  // LDS reuse by the next item
     }
 }
 
-// a_i = sum over the items of row T (J order) of (hi + lo) + sum over I < T
-// of the (I, T) pair partials, in FP64; optional closing half kick.  A
-// block of 16 groups of 64 lanes merges tile T, 4 targets per lane (float4
-// loads); group g adds the g-th of 16 consecutive ranges of the row's
-// items and of the column's pair partials, and the 16 group sums are added
-// in group order, so the result is deterministic.  With items of 2 source
-// tiles a row's item partials and a column's pair partials are about the
-// same bytes, ~3 KB per source tile.  Blocks run the longest columns
-// (largest T) first.
+// a_i = sum over J >= T of the (T, J) row partials (J order) + sum over
+// I < T of the (I, T) column partials (I order), in FP64; optional closing
+// half kick.  A block of 16 groups of 64 lanes merges tile T, 4 targets per
+// lane (float4 loads); group g adds the g-th of 16 consecutive ranges of
+// that sequence of nt partials (nt: the tile's run length), and the 16 group
+// sums are added in group order, so the result is deterministic.  Every
+// tile reads nt partials of 3 KB.  Blocks run the longest columns (largest
+// T) first.
 constexpr int kMergeGroups = 16;
 constexpr int kMergeLanes = kTile / 4;  // 64
 
-// Row and column partials of target tile T summed per group into part
-// (group g adds its share of the row items, then its share of the column
-// pairs, in order).
+// Row and column partials of target tile T summed per group into part.
 __device__ __forceinline__ void merge_sym_groups(
-    int T, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    int T, const int* __restrict__ tile_t0, const int* __restrict__ tile_nt,
     const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
     const float* __restrict__ jpart, double (*part)[3][kTile]) {
 #pragma clang fp contract(off)
@@ -202,27 +315,18 @@ __device__ __forceinline__ void merge_sym_groups(
     double a[3][4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) a[c][0] = a[c][1] = a[c][2] = a[c][3] = 0.0;
-    // row: item partials (hi, lo) of row T
-    const int r0 = row_start[T], nr = row_start[T + 1] - r0;
-    const int s1 = r0 + (g + 1) * nr / kMergeGroups;
-    for (int s = r0 + g * nr / kMergeGroups; s < s1; ++s) {
-        const float* b = ipart + static_cast<size_t>(s) * 6 * kTile + 4 * lane;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float4 h = *reinterpret_cast<const float4*>(b + c * kTile);
-            const float4 lo = *reinterpret_cast<const float4*>(b + (c + 3) * kTile);
-            a[c][0] += static_cast<double>(h.x) + static_cast<double>(lo.x);
-            a[c][1] += static_cast<double>(h.y) + static_cast<double>(lo.y);
-            a[c][2] += static_cast<double>(h.z) + static_cast<double>(lo.z);
-            a[c][3] += static_cast<double>(h.w) + static_cast<double>(lo.w);
+    const int t0 = tile_t0[T], nt = tile_nt[T], r = T - t0;
+    const long long pbase = tile_pbase[T];
+    const int nrow = nt - r;  // row partials (T, J), J = T .. t0 + nt - 1
+    const int q0 = g * nt / kMergeGroups, q1 = (g + 1) * nt / kMergeGroups;
+    for (int q = q0; q < q1; ++q) {
+        const float* b;
+        if (q < nrow) {
+            b = ipart + (pbase + t0 + row_slot(r, r + q)) * 3 * kTile;
+        } else {
+            b = jpart + (pbase + pair_slot(q - nrow, r)) * 3 * kTile;
         }
-    }
-    // column: pair partials (I, T) for I in [t0, T) of T's run
-    const int t0 = tile_t0[T], cols = T - t0;
-    const float* col = jpart + (tile_pbase[T] + pair_slot(0, cols)) * 3 * kTile + 4 * lane;
-    const int i1 = (g + 1) * cols / kMergeGroups;
-    for (int I = g * cols / kMergeGroups; I < i1; ++I) {
-        const float* b = col + static_cast<size_t>(I) * 3 * kTile;
+        b += 4 * lane;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float4 v = *reinterpret_cast<const float4*>(b + c * kTile);
@@ -239,7 +343,7 @@ __device__ __forceinline__ void merge_sym_groups(
 }
 
 __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
-    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    int n, int ntiles, const int* __restrict__ tile_t0, const int* __restrict__ tile_nt,
     const long long* __restrict__ tile_pbase, const int* __restrict__ tile_valid,
     const float* __restrict__ ipart, const float* __restrict__ jpart, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
@@ -247,7 +351,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
 #pragma clang fp contract(off)
     __shared__ double part[kMergeGroups][3][kTile];
     const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
-    merge_sym_groups(T, row_start, tile_t0, tile_pbase, ipart, jpart, part);
+    merge_sym_groups(T, tile_t0, tile_nt, tile_pbase, ipart, jpart, part);
     __syncthreads();
     // thread (c, e) of the first 3 * kTile adds target e's component c
     const int t = threadIdx.x;
@@ -273,7 +377,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
 // unchanged; the state and the tiles are touched once instead of three
 // times.
 __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_kernel(
-    int n, int ntiles, const int* __restrict__ row_start, const int* __restrict__ tile_t0,
+    int n, int ntiles, const int* __restrict__ tile_t0, const int* __restrict__ tile_nt,
     const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
     const float* __restrict__ jpart, double* __restrict__ x, double* __restrict__ y,
     double* __restrict__ z, double* __restrict__ vx, double* __restrict__ vy,
@@ -285,7 +389,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_ker
     __shared__ double box[2][3][4];   // per component: wave minima, maxima
     __shared__ float offs[3][kTile];
     const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
-    merge_sym_groups(T, row_start, tile_t0, tile_pbase, ipart, jpart, part);
+    merge_sym_groups(T, tile_t0, tile_nt, tile_pbase, ipart, jpart, part);
     __syncthreads();
     const int t = threadIdx.x;
     const bool active = t < 3 * kTile;  // thread (c, e): component c of target e
@@ -592,11 +696,12 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
     SymHostPlan h;
     if (!build_sym_plan(run_t0, run_nt, nruns, len, kSymMaxTiles, h, err)) return 1;
     plan.nitems = static_cast<int>(h.items.size());
-    plan.ipart_floats = static_cast<size_t>(plan.nitems) * 6 * kTile;
+    plan.ipart_floats = (static_cast<size_t>(std::max<long long>(0, h.pairs)) + h.ntiles) * 3 * kTile;
     plan.jpart_floats = std::max<size_t>(1, static_cast<size_t>(h.pairs)) * 3 * kTile;
     hipError_t e = hipMalloc(&plan.items, sizeof(SymItem) * h.items.size());
     if (e == hipSuccess) e = hipMalloc(&plan.row_start, sizeof(int) * h.row_start.size());
     if (e == hipSuccess) e = hipMalloc(&plan.tile_t0, sizeof(int) * h.tile_t0.size());
+    if (e == hipSuccess) e = hipMalloc(&plan.tile_nt, sizeof(int) * h.tile_nt.size());
     if (e == hipSuccess) e = hipMalloc(&plan.tile_pbase, sizeof(long long) * h.tile_pbase.size());
     if (e == hipSuccess) e = hipMalloc(&plan.ipart, sizeof(float) * plan.ipart_floats);
     if (e == hipSuccess) e = hipMalloc(&plan.jpart, sizeof(float) * plan.jpart_floats);
@@ -608,6 +713,9 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(plan.tile_t0, h.tile_t0.data(), sizeof(int) * h.tile_t0.size(),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(plan.tile_nt, h.tile_nt.data(), sizeof(int) * h.tile_nt.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(plan.tile_pbase, h.tile_pbase.data(),
@@ -638,6 +746,7 @@ void sym_release(SymPlan& plan) {
     (void)hipFree(plan.items);
     (void)hipFree(plan.row_start);
     (void)hipFree(plan.tile_t0);
+    (void)hipFree(plan.tile_nt);
     (void)hipFree(plan.tile_pbase);
     (void)hipFree(plan.ipart);
     (void)hipFree(plan.jpart);
@@ -650,12 +759,13 @@ void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const dou
                     hipStream_t st) {
     const int n = static_cast<int>(s.n);
     const dim3 grid(static_cast<unsigned>(blocks > 0 ? std::min(blocks, plan.nitems) : plan.nitems));
+    const unsigned lds = IPT == 4 ? kSymLdsPad : 0;
     if (safe) {
-        hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), grid, dim3(kTile / IPT), 0, st,
+        hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), grid, dim3(kTile / IPT), lds, st,
                            plan.items, plan.nitems, n, tile_valid, ntiles, s.x, s.y, s.z, jp, centre,
                            eps2, plan.ipart, plan.jpart);
     } else {
-        hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), grid, dim3(kTile / IPT), 0, st,
+        hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), grid, dim3(kTile / IPT), lds, st,
                            plan.items, plan.nitems, n, tile_valid, ntiles, s.x, s.y, s.z, jp, centre,
                            eps2, plan.ipart, plan.jpart);
     }
@@ -682,7 +792,7 @@ void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
                       bool close_kick, double half_dt, hipStream_t st) {
     hipLaunchKernelGGL(merge_sym_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups), 0, st,
-                       static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
+                       static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.tile_nt,
                        plan.tile_pbase, tile_valid, plan.ipart, plan.jpart, s.ax, s.ay, s.az, s.vx,
                        s.vy, s.vz, close_kick ? 1 : 0, half_dt);
 }
@@ -718,7 +828,7 @@ void launch_vjp_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& p
 void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,
                            double half_dt, double dt, hipStream_t st) {
     hipLaunchKernelGGL(merge_sym_open_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups),
-                       0, st, static_cast<int>(s.n), plan.ntiles, plan.row_start, plan.tile_t0,
+                       0, st, static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.tile_nt,
                        plan.tile_pbase, plan.ipart, plan.jpart, s.x, s.y, s.z, s.vx, s.vy, s.vz,
                        s.ax, s.ay, s.az, s.m, half_dt, dt, G, L.jp, L.centre);
 }

@@ -3,13 +3,20 @@
 //
 // The work of independent runs of 256-particle tiles is the upper triangle
 // of tile pairs (I, J), J >= I, within each run.  Each row I of a run is cut
-// into items of `len` consecutive J tiles (the last item of a row may be
-// shorter); by default len = max(1, T / 16384) for a run with T tile pairs,
-// about 16,384 items per run of 181 or more tiles.  An item's row partial
+// into the diagonal item (I, I) and items of `len` consecutive J > I tiles
+// (the last item of a row may be shorter); by default len = max(1, T / 16384)
+// for a run with T tile pairs, about 16,384 items per run of 181 or more
+// tiles.  An item's row partial
 // goes to slot islot, numbered in (run, row, J) order, so row T's slots are
 // [row_start[T], row_start[T + 1]) in J order.  The launch order is longest
-// item first (ties in slot order), one block per item, so the hardware
-// dispatcher balances the tail with the shortest items.
+// item first (ties in slot order, diagonal items after single tiles), one
+// block per item, so the hardware dispatcher balances the tail with the
+// shortest items.  The force kernel
+// writes one row partial per tile pair (I, J), J >= I: slot
+// pbase + t0 + (J - t0)(J - t0 + 1) / 2 + (I - t0) of (pairs + ntiles) slots
+// (runs' slot ranges do not overlap since each run's pbase + t0 is at least
+// the previous run's end); the item slots serve the adjoint's (hi, lo) row
+// partials.
 #pragma once
 
 #include <algorithm>
@@ -33,6 +40,7 @@ struct SymHostPlan {
     std::vector<SymItem> items;         // launch order
     std::vector<int> row_start;         // ntiles + 1 slot offsets
     std::vector<int> tile_t0;           // ntiles
+    std::vector<int> tile_nt;           // ntiles: tile count of each tile's run
     std::vector<long long> tile_pbase;  // ntiles
     long long pairs = 0;                // tile pairs I < J over all runs
     int ntiles = 0;
@@ -63,6 +71,7 @@ inline bool build_sym_plan(const int* run_t0, const int* run_nt, int nruns, int 
     }
     out.row_start.assign(ntiles + 1, 0);
     out.tile_t0.assign(ntiles, 0);
+    out.tile_nt.assign(ntiles, 0);
     out.tile_pbase.assign(ntiles, 0);
     long long pbase = 0;
     int next_tile = 0;
@@ -73,12 +82,15 @@ inline bool build_sym_plan(const int* run_t0, const int* run_nt, int nruns, int 
         for (int I = t0; I < t0 + nt; ++I) {
             out.row_start[I] = static_cast<int>(out.items.size());
             out.tile_t0[I] = t0;
+            out.tile_nt[I] = nt;
             out.tile_pbase[I] = pbase;
-            for (int J = I; J < t0 + nt; J += step) {
+            // the diagonal tile is an item of its own; the row's other
+            // tiles are cut into items of `step` tiles
+            for (int J = I, first = 1; J < t0 + nt; J = first ? J + 1 : J + step, first = 0) {
                 SymItem it{};
                 it.I = I;
                 it.jb = J;
-                it.je = std::min(J + step, t0 + nt);
+                it.je = first ? J + 1 : std::min(J + step, t0 + nt);
                 it.islot = static_cast<int>(out.items.size());
                 it.pbase = pbase;
                 it.t0 = t0;
@@ -89,9 +101,11 @@ inline bool build_sym_plan(const int* run_t0, const int* run_nt, int nruns, int 
         next_tile = std::max(next_tile, t0 + nt);
     }
     for (int T = next_tile; T <= ntiles; ++T) out.row_start[T] = static_cast<int>(out.items.size());
-    std::stable_sort(out.items.begin(), out.items.end(), [](const SymItem& a, const SymItem& b) {
-        return (a.je - a.jb) > (b.je - b.jb);
-    });
+    // longest first; a diagonal item (half a tile pair in the symmetric
+    // kernel) after the single-tile items of its length
+    auto cost = [](const SymItem& a) { return 2 * (a.je - a.jb) - (a.jb == a.I ? 1 : 0); };
+    std::stable_sort(out.items.begin(), out.items.end(),
+                     [&](const SymItem& a, const SymItem& b) { return cost(a) > cost(b); });
     out.pairs = pbase;
     out.ntiles = ntiles;
     return true;

@@ -1,9 +1,10 @@
 // Host check of the symmetric kernel's work list
 // (galaxy-collision-simulation_amd/csrc/sym_plan.hpp): every tile pair
 // (I < J) of a run and every diagonal tile is covered by exactly one item,
-// no pair crosses runs, row-partial slots are numbered in (row, J) order
-// and indexed by row_start, pair-partial slots stay inside their buffer,
-// and items launch longest first.
+// each diagonal tile is an item of its own, no pair crosses runs, item
+// slots are numbered in (row, J) order and indexed by row_start, pair and
+// row-partial slots stay inside their buffers and are unique, and items
+// launch by decreasing cost (a diagonal counting half a tile pair).
 #include <cstdio>
 #include <random>
 #include <string>
@@ -43,6 +44,8 @@ static void check(const std::vector<int>& t0, const std::vector<int>& nt, int le
     }
     CHECK(p.pairs == pairs, "pairs %lld vs %lld", p.pairs, pairs);
     std::vector<int> diag(ntiles, 0), cover(static_cast<size_t>(pairs), 0);
+    std::vector<int> rows(static_cast<size_t>(pairs + ntiles), 0);  // row-partial slots
+    auto cost = [](const SymItem& q) { return 2 * (q.je - q.jb) - (q.jb == q.I ? 1 : 0); };
     std::vector<const SymItem*> by_slot(nitems, nullptr);
     for (int k = 0; k < nitems; ++k) {
         const SymItem& it = p.items[k];
@@ -57,7 +60,12 @@ static void check(const std::vector<int>& t0, const std::vector<int>& nt, int le
         CHECK(it.pbase == p.tile_pbase[it.I], "pbase of item %d", k);
         CHECK(it.jb >= it.I && it.je > it.jb && it.je <= e, "J range of item %d", k);
         CHECK(it.je - it.jb <= gc::sym_item_tiles(nt[r], len), "item %d too long", k);
+        CHECK(it.jb != it.I || it.je == it.I + 1, "diagonal item %d holds other tiles", k);
+        CHECK(p.tile_nt[it.I] == nt[r], "run length of row %d", it.I);
         for (int J = it.jb; J < it.je; ++J) {
+            const long long rs = it.pbase + a + static_cast<long long>(J - a) * (J - a + 1) / 2 + (it.I - a);
+            CHECK(rs >= 0 && rs < pairs + ntiles, "row slot %lld of %lld", rs, pairs + ntiles);
+            if (rs >= 0 && rs < pairs + ntiles) ++rows[rs];
             if (J == it.I) {
                 ++diag[J];
                 continue;
@@ -68,12 +76,13 @@ static void check(const std::vector<int>& t0, const std::vector<int>& nt, int le
         }
         if (k > 0) {
             const SymItem& q = p.items[k - 1];
-            CHECK(q.je - q.jb >= it.je - it.jb, "launch order at %d", k);
+            CHECK(cost(q) >= cost(it), "launch order at %d", k);
         }
     }
     for (int T = 0; T < ntiles; ++T)
         CHECK(diag[T] == (run_of[T] >= 0 ? 1 : 0), "diagonal %d covered %d times", T, diag[T]);
     for (size_t u = 0; u < cover.size(); ++u) CHECK(cover[u] == 1, "pair %zu covered %d times", u, cover[u]);
+    for (size_t u = 0; u < rows.size(); ++u) CHECK(rows[u] <= 1, "row slot %zu used %d times", u, rows[u]);
     // row slots: consecutive, in J order, starting at the diagonal
     CHECK(static_cast<int>(p.row_start.size()) == ntiles + 1 && p.row_start[ntiles] == nitems,
           "row_start end");
@@ -105,12 +114,22 @@ int main() {
         }
         for (int len : {0, 1, 2, 4}) check(t0, nt, len);
     }
-    // the default: items of 2 tiles for one run of 256 tiles (N = 65,536)
+    // the default for one run of 256 tiles (N = 65,536): 256 diagonal items
+    // and the 32,640 tile pairs I < J in 16,256 items of 2 tiles and 128 of
+    // one (rows with an odd count), 32,768 tile-pair units in all: 8 per
+    // resident wave of 4,096
     SymHostPlan p;
     std::string err;
     const int z = 0, n256 = 256, big = 1025;
-    CHECK(gc::build_sym_plan(&z, &n256, 1, 0, 1024, p, &err) && p.items.size() == 16512,
+    CHECK(gc::build_sym_plan(&z, &n256, 1, 0, 1024, p, &err) && p.items.size() == 16640,
           "N = 65,536 items %zu", p.items.size());
+    int two = 0, one = 0, dg = 0;
+    for (const SymItem& it : p.items) {
+        if (it.jb == it.I) ++dg;
+        else if (it.je - it.jb == 2) ++two;
+        else ++one;
+    }
+    CHECK(two == 16256 && one == 128 && dg == 256, "item sizes %d %d %d", two, one, dg);
     CHECK(!gc::build_sym_plan(&z, &z, 1, 0, 1024, p, &err), "zero tiles accepted");
     CHECK(!gc::build_sym_plan(&z, &big, 1, 0, 1024, p, &err), "oversized run accepted");
     if (failures) {
