@@ -80,6 +80,11 @@ def workload_fmm(args):
     wall = time.perf_counter() - t0
     st = eng.fmm_stats()
     k_ms, _ = eng.time_force_kernel(reps=3)
+    # a target range of N/8 (one rank's share under the reference's
+    # ownership split): whole build, the range's targets evaluated
+    eng.compute_accelerations(0, n // 8)
+    eng.synchronize()
+    range_st = eng.fmm_stats()
     eng.close()
     O = _oracle()
     threads = O.direct_targets_omp_threads(0)
@@ -102,6 +107,8 @@ def workload_fmm(args):
         "tflops_eval_20_440": flops / (k_ms * 1e-3) / 1e12,
         "stats": {k: st0[k] for k in ("node_count", "leaf_count", "far_interactions",
                                       "near_interactions", "max_depth")},
+        "range_eighth": {"build_ms": range_st["build_ms"], "eval_ms": range_st["eval_ms"],
+                         "eval_over_full": range_st["eval_ms"] / st["eval_ms"]},
         "accuracy_vs_fp64_direct_4096": {"max": float(err.max()), "median": float(np.median(err)),
                                          "p95": float(np.percentile(err, 95))},
         "fp32_vs_oracle_fmm_4096_max": float(err_vs_cpu_fmm.max()),
@@ -700,8 +707,8 @@ def main() -> int:
         out["c2_fmm"] = {k: fmm[k] for k in (
             "workload", "ms_per_step", "build_ms", "eval_ms", "eval_kernel_ms", "p2p_pairs",
             "m2p_evaluations", "Gp2p_per_s_eval", "Gm2p_per_s_eval", "tflops_eval_20_440",
-            "stats", "accuracy_vs_fp64_direct_4096", "fp32_vs_oracle_fmm_4096_max",
-            "cpu_baseline")}
+            "stats", "range_eighth", "accuracy_vs_fp64_direct_4096",
+            "fp32_vs_oracle_fmm_4096_max", "cpu_baseline")}
         batch = workload_batch(args, steps=100)
         out["c4_batch"] = {k: batch[k] for k in (
             "workload", "runs", "particles", "pairs_per_step", "schedule", "ms_per_step",

@@ -689,14 +689,20 @@ __host__ __device__ inline int far_targets_per_lane(int c) {
 // evaluates cells g, g + groups, .. of each staged chunk for all of them,
 // reading each cell's table once.  Group sums are added in group order and
 // written by group 0.
+// Sorted position of target ti of a leaf chunk starting at list entry base:
+// the leaf's own particles (tpos null) or its entries of the target list.
+__device__ __forceinline__ int target_pos(const int* tpos, int base, int ti) {
+    return tpos ? tpos[base + ti] : base + ti;
+}
+
 template <typename F, int ORDER, int TPL>
 __device__ __forceinline__ void far_leaf_chunk(
     FarStage<F>& fs, double (&red)[3][kEvalLanes], int base, int c, double cx, double cy,
     double cz, F eps2, const double* __restrict__ mcom, const F* __restrict__ kcoef,
     const int* __restrict__ sidx, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, long long fo, int nf, const int* __restrict__ far, double G,
-    int begin, int end, double* __restrict__ ax, double* __restrict__ ay,
-    double* __restrict__ az) {
+    int begin, int end, const int* __restrict__ tpos, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az) {
 #pragma clang fp contract(off)
     constexpr int kChunk = far_chunk<F>();
     const int lane = threadIdx.x;
@@ -710,9 +716,10 @@ __device__ __forceinline__ void far_leaf_chunk(
 #pragma unroll
     for (int k = 0; k < TPL; ++k) {
         const int ti = min(tl + k * h, c - 1);  // a spare slot repeats the last target
-        tx[k] = static_cast<F>(sx[base + ti] - cx);
-        ty[k] = static_cast<F>(sy[base + ti] - cy);
-        tz[k] = static_cast<F>(sz[base + ti] - cz);
+        const int pos = target_pos(tpos, base, ti);
+        tx[k] = static_cast<F>(sx[pos] - cx);
+        ty[k] = static_cast<F>(sy[pos] - cy);
+        tz[k] = static_cast<F>(sz[pos] - cz);
         far_a[k][0] = far_a[k][1] = far_a[k][2] = 0.0;
     }
     for (int f0 = 0; f0 < nf; f0 += kChunk) {
@@ -764,7 +771,7 @@ __device__ __forceinline__ void far_leaf_chunk(
             for (int j = 1; j < groups; ++j) acc += red[d][tl + j * h];
             a[d] = G * acc;
         }
-        const int me = sidx[base + ti];
+        const int me = sidx[target_pos(tpos, base, ti)];
         if (me < begin || me >= end) continue;
         ax[me] = a[0];
         ay[me] = a[1];
@@ -780,20 +787,24 @@ __global__ __launch_bounds__(kEvalLanes) void eval_far_kernel(
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const long long* __restrict__ far_off, const int* __restrict__ nfar,
     const int* __restrict__ far, double G, double eps, int begin, int end,
+    const int* __restrict__ tpos, const int* __restrict__ toff, const int* __restrict__ tcnt,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
     __shared__ FarStage<F> fs;
     __shared__ double red[3][kEvalLanes];
     const int leaf = leaves[blockIdx.x];
+    // targets: the leaf's particles, or its entries of the range's target list
+    const int tb = tpos ? toff[blockIdx.x] : nb[leaf];
+    const int te = tpos ? tb + tcnt[blockIdx.x] : ne[leaf];
+    if (tb >= te) return;
     const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
     const F eps2 = static_cast<F>(eps * eps);
-    const int tb = nb[leaf], te = ne[leaf];
     const long long fo = far_off[blockIdx.x];
     const int nf = nfar[blockIdx.x];
     for (int base = tb; base < te; base += kEvalLanes) {
         const int c = min(kEvalLanes, te - base);
 #define GC_CHUNK(T)                                                                              \
     far_leaf_chunk<F, ORDER, T>(fs, red, base, c, cx, cy, cz, eps2, mcom, kcoef, sidx, sx, sy, sz, \
-                                fo, nf, far, G, begin, end, ax, ay, az)
+                                fo, nf, far, G, begin, end, tpos, ax, ay, az)
         switch (far_targets_per_lane(c)) {  // uniform per block
         case 3: GC_CHUNK(3); break;
         case 2: GC_CHUNK(2); break;
@@ -891,10 +902,11 @@ __device__ __forceinline__ void near_round(const NearSrc<F>* tile, int g, int cn
 template <typename F, bool SAFE, int TPL>
 __device__ __forceinline__ void near_leaf_chunk(
     NearSrc<F> (&tile)[kNearStage], int (&qs)[kNearPiece], int (&qpre)[kNearPiece + 1],
-    double (&red)[3][kEvalLanes], const NearItem& item, int tb, int base, int c, double cx,
-    double cy, double cz, F eps2, const int* __restrict__ nb, const int* __restrict__ ne,
-    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const double* __restrict__ sm, long long no, const int* __restrict__ near, double G,
+    double (&red)[3][kEvalLanes], const NearItem& item, int lb, int lsize, int base, int c,
+    double cx, double cy, double cz, F eps2, const int* __restrict__ nb,
+    const int* __restrict__ ne, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, const double* __restrict__ sm, long long no,
+    const int* __restrict__ near, double G, const int* __restrict__ tpos,
     double* __restrict__ part) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x;
@@ -904,13 +916,15 @@ __device__ __forceinline__ void near_leaf_chunk(
     const int tl = lane - g * h;
     const bool active = g < groups;
     F tx[TPL], ty[TPL], tz[TPL];
+    int tpl_pos[TPL];  // sorted positions of the lane's targets
     double near_a[TPL][3];
 #pragma unroll
     for (int k = 0; k < TPL; ++k) {
         const int ti = min(tl + k * h, c - 1);  // a spare slot repeats the last target
-        tx[k] = static_cast<F>(sx[base + ti] - cx);
-        ty[k] = static_cast<F>(sy[base + ti] - cy);
-        tz[k] = static_cast<F>(sz[base + ti] - cz);
+        tpl_pos[k] = target_pos(tpos, base, ti);
+        tx[k] = static_cast<F>(sx[tpl_pos[k]] - cx);
+        ty[k] = static_cast<F>(sy[tpl_pos[k]] - cy);
+        tz[k] = static_cast<F>(sz[tpl_pos[k]] - cz);
         near_a[k][0] = near_a[k][1] = near_a[k][2] = 0.0;
     }
     // source j of the current piece: its sorted index (-1 past the end)
@@ -943,16 +957,14 @@ __device__ __forceinline__ void near_leaf_chunk(
         const int total = __shfl(incl, 63, 64);
         __syncthreads();
         // the target leaf among this piece's near leaves, if it is there:
-        // target ti is then source qpre[qself] + (base - tb) + ti
-        const unsigned long long own = __ballot(lane < kq && qs[lane] == tb);
-        // the chunk's targets are sources [self_lo, self_lo + c) of the piece
-        const int self_lo = own ? qpre[__ffsll(static_cast<long long>(own)) - 1] + (base - tb) : -1;
+        // the particle at sorted position p of the leaf is then source
+        // qpre[qself] + (p - lb) of the piece, and the leaf's particles are
+        // sources [self_lo, self_lo + lsize)
+        const unsigned long long own = __ballot(lane < kq && qs[lane] == lb);
+        const int self_lo = own ? qpre[__ffsll(static_cast<long long>(own)) - 1] : -1;
         int selfpos[TPL];
 #pragma unroll
-        for (int k = 0; k < TPL; ++k) {
-            const int ti = min(tl + k * h, c - 1);
-            selfpos[k] = own ? qpre[__ffsll(static_cast<long long>(own)) - 1] + (base - tb) + ti : -1;
-        }
+        for (int k = 0; k < TPL; ++k) selfpos[k] = own ? self_lo + (tpl_pos[k] - lb) : -1;
         // registers hold the next round's sources
         double nxv[kNearPer][4];
         int nid[kNearPer];
@@ -995,7 +1007,7 @@ __device__ __forceinline__ void near_leaf_chunk(
             for (int q = 0; q < TPL; ++q) px[q] = py[q] = pz[q] = 0;
             // only a round holding the target leaf's own sources can meet a
             // self pair (wave-uniform test); the others skip the check
-            if (own && cb < self_lo + c && self_lo < cb + cnt)
+            if (own && cb < self_lo + lsize && self_lo < cb + cnt)
                 near_round<F, SAFE, TPL, true>(tile, g, cnt, groups, cb, eps2, tx, ty, tz, selfpos,
                                                px, py, pz);
             else
@@ -1016,7 +1028,7 @@ __device__ __forceinline__ void near_leaf_chunk(
         __syncthreads();
         const int ti = tl + q * h;
         if (g != 0 || ti >= c) continue;
-        double* row = part + 3 * (static_cast<size_t>(item.slot) + (base - tb) + ti);
+        double* row = part + 3 * (static_cast<size_t>(item.slot) + (tpl_pos[q] - lb));
         for (int d = 0; d < 3; ++d) {
             double acc = red[d][tl];
             for (int j = 1; j < groups; ++j) acc += red[d][tl + j * h];
@@ -1031,21 +1043,26 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
     const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const double* __restrict__ sm, const long long* __restrict__ near_off,
-    const int* __restrict__ near, double G, double eps, double* __restrict__ part) {
+    const int* __restrict__ near, double G, double eps, const int* __restrict__ tpos,
+    const int* __restrict__ toff, const int* __restrict__ tcnt, double* __restrict__ part) {
     __shared__ NearSrc<F> tile[kNearStage];
     __shared__ int qs[kNearPiece], qpre[kNearPiece + 1];
     __shared__ double red[3][kEvalLanes];
     const NearItem item = items[blockIdx.x];
     const int leaf = leaves[item.leaf];
+    const int lb = nb[leaf], lsize = ne[leaf] - lb;
+    // targets: the leaf's particles, or its entries of the range's target list
+    const int tb = tpos ? toff[item.leaf] : lb;
+    const int te = tpos ? tb + tcnt[item.leaf] : lb + lsize;
+    if (tb >= te) return;
     const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
     const F eps2 = static_cast<F>(eps * eps);
-    const int tb = nb[leaf], te = ne[leaf];
     const long long no = near_off[item.leaf];
     for (int base = tb; base < te; base += kEvalLanes) {
         const int c = min(kEvalLanes, te - base);
 #define GC_CHUNK(T)                                                                              \
-    near_leaf_chunk<F, SAFE, T>(tile, qs, qpre, red, item, tb, base, c, cx, cy, cz, eps2, nb, ne,  \
-                                sx, sy, sz, sm, no, near, G, part)
+    near_leaf_chunk<F, SAFE, T>(tile, qs, qpre, red, item, lb, lsize, base, c, cx, cy, cz, eps2,   \
+                                nb, ne, sx, sy, sz, sm, no, near, G, tpos, part)
         // at most 2 targets per lane: a third costs more registers than the
         // slots it fills save here (measured)
         if (far_targets_per_lane(c) >= 2) GC_CHUNK(2);  // uniform per block
@@ -1060,18 +1077,22 @@ __global__ __launch_bounds__(kEvalLanes) void near_merge_kernel(
     const int* __restrict__ leaves, const int* __restrict__ nb, const int* __restrict__ ne,
     const int* __restrict__ sidx, const int* __restrict__ icount, const int* __restrict__ ioff,
     const NearItem* __restrict__ items, const double* __restrict__ part, int begin, int end,
+    const int* __restrict__ tpos, const int* __restrict__ toff, const int* __restrict__ tcnt,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz, int close_kick,
     double half_dt) {
 #pragma clang fp contract(off)
     const int li = blockIdx.x;
     const int leaf = leaves[li];
-    const int tb = nb[leaf], te = ne[leaf];
+    const int lb = nb[leaf];
+    const int tb = tpos ? toff[li] : lb;
+    const int te = tpos ? tb + tcnt[li] : ne[leaf];
     const int k = icount[li], i0 = ioff[li];
-    for (int t = tb + static_cast<int>(threadIdx.x); t < te; t += kEvalLanes) {
+    for (int e = tb + static_cast<int>(threadIdx.x); e < te; e += kEvalLanes) {
+        const int t = tpos ? tpos[e] : e;  // sorted position
         double a[3] = {0.0, 0.0, 0.0};
         for (int j = 0; j < k; ++j) {
-            const double* row = part + 3 * (static_cast<size_t>(items[i0 + j].slot) + (t - tb));
+            const double* row = part + 3 * (static_cast<size_t>(items[i0 + j].slot) + (t - lb));
             a[0] += row[0];
             a[1] += row[1];
             a[2] += row[2];
@@ -1344,6 +1365,35 @@ __global__ __launch_bounds__(256) void zero_range_kernel(int begin, int end, dou
     az[i] = 0.0;
 }
 
+// Targets of a range [begin, end): per leaf the count of its particles in
+// the range, then (after a scan) their sorted positions in ascending order.
+__global__ __launch_bounds__(256) void range_count_kernel(int nleaves, const int* __restrict__ leaves,
+                                                          const int* __restrict__ nb,
+                                                          const int* __restrict__ ne,
+                                                          const int* __restrict__ sidx, int begin,
+                                                          int end, int* __restrict__ tcnt) {
+    const int li = blockIdx.x * 256 + threadIdx.x;
+    if (li >= nleaves) return;
+    const int l = leaves[li];
+    int c = 0;
+    for (int t = nb[l]; t < ne[l]; ++t) c += (sidx[t] >= begin && sidx[t] < end) ? 1 : 0;
+    tcnt[li] = c;
+}
+
+__global__ __launch_bounds__(256) void range_fill_kernel(int nleaves, const int* __restrict__ leaves,
+                                                         const int* __restrict__ nb,
+                                                         const int* __restrict__ ne,
+                                                         const int* __restrict__ sidx, int begin,
+                                                         int end, const int* __restrict__ toff,
+                                                         int* __restrict__ tpos) {
+    const int li = blockIdx.x * 256 + threadIdx.x;
+    if (li >= nleaves) return;
+    const int l = leaves[li];
+    int o = toff[li];
+    for (int t = nb[l]; t < ne[l]; ++t)
+        if (sidx[t] >= begin && sidx[t] < end) tpos[o++] = t;
+}
+
 __global__ __launch_bounds__(256) void stats_kernel(
     int nleaves, const int* __restrict__ leaves, const int* __restrict__ nb,
     const int* __restrict__ ne, const int* __restrict__ nfar, const int* __restrict__ nnear,
@@ -1462,6 +1512,10 @@ struct FmmEngine::Impl {
     // near-field work items and their partial sums
     int *icount = nullptr, *ioff = nullptr;
     long long* row_off = nullptr;
+    // target range: per leaf its count and first entry, and the list of
+    // sorted positions of the range's targets
+    int *tcnt = nullptr, *toff = nullptr, *tpos = nullptr;
+    size_t tleaf_cap = 0, tpos_cap = 0;
     size_t icap = 0, itemcap = 0, partcap = 0;
     NearItem* items = nullptr;
     double* npart = nullptr;
@@ -1481,7 +1535,8 @@ struct FmmEngine::Impl {
         void* ptrs[] = {keys, keys_sorted, keys_lo, keys_lo_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
                         kc64, kc32, nb, ne, first_child, nchild, bounds, count, offset,
                         scan_tmp, overflow, lvl, deepest, geom_root, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
-                        near_off, frontier, far, near, stash_far, stash_near, lpos_far, lpos_near, wtot, icount, ioff, row_off, items, npart, stats_dev};
+                        near_off, frontier, far, near, stash_far, stash_near, lpos_far, lpos_near, wtot, icount, ioff, row_off, items, npart, stats_dev,
+                        tcnt, toff, tpos};
         for (void* p : ptrs) (void)hipFree(p);
         if (tree_counts) (void)hipHostFree(tree_counts);
         for (auto& e : ev)
@@ -1987,14 +2042,40 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
         tree_counts_pending = true;
         return GC_OK;
     }
+    // a target range evaluates only its own targets (the tree and lists are
+    // the whole set's, as in compute_targets, fmm_solver.cpp:24-46)
+    const int* tp = nullptr;
+    if (begin > 0 || end < s.n) {
+        if (static_cast<size_t>(nleaves) > tleaf_cap) {
+            size_t c2 = 0;
+            tleaf_cap = 0;  // set by the last allocation only
+            FMM_HIP(realloc_dev(&toff, &c2, static_cast<size_t>(nleaves) * 2), "allocate range lists");
+            FMM_HIP(realloc_dev(&tcnt, &tleaf_cap, c2), "allocate range lists");
+        }
+        if (end - begin > tpos_cap)
+            FMM_HIP(realloc_dev(&tpos, &tpos_cap, end - begin), "allocate range lists");
+        size_t need = 0;
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, tcnt, toff, nleaves, st), "size scan");
+        if (need > scan_tmp_bytes)
+            FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
+                    "allocate scan storage");
+        const unsigned lblk = static_cast<unsigned>((nleaves + 255) / 256);
+        hipLaunchKernelGGL(range_count_kernel, dim3(lblk), dim3(256), 0, st, nleaves, leaves, nb, ne,
+                           idx_sorted, b, e, tcnt);
+        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, tcnt, toff, nleaves, st),
+                "scan range targets");
+        hipLaunchKernelGGL(range_fill_kernel, dim3(lblk), dim3(256), 0, st, nleaves, leaves, nb, ne,
+                           idx_sorted, b, e, toff, tpos);
+        tp = tpos;
+    }
 #define GC_FAR(F, O, KC)                                                                         \
     hipLaunchKernelGGL((eval_far_kernel<F, O>), grid, block, 0, st, leaves, geom, mcom, KC, nb, ne, \
-                       idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, s.ax, s.ay,    \
-                       s.az)
+                       idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, tp, toff,     \
+                       tcnt, s.ax, s.ay, s.az)
 #define GC_NEAR(F, S)                                                                            \
     hipLaunchKernelGGL((eval_near_kernel<F, S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
                        items, leaves, geom, nb, ne, sxs, sys, szs, sms, near_off, near, G, eps,   \
-                       npart)
+                       tp, toff, tcnt, npart)
     if (mixed) {
         if (order == 0) GC_FAR(float, 0, kc32);
         else if (order == 2) GC_FAR(float, 2, kc32);
@@ -2011,8 +2092,8 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
 #undef GC_FAR
 #undef GC_NEAR
     hipLaunchKernelGGL(near_merge_kernel, grid, block, 0, st, leaves, nb, ne, idx_sorted, icount, ioff,
-                       items, npart, b, e, s.ax, s.ay, s.az, s.vx, s.vy, s.vz, close_kick ? 1 : 0,
-                       half_dt);
+                       items, npart, b, e, tp, toff, tcnt, s.ax, s.ay, s.az, s.vx, s.vy, s.vz,
+                       close_kick ? 1 : 0, half_dt);
     FMM_HIP(hipGetLastError(), "launch FMM evaluation");
     return GC_OK;
 }

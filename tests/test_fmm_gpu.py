@@ -183,3 +183,32 @@ def test_one_traversal_lists_match_two_pass_and_regrow(gpu):
     assert rel_err(b1, b64).max() < MIXED_TOL
     want, _ = O.fmm(pos, m, 1.0, 0.01, 0.5, 16, 32, 4)
     assert rel_err(a1, want).max() < MIXED_TOL
+
+
+@pytest.mark.parametrize("begin,end", [(0, 375), (1000, 1001), (2900, 3000), (700, 2300)])
+def test_target_ranges_evaluate_only_their_targets(gpu, begin, end):
+    # compute_targets (cpp/fmm/fmm_solver.cpp:24-46): the whole tree and
+    # lists, accelerations written for [begin, end) only; the device walks
+    # only the range's targets, and the statistics are the whole tree's
+    pos, vel, m = gpu.generators.plummer(3000, seed=31)
+    want, st = O.fmm(pos, m, 1.0, 0.02, 0.5, 16, 32, 4, begin=begin, end=end)
+    for precision, tol in (("fp64", FP64_TOL), ("mixed", MIXED_TOL)):
+        got, gst = _fmm(gpu, precision, pos, m, 0.02, 0.5, 16, begin=begin, end=end)
+        assert not got[:begin].any() and not got[end:].any()
+        assert rel_err(got[begin:end], want[begin:end]).max() < tol, precision
+        for k in STAT_KEYS:
+            assert gst[k] == st[k], (precision, k)
+
+
+def test_target_range_steps_match_full_range(gpu):
+    # one engine stepping the whole set, another evaluating two halves as
+    # ranges: the same accelerations to rounding
+    pos, vel, m = gpu.generators.plummer(5000, seed=32)
+    with gpu.Engine("hip-fmm", "fp64", softening=0.02, theta=0.5, leaf_capacity=16) as eng:
+        eng.upload(pos, vel, m)
+        eng.compute_accelerations()
+        full = eng.download()[2]
+        eng.compute_accelerations(0, 2500)
+        eng.compute_accelerations(2500, 5000)
+        halves = eng.download()[2]
+    assert rel_err(halves, full).max() < FP64_TOL
