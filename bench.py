@@ -86,17 +86,8 @@ def workload_fmm(args):
     eng.synchronize()
     range_st = eng.fmm_stats()
     eng.close()
-    O = _oracle()
-    threads = O.direct_targets_omp_threads(0)
-    sample = 4096
-    direct = O.direct_targets(pos, mass, 1.0, eps, 0, sample, threads=threads)
-    err = _rel_err(a0[:sample], direct[:sample])
-    tc = time.perf_counter()
-    acc_cpu, _ = O.fmm(pos, mass, 1.0, eps, 0.5, 64, 32, 4, begin=0, end=sample)
-    cpu_s = time.perf_counter() - tc
-    err_vs_cpu_fmm = _rel_err(a0[:sample], acc_cpu[:sample])
     flops = 20.0 * st["p2p_pairs"] + 440.0 * st["m2p_evaluations"]
-    return {
+    out = {
         "workload": "fmm: Plummer N=2^20, leaf<=64, p=4, theta=0.5, eps=0.01, dt=1e-3, 10 KDK",
         "ms_per_step": wall * 1e3 / steps,
         "eval_kernel_ms": k_ms,
@@ -109,6 +100,19 @@ def workload_fmm(args):
                                       "near_interactions", "max_depth")},
         "range_eighth": {"build_ms": range_st["build_ms"], "eval_ms": range_st["eval_ms"],
                          "eval_over_full": range_st["eval_ms"] / st["eval_ms"]},
+    }
+    if args.no_cpu:  # GPU legs only (profiling runs)
+        return out
+    O = _oracle()
+    threads = O.direct_targets_omp_threads(0)
+    sample = 4096
+    direct = O.direct_targets(pos, mass, 1.0, eps, 0, sample, threads=threads)
+    err = _rel_err(a0[:sample], direct[:sample])
+    tc = time.perf_counter()
+    acc_cpu, _ = O.fmm(pos, mass, 1.0, eps, 0.5, 64, 32, 4, begin=0, end=sample)
+    cpu_s = time.perf_counter() - tc
+    err_vs_cpu_fmm = _rel_err(a0[:sample], acc_cpu[:sample])
+    out.update({
         "accuracy_vs_fp64_direct_4096": {"max": float(err.max()), "median": float(np.median(err)),
                                          "p95": float(np.percentile(err, 95))},
         "fp32_vs_oracle_fmm_4096_max": float(err_vs_cpu_fmm.max()),
@@ -116,7 +120,8 @@ def workload_fmm(args):
                          "sample": f"oracle FMM, full tree + lists, {sample} targets",
                          "seconds": cpu_s,
                          "extrapolated_s_per_evaluation": cpu_s * n / sample},
-    }
+    })
+    return out
 
 
 def workload_tree(args):
@@ -143,17 +148,8 @@ def workload_tree(args):
     st = eng.fmm_stats()
     k_ms, _ = eng.time_force_kernel(reps=2)
     eng.close()
-    O = _oracle()
-    threads = O.direct_targets_omp_threads(0)
-    sample = 4096
-    direct = O.direct_targets(pos, mass, 1.0, eps, 0, sample, threads=threads)
-    err = _rel_err(a0[:sample], direct[:sample])
-    tc = time.perf_counter()
-    acc_cpu, _ = O.tree(pos, mass, 1.0, eps, 0.6, 16, 32, 4, begin=0, end=sample)
-    cpu_s = time.perf_counter() - tc
-    err_vs_cpu = _rel_err(a0[:sample], acc_cpu[:sample])
     flops = 20.0 * st0["p2p_pairs"] + 440.0 * st0["m2p_evaluations"]
-    return {
+    out = {
         "workload": "tree: Plummer N=2^20, leaf<=16, p=4, theta=0.6, eps=0.01, dt=1e-3, 3 KDK",
         "ms_per_step": wall * 1e3 / steps,
         "eval_kernel_ms": k_ms,
@@ -163,6 +159,19 @@ def workload_tree(args):
         "Gm2p_per_s_eval": st0["m2p_evaluations"] / (k_ms * 1e-3) / 1e9,
         "tflops_eval_20_440": flops / (k_ms * 1e-3) / 1e12,
         "stats": {k: st0[k] for k in ("node_count", "leaf_count", "max_depth")},
+    }
+    if args.no_cpu:  # GPU legs only (profiling runs)
+        return out
+    O = _oracle()
+    threads = O.direct_targets_omp_threads(0)
+    sample = 4096
+    direct = O.direct_targets(pos, mass, 1.0, eps, 0, sample, threads=threads)
+    err = _rel_err(a0[:sample], direct[:sample])
+    tc = time.perf_counter()
+    acc_cpu, _ = O.tree(pos, mass, 1.0, eps, 0.6, 16, 32, 4, begin=0, end=sample)
+    cpu_s = time.perf_counter() - tc
+    err_vs_cpu = _rel_err(a0[:sample], acc_cpu[:sample])
+    out.update({
         "accuracy_vs_fp64_direct_4096": {"max": float(err.max()), "median": float(np.median(err)),
                                          "p95": float(np.percentile(err, 95))},
         "fp32_vs_oracle_tree_4096_max": float(err_vs_cpu.max()),
@@ -170,7 +179,8 @@ def workload_tree(args):
                          "sample": f"oracle tree, full build, {sample} targets",
                          "seconds": cpu_s,
                          "extrapolated_s_per_evaluation": cpu_s * n / sample},
-    }
+    })
+    return out
 
 
 def workload_batch(args, steps=None):
@@ -708,7 +718,7 @@ def main() -> int:
             "workload", "ms_per_step", "build_ms", "eval_ms", "eval_kernel_ms", "p2p_pairs",
             "m2p_evaluations", "Gp2p_per_s_eval", "Gm2p_per_s_eval", "tflops_eval_20_440",
             "stats", "range_eighth", "accuracy_vs_fp64_direct_4096",
-            "fp32_vs_oracle_fmm_4096_max", "cpu_baseline")}
+            "fp32_vs_oracle_fmm_4096_max", "cpu_baseline") if k in fmm}
         batch = workload_batch(args, steps=100)
         out["c4_batch"] = {k: batch[k] for k in (
             "workload", "runs", "particles", "pairs_per_step", "schedule", "ms_per_step",

@@ -654,9 +654,11 @@ struct alignas(16) FarStage {
 // c <= 64 targets is served by G = floor(64 / c) lane groups; per lane the
 // FP32 (or FP64) terms of a chunk are added into FP64 sums, and the groups'
 // sums are added in group order through LDS (deterministic).
-//  far  (eval_far_kernel): chunks of 64 far cells are staged in LDS by one
-//       cooperative load (cell indices, COM offsets, moments); group g
+//  far  in FP64 (eval_far_kernel): chunks of 16 far cells are staged in LDS
+//       by one cooperative load (COM offsets, coefficient tables); group g
 //       evaluates cells g, g + G, ... of the chunk by M2P; writes G * sum.
+//       In FP32 the far field runs with the cells across the lanes instead
+//       (eval_far_cells_kernel, below).
 //  near (eval_near_kernel): the near leaves' particles, 64 leaves at a time
 //       laid end to end, are staged kNearStage sources per round, the next
 //       round's sources loaded into registers while the current one is
@@ -814,7 +816,140 @@ __global__ __launch_bounds__(kEvalLanes) void eval_far_kernel(
     }
 }
 
+// Far field in FP32 with the cells across the lanes: each lane holds one far
+// cell's coefficient table in registers (64 cells per pass, loaded once per
+// leaf chunk) and the chunk's targets are taken four at a time from LDS, the
+// same for every lane.  A lane's four contributions are summed over the 64
+// lanes by a fixed network (halving over lane bits 0 and 1, then rotations
+// by 4 and 8 within each 16-lane row, then the four row sums in row order
+// through LDS), so lane t adds target t's FP32 pass sum to its FP64 sum.
+// No table is read from LDS, and no lane is idle but in a leaf's last pass.
+// Lanes past the list's end hold a zero table at a point 4 half-widths plus
+// one away from the leaf's centre, so their terms are exactly zero.
+constexpr int kRowQuad = 4;  // targets per network evaluation
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+constexpr int kDppXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int kDppRor4 = 0x124;  // row_ror:4
+constexpr int kDppRor8 = 0x128;  // row_ror:8
+
+// Lane l returns the sum over its 16-lane row of a[l & 3] (every lane active).
+__device__ __forceinline__ float row_quad_sum(float a0, float a1, float a2, float a3, int lane) {
+#pragma clang fp contract(off)
+    const bool b0 = lane & 1, b1 = lane & 2;
+    const float n0 = (b0 ? a1 : a0) + dpp_f<kDppXor1>(b0 ? a0 : a1);
+    const float n1 = (b0 ? a3 : a2) + dpp_f<kDppXor1>(b0 ? a2 : a3);
+    float v = (b1 ? n1 : n0) + dpp_f<kDppXor2>(b1 ? n0 : n1);
+    v += dpp_f<kDppRor4>(v);
+    v += dpp_f<kDppRor8>(v);
+    return v;
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(kEvalLanes) void eval_far_cells_kernel(
+    const int* __restrict__ leaves, const double* __restrict__ geom,
+    const double* __restrict__ mcom, const float* __restrict__ kcoef,
+    const int* __restrict__ nb, const int* __restrict__ ne, const int* __restrict__ sidx,
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const long long* __restrict__ far_off, const int* __restrict__ nfar,
+    const int* __restrict__ far, double G, double eps, int begin, int end,
+    const int* __restrict__ tpos, const int* __restrict__ toff, const int* __restrict__ tcnt,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
+#pragma clang fp contract(off)
+    constexpr int kVec = ORDER == 4 ? 17 : (ORDER == 2 ? 2 : 1);  // float4s of the table used
+    __shared__ float tgt[kEvalLanes][4];
+    __shared__ float red[4][kRowQuad][4];
+    const int li = blockIdx.x;  // leaf-list position
+    const int leaf = leaves[li];
+    const int tb = tpos ? toff[li] : nb[leaf];
+    const int te = tpos ? tb + tcnt[li] : ne[leaf];
+    if (tb >= te) return;
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    const float pad = static_cast<float>(4.0 * geom[4 * leaf + 3] + 1.0);
+    const float eps2 = static_cast<float>(eps * eps);
+    const long long fo = far_off[li];
+    const int nf = nfar[li];
+    const int lane = threadIdx.x, q = lane & 3, row = lane >> 4;
+    for (int base = tb; base < te; base += kEvalLanes) {
+        const int c = min(kEvalLanes, te - base);
+        __syncthreads();
+        if (lane < c) {
+            const int pos = target_pos(tpos, base, lane);
+            tgt[lane][0] = static_cast<float>(sx[pos] - cx);
+            tgt[lane][1] = static_cast<float>(sy[pos] - cy);
+            tgt[lane][2] = static_cast<float>(sz[pos] - cz);
+        }
+        __syncthreads();
+        double a64[3] = {0.0, 0.0, 0.0};
+        int next = lane < nf ? far[fo + lane] : -1;  // cell ids run one pass ahead
+        for (int f0 = 0; f0 < nf; f0 += kEvalLanes) {
+            const int cell = next;
+            next = f0 + kEvalLanes + lane < nf ? far[fo + f0 + kEvalLanes + lane] : -1;
+            float K[4 * kVec];
+            float px = pad, py = pad, pz = pad;
+            if (cell >= 0) {
+                const float4* src =
+                    reinterpret_cast<const float4*>(kcoef + static_cast<size_t>(kFarCoef) * cell);
+#pragma unroll
+                for (int v = 0; v < kVec; ++v) {
+                    const float4 k4 = src[v];
+                    K[4 * v] = k4.x;
+                    K[4 * v + 1] = k4.y;
+                    K[4 * v + 2] = k4.z;
+                    K[4 * v + 3] = k4.w;
+                }
+                px = static_cast<float>(mcom[4 * cell + 1] - cx);
+                py = static_cast<float>(mcom[4 * cell + 2] - cy);
+                pz = static_cast<float>(mcom[4 * cell + 3] - cz);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4 * kVec; ++i) K[i] = 0.0f;
+            }
+            for (int t0 = 0; t0 < c; t0 += kRowQuad) {
+                float acc[kRowQuad][3];
+#pragma unroll
+                for (int u = 0; u < kRowQuad; ++u) {
+                    acc[u][0] = acc[u][1] = acc[u][2] = 0.0f;
+                    if (t0 + u < c) {  // uniform
+                        const float* T = tgt[t0 + u];
+                        far_m2p<float, ORDER>(px - T[0], py - T[1], pz - T[2], eps2, K, acc[u][0],
+                                              acc[u][1], acc[u][2]);
+                    }
+                }
+                float r[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    r[d] = row_quad_sum(acc[0][d], acc[1][d], acc[2][d], acc[3][d], lane);
+                __syncthreads();  // the previous quad's row sums are read
+                if ((lane & 15) < kRowQuad)
+                    for (int d = 0; d < 3; ++d) red[row][q][d] = r[d];
+                __syncthreads();
+                if ((lane >> 2) == (t0 >> 2)) {  // lane t0 + q: target t0 + q
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        a64[d] += static_cast<double>(((red[0][q][d] + red[1][q][d]) + red[2][q][d]) +
+                                                      red[3][q][d]);
+                }
+            }
+        }
+        if (lane < c) {
+            const int me = sidx[target_pos(tpos, base, lane)];
+            if (me >= begin && me < end) {
+                ax[me] = G * a64[0];
+                ay[me] = G * a64[1];
+                az[me] = G * a64[2];
+            }
+        }
+    }
+}
+
 constexpr int kNearStage = 256;     // sources staged per round
+constexpr int kNearPad = kEvalLanes;  // zero-mass entries past the stage
 constexpr int kNearPer = kNearStage / kEvalLanes;
 constexpr int kNearItem = 256;      // near leaves per work item
 
@@ -867,13 +1002,18 @@ struct alignas(4 * sizeof(F)) NearSrc {
 // Pair sums of one staged round for TPL targets per lane: group g takes
 // sources g, g + groups, ...; CHECK drops the self pair (source cb + jj is
 // the target's own particle when it equals selfpos).
+// The round's trip count is the same for every lane group (a scalar loop):
+// entries past cnt, up to kNearPad past the stage, hold zero-mass sources.
 template <typename F, bool SAFE, int TPL, bool CHECK>
 __device__ __forceinline__ void near_round(const NearSrc<F>* tile, int g, int cnt, int groups,
                                            int cb, F eps2, const F* tx, const F* ty, const F* tz,
                                            const int* selfpos, F* px, F* py, F* pz) {
 #pragma clang fp contract(off)
-    for (int jj = g; jj < cnt; jj += groups) {
-        const NearSrc<F> sj = tile[jj];
+    const int steps = (cnt + groups - 1) / groups;
+    const NearSrc<F>* src = tile + g;
+    int jj = g;
+    for (int k = 0; k < steps; ++k, src += groups, jj += groups) {
+        const NearSrc<F> sj = *src;
 #pragma unroll
         for (int q = 0; q < TPL; ++q) {
             const F dx = sj.x - tx[q], dy = sj.y - ty[q], dz = sj.z - tz[q];
@@ -901,7 +1041,7 @@ __device__ __forceinline__ void near_round(const NearSrc<F>* tile, int g, int cn
 // its near leaves) and zeroes that pair's factor.
 template <typename F, bool SAFE, int TPL>
 __device__ __forceinline__ void near_leaf_chunk(
-    NearSrc<F> (&tile)[kNearStage], int (&qs)[kNearPiece], int (&qpre)[kNearPiece + 1],
+    NearSrc<F> (&tile)[kNearStage + kNearPad], int (&qs)[kNearPiece], int (&qpre)[kNearPiece + 1],
     double (&red)[3][kEvalLanes], const NearItem& item, int lb, int lsize, int base, int c,
     double cx, double cy, double cz, F eps2, const int* __restrict__ nb,
     const int* __restrict__ ne, const double* __restrict__ sx, const double* __restrict__ sy,
@@ -1045,10 +1185,13 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
     const double* __restrict__ sm, const long long* __restrict__ near_off,
     const int* __restrict__ near, double G, double eps, const int* __restrict__ tpos,
     const int* __restrict__ toff, const int* __restrict__ tcnt, double* __restrict__ part) {
-    __shared__ NearSrc<F> tile[kNearStage];
+    __shared__ NearSrc<F> tile[kNearStage + kNearPad];
     __shared__ int qs[kNearPiece], qpre[kNearPiece + 1];
     __shared__ double red[3][kEvalLanes];
     const NearItem item = items[blockIdx.x];
+    // the padding: zero mass at the leaf's centre (never rewritten; the
+    // stage's own entries past a round's end carry zero mass too)
+    tile[kNearStage + threadIdx.x] = NearSrc<F>{F(0), F(0), F(0), F(0)};
     const int leaf = leaves[item.leaf];
     const int lb = nb[leaf], lsize = ne[leaf] - lb;
     // targets: the leaf's particles, or its entries of the range's target list
@@ -2076,10 +2219,14 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     hipLaunchKernelGGL((eval_near_kernel<F, S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
                        items, leaves, geom, nb, ne, sxs, sys, szs, sms, near_off, near, G, eps,   \
                        tp, toff, tcnt, npart)
+#define GC_FAR32(O)                                                                              \
+    hipLaunchKernelGGL((eval_far_cells_kernel<O>), grid, block, 0, st, leaves, geom, mcom, kc32, nb, \
+                       ne, idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, tp, toff, \
+                       tcnt, s.ax, s.ay, s.az)
     if (mixed) {
-        if (order == 0) GC_FAR(float, 0, kc32);
-        else if (order == 2) GC_FAR(float, 2, kc32);
-        else GC_FAR(float, 4, kc32);
+        if (order == 0) GC_FAR32(0);
+        else if (order == 2) GC_FAR32(2);
+        else GC_FAR32(4);
         if (safe) GC_NEAR(float, true);
         else GC_NEAR(float, false);
     } else {
@@ -2090,6 +2237,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
         else GC_NEAR(double, false);
     }
 #undef GC_FAR
+#undef GC_FAR32
 #undef GC_NEAR
     hipLaunchKernelGGL(near_merge_kernel, grid, block, 0, st, leaves, nb, ne, idx_sorted, icount, ioff,
                        items, npart, b, e, tp, toff, tcnt, s.ax, s.ay, s.az, s.vx, s.vy, s.vz,
