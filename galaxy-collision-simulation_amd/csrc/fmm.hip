@@ -54,6 +54,7 @@ constexpr int kEvalLanes = 64;
 // (tree), then internal nodes at kStatInternal
 constexpr int kStats = 8;
 constexpr int kStatInternal = 4;
+constexpr int kStatMassMax = 5;  // largest |m| of the build (double bits)
 
 __device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double gc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -435,14 +436,18 @@ __global__ __launch_bounds__(256) void leaf_collect_kernel(int nnodes, const int
 }
 
 // ---- 6. interaction lists ----------------------------------------------
-__device__ __forceinline__ bool well_separated(const double* __restrict__ geom, int t, int s,
-                                               double theta) {
+
+// The reference's acceptance test (fmm_solver.cpp:243-277): cell s is far
+// from target t when 2 (h_t + h_s) / |c_s - c_t| < theta (never at zero
+// distance); h is the half width.
+__device__ __forceinline__ bool well_separated_at(double tx, double ty, double tz, double th,
+                                                  double sx, double sy, double sz, double sh,
+                                                  double theta) {
 #pragma clang fp contract(off)
-    const double dx = geom[4 * s] - geom[4 * t], dy = geom[4 * s + 1] - geom[4 * t + 1],
-                 dz = geom[4 * s + 2] - geom[4 * t + 2];
+    const double dx = sx - tx, dy = sy - ty, dz = sz - tz;
     const double dist = __builtin_sqrt(dx * dx + dy * dy + dz * dz);
     if (dist == 0.0) return false;
-    const double w = 2.0 * (geom[4 * t + 3] + geom[4 * s + 3]);
+    const double w = 2.0 * (th + sh);
     return w / dist < theta;
 }
 
@@ -456,7 +461,8 @@ __device__ __forceinline__ bool well_separated(const double* __restrict__ geom, 
 // prefix sums), so the lists are deterministic; they hold the reference's
 // cells, in breadth-first rather than depth-first order.  The grid is
 // persistent: wave w takes leaves w, w + G, ...; its two frontier buffers of
-// `fcap` nodes live in `scratch`.  A frontier that would outgrow `fcap` sets
+// `fcap` nodes hold their first kListLds entries in LDS and the rest in
+// `scratch`.  A round loads all the fields of its 64 nodes at once.  A frontier that would outgrow `fcap` sets
 // `overflow` and the host repeats the pass with larger buffers.
 // FILL == false: count far cells, near leaves and near source particles per
 // leaf; FILL == true: write the lists at the scanned offsets.
@@ -468,6 +474,8 @@ __device__ __forceinline__ bool well_separated(const double* __restrict__ geom, 
 // once the counts are scanned.  Bit 1 of *overflow flags a wave whose
 // lists outgrew its region (counts stay exact; MODE 1 then writes the
 // lists), bit 0 a frontier overflow.
+constexpr int kListLds = 512;  // frontier entries per buffer held in LDS
+
 template <int MODE>
 __global__ __launch_bounds__(64) void lists_kernel(
     int nleaves, const int* __restrict__ leaves, const double* __restrict__ geom,
@@ -487,11 +495,16 @@ __global__ __launch_bounds__(64) void lists_kernel(
     int* const wfar = far + static_cast<size_t>(blockIdx.x) * wcap_far;
     int* const wnear = near + static_cast<size_t>(blockIdx.x) * wcap_near;
     long long used_f = 0, used_n = 0;
+    // frontier entries below kListLds live in LDS, the rest in `scratch`
+    __shared__ int lfront[2][kListLds];
     for (int li = blockIdx.x; li < nleaves; li += gridDim.x) {
         const int t = leaves[li];
+        const double tx = geom[4 * t], ty = geom[4 * t + 1], tz = geom[4 * t + 2], th = geom[4 * t + 3];
         int* cur = bufa;
         int* nxt = bufb;
-        if (lane == 0) cur[0] = 0;
+        int* lcur = lfront[0];
+        int* lnxt = lfront[1];
+        if (lane == 0) lcur[0] = 0;
         __syncthreads();  // one wave: orders the frontier writes before the reads
         int ncur = 1;
         long long cf = 0, cn = 0, cs = 0;
@@ -515,14 +528,19 @@ __global__ __launch_bounds__(64) void lists_kernel(
             int nnext = 0;
             for (int base = 0; base < ncur; base += 64) {
                 const int j = base + lane;
-                int node = -1, kind = 3, nc = 0;
+                int node = -1, kind = 3, nc = 0, fc = 0;
                 if (j < ncur) {
-                    node = cur[j];
-                    if (mcom[4 * node] > 0.0) {
-                        if (node != t && well_separated(geom, t, node, theta)) {
+                    node = j < kListLds ? lcur[j] : cur[j];
+                    // the node's fields load together (one memory round trip)
+                    const double mass = mcom[4 * node];
+                    const double gx = geom[4 * node], gy = geom[4 * node + 1],
+                                 gz = geom[4 * node + 2], gh = geom[4 * node + 3];
+                    nc = nchild[node];
+                    fc = first_child[node];
+                    if (mass > 0.0) {
+                        if (node != t && well_separated_at(tx, ty, tz, th, gx, gy, gz, gh, theta)) {
                             kind = 0;
                         } else {
-                            nc = nchild[node];
                             kind = nc == 0 ? 1 : 2;
                         }
                     }
@@ -537,39 +555,44 @@ __global__ __launch_bounds__(64) void lists_kernel(
                 }
                 cf += __popcll(mfar);
                 cn += __popcll(mnear);
-                if (COUNT) {
-                    int src = kind == 1 ? ne[node] - nb[node] : 0;
-                    for (int o = 32; o > 0; o >>= 1) src += __shfl_xor(src, o, 64);
-                    cs += src;
+                if (COUNT && kind == 1) cs += ne[node] - nb[node];  // this lane's part
+                // children of opened nodes (at most 8) join the next frontier
+                // in lane order: prefix sums of the counts' bits by ballot
+                const int open = kind == 2 ? nc : 0;
+                int excl = 0, total = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const unsigned long long mb = __ballot((open >> b) & 1);
+                    excl += __popcll(mb & below) << b;
+                    total += __popcll(mb) << b;
                 }
-                // children of opened nodes join the next frontier in lane order
-                int open = kind == 2 ? nc : 0, excl = open;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int v = __shfl_up(excl, o, 64);
-                    if (lane >= o) excl += v;
-                }
-                const int total = __shfl(excl, 63, 64);
-                excl -= open;
                 if (nnext + total > fcap) {
                     ovf = true;
                     break;
                 }
-                if (kind == 2) {
-                    const int fc = first_child[node];
-                    for (int c = 0; c < nc; ++c) nxt[nnext + excl + c] = fc + c;
-                }
+                if (kind == 2)
+                    for (int c = 0; c < nc; ++c) {
+                        const int k = nnext + excl + c;
+                        if (k < kListLds) lnxt[k] = fc + c;
+                        else nxt[k] = fc + c;
+                    }
                 nnext += total;
             }
             __syncthreads();
             int* tmp = cur;
             cur = nxt;
             nxt = tmp;
+            tmp = lcur;
+            lcur = lnxt;
+            lnxt = tmp;
             ncur = nnext;
         }
         if (ovf) {
             if (lane == 0) atomicOr(overflow, 1);
             continue;
         }
+        if (COUNT)
+            for (int o = 32; o > 0; o >>= 1) cs += __shfl_xor(cs, o, 64);
         if (COUNT && lane == 0) {
             nfar[li] = static_cast<int>(cf);
             nnear[li] = static_cast<int>(cn);
@@ -1214,6 +1237,240 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
     }
 }
 
+// Near field in FP32 with the sources across the lanes.  A chunk of c <= 24
+// targets is evaluated for NT = c rounded up to a multiple of 4 target
+// slots, each with one FP32 sum per component in every lane's registers
+// (the spare slots hold a padding target whose sums are never written).
+// The item's sources, 64 near leaves at a time laid end to end, pass 64 per
+// step: lane k takes source k of the step, whose leaf it finds from a bitmap
+// of the leaves' first sources (one 64-bit word and its prefix count), and
+// adds its pair term to every slot, the target read from LDS (the same
+// address for all lanes).  A source is its FP32 record (offset from its own
+// leaf's centre, G m; near_records_kernel) plus the FP32 offset of its
+// leaf's centre from the target leaf's; the next step's record is loaded
+// while this step is evaluated.  After each piece of near leaves the slot
+// sums are added over the wave by the far field's fixed network
+// (row_quad_sum, then the four row sums in row order) into FP64 sums held
+// by lane t for target t.
+// The self pair (fmm_solver.cpp:295-310) adds exactly zero here: its offset
+// is zero (the target leaf's centre offset is 0, so the source is its
+// record, the target's own offset), and its factor G m r^3 is finite (the
+// caller uses this kernel only when G times the largest mass times eps^-3
+// is far below FP32's range) or zero (SAFE: a zero distance gives r = 0).
+constexpr int kNearTargets = 24;
+
+// Source records of the listed leaves' particles (sorted positions): offset
+// from the leaf's centre and G m, in FP32.  One block per leaf.
+__global__ __launch_bounds__(kEvalLanes) void near_records_kernel(
+    const int* __restrict__ leaves, const double* __restrict__ geom, const int* __restrict__ nb,
+    const int* __restrict__ ne, const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ sz, const double* __restrict__ sm, double G,
+    float4* __restrict__ rec) {
+    const int leaf = leaves[blockIdx.x];
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    for (int e = nb[leaf] + static_cast<int>(threadIdx.x); e < ne[leaf]; e += kEvalLanes)
+        rec[e] = make_float4(static_cast<float>(sx[e] - cx), static_cast<float>(sy[e] - cy),
+                             static_cast<float>(sz[e] - cz), static_cast<float>(G * sm[e]));
+}
+
+// A piece of near leaves in LDS: per leaf (first sorted index - first piece
+// index, centre offset from the target leaf's centre as FP32 bits), the
+// bitmap of first sources and each word's count of earlier first sources;
+// pieces of more than 64 x 64 sources use a binary search over qpre.
+struct NearPiece {
+    int4 info[kNearPiece];
+    unsigned long long starts[kNearPiece];
+    int wbase[kNearPiece];
+    int qpre[kNearPiece + 1];
+};
+
+template <bool SAFE, int NT>
+__device__ __forceinline__ void near_cells_chunk(
+    const NearItem& item, double cx, double cy, double cz, float pad, float eps2,
+    const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
+    const float4* __restrict__ srec, long long no, const int* __restrict__ near,
+    const float4* tgt, NearPiece& pc, float (&red)[4][kRowQuad][4], double (&a64)[3]) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x, q = lane & 3, row = lane >> 4;
+    for (int q0 = item.q0; q0 < item.q1; q0 += kNearPiece) {
+        const int kq = min(kNearPiece, item.q1 - q0);
+        __syncthreads();
+        int len = 0, first = 0;
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+        if (lane < kq) {
+            const int sl = near[no + q0 + lane];
+            first = nb[sl];
+            len = ne[sl] - first;
+            ox = static_cast<float>(geom[4 * sl] - cx);
+            oy = static_cast<float>(geom[4 * sl + 1] - cy);
+            oz = static_cast<float>(geom[4 * sl + 2] - cz);
+        }
+        int incl = len;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const int excl = incl - len;
+        const int total = __shfl(incl, 63, 64);
+        const bool bits = total <= kNearPiece * 64;  // uniform
+        pc.info[lane] = make_int4(first - excl, __float_as_int(ox), __float_as_int(oy), __float_as_int(oz));
+        pc.qpre[lane + 1] = incl;
+        if (lane == 0) pc.qpre[0] = 0;
+        pc.starts[lane] = 0ull;
+        __syncthreads();
+        if (bits && lane < kq) atomicOr(&pc.starts[excl >> 6], 1ull << (excl & 63));
+        __syncthreads();
+        if (bits) {
+            const int cnt = __popcll(pc.starts[lane]);
+            int wi = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(wi, o, 64);
+                if (lane >= o) wi += v;
+            }
+            pc.wbase[lane] = wi - cnt;
+        }
+        __syncthreads();
+        // leaf slot of piece source j (-1 past the end)
+        auto locate = [&](int j) -> int {
+            if (j >= total) return -1;
+            if (bits) {
+                const int w = j >> 6;
+                return pc.wbase[w] + __popcll(pc.starts[w] << (63 - (j & 63))) - 1;
+            }
+            int lo = 0, hi = kq - 1;  // last leaf with qpre <= j
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (pc.qpre[mid] <= j) lo = mid;
+                else hi = mid - 1;
+            }
+            return lo;
+        };
+        float acc[NT][3];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = acc[t][2] = 0.0f;
+        int nlo = locate(lane);
+        int4 ninf = make_int4(0, 0, 0, 0);
+        float4 nrec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (nlo >= 0) {
+            ninf = pc.info[nlo];
+            nrec = srec[lane + ninf.x];
+        }
+        for (int cb = 0; cb < total; cb += kEvalLanes) {
+            float x = pad, y = pad, z = pad, gm = 0.0f;
+            if (nlo >= 0) {
+                x = __int_as_float(ninf.y) + nrec.x;
+                y = __int_as_float(ninf.z) + nrec.y;
+                z = __int_as_float(ninf.w) + nrec.z;
+                gm = nrec.w;
+            }
+            const int jn = cb + kEvalLanes + lane;
+            nlo = cb + kEvalLanes < total ? locate(jn) : -1;
+            if (nlo >= 0) {
+                ninf = pc.info[nlo];
+                nrec = srec[jn + ninf.x];
+            }
+            // the targets stay in LDS (read each step, not held in registers)
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const float4 T = tgt[t];
+                const float dx = x - T.x, dy = y - T.y, dz = z - T.z;
+                const float s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+                float r;
+                if (SAFE) {
+                    r = s2 > 1e-30f ? gc_rsq(s2) : 0.0f;
+                } else {
+                    r = gc_rsq(s2);
+                }
+                const float kk = (gm * r) * (r * r);
+                acc[t][0] = gc_fma(dx, kk, acc[t][0]);
+                acc[t][1] = gc_fma(dy, kk, acc[t][1]);
+                acc[t][2] = gc_fma(dz, kk, acc[t][2]);
+            }
+        }
+#pragma unroll
+        for (int t0 = 0; t0 < NT; t0 += kRowQuad) {
+            float r[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                r[d] = row_quad_sum(acc[t0][d], acc[t0 + 1][d], acc[t0 + 2][d], acc[t0 + 3][d], lane);
+            __syncthreads();
+            if ((lane & 15) < kRowQuad)
+                for (int d = 0; d < 3; ++d) red[row][q][d] = r[d];
+            __syncthreads();
+            if ((lane >> 2) == (t0 >> 2)) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    a64[d] += static_cast<double>(((red[0][q][d] + red[1][q][d]) + red[2][q][d]) +
+                                                  red[3][q][d]);
+            }
+        }
+    }
+}
+
+template <bool SAFE>
+__global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4))) void eval_near_cells_kernel(
+    const NearItem* __restrict__ items, const int* __restrict__ leaves,
+    const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
+    const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
+    const float4* __restrict__ srec, const long long* __restrict__ near_off,
+    const int* __restrict__ near, double eps, const int* __restrict__ tpos,
+    const int* __restrict__ toff, const int* __restrict__ tcnt, double* __restrict__ part) {
+    __shared__ float4 tgt[kNearTargets];
+    __shared__ int tgp[kNearTargets];  // sorted positions of the chunk's targets
+    __shared__ NearPiece pc;
+    __shared__ float red[4][kRowQuad][4];
+    const NearItem item = items[blockIdx.x];
+    const int leaf = leaves[item.leaf];
+    const int lb = nb[leaf], lsize = ne[leaf] - lb;
+    // targets: the leaf's particles, or its entries of the range's target list
+    const int tb = tpos ? toff[item.leaf] : lb;
+    const int te = tpos ? tb + tcnt[item.leaf] : lb + lsize;
+    if (tb >= te) return;
+    const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
+    // padding sources and targets sit here: at least 1 + 3 half-widths from
+    // the leaf's targets, so their terms stay finite
+    const float pad = static_cast<float>(4.0 * geom[4 * leaf + 3] + 1.0);
+    const float eps2 = static_cast<float>(eps * eps);
+    const long long no = near_off[item.leaf];
+    const int lane = threadIdx.x;
+    for (int base = tb; base < te; base += kNearTargets) {
+        const int c = min(kNearTargets, te - base);
+        __syncthreads();
+        if (lane < kNearTargets) {
+            if (lane < c) {
+                const int pos = target_pos(tpos, base, lane);
+                tgt[lane] = make_float4(static_cast<float>(sx[pos] - cx), static_cast<float>(sy[pos] - cy),
+                                        static_cast<float>(sz[pos] - cz), 0.0f);
+                tgp[lane] = pos;
+            } else {
+                tgt[lane] = make_float4(-pad, -pad, -pad, 0.0f);
+                tgp[lane] = -1;
+            }
+        }
+        double a64[3] = {0.0, 0.0, 0.0};  // lane t: target t
+#define GC_NC(NT)                                                                                  \
+    near_cells_chunk<SAFE, NT>(item, cx, cy, cz, pad, eps2, geom, nb, ne, srec, no, near, tgt, pc,    \
+                               red, a64)
+        switch ((c + 3) / 4) {  // uniform per block
+        case 1: GC_NC(4); break;
+        case 2: GC_NC(8); break;
+        case 3: GC_NC(12); break;
+        case 4: GC_NC(16); break;
+        case 5: GC_NC(20); break;
+        default: GC_NC(24); break;
+        }
+#undef GC_NC
+        __syncthreads();
+        if (lane < c) {
+            double* out = part + 3 * (static_cast<size_t>(item.slot) + (tgp[lane] - lb));
+            out[0] = a64[0];
+            out[1] = a64[1];
+            out[2] = a64[2];
+        }
+    }
+}
+
 // a = far (already in ax) + the leaf's near item partials in item order;
 // optional closing half kick.  One thread per target.
 __global__ __launch_bounds__(kEvalLanes) void near_merge_kernel(
@@ -1562,6 +1819,10 @@ struct ToI64 {
 };
 
 // Partial rows a leaf's near items need: items x targets.
+struct AbsValue {
+    __host__ __device__ double operator()(double v) const { return v < 0.0 ? -v : v; }
+};
+
 struct RowsPerLeaf {
     const int *leaves, *nb, *ne, *icount;
     __host__ __device__ long long operator()(int li) const {
@@ -1664,6 +1925,9 @@ struct FmmEngine::Impl {
     double* npart = nullptr;
     int nitems = 0;
     unsigned long long* stats_dev = nullptr;
+    double mass_abs_max = 0.0;  // largest |m| of the last build
+    float4* srec = nullptr;     // FP32 source records of the near field
+    size_t srec_cap = 0;
     // Barnes-Hut work counts (M2P, pairs, leaves) of the last evaluation, in
     // pinned host memory; read once the evaluation has finished
     unsigned long long* tree_counts = nullptr;
@@ -1679,7 +1943,7 @@ struct FmmEngine::Impl {
                         kc64, kc32, nb, ne, first_child, nchild, bounds, count, offset,
                         scan_tmp, overflow, lvl, deepest, geom_root, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
                         near_off, frontier, far, near, stash_far, stash_near, lpos_far, lpos_near, wtot, icount, ioff, row_off, items, npart, stats_dev,
-                        tcnt, toff, tpos};
+                        tcnt, toff, tpos, srec};
         for (void* p : ptrs) (void)hipFree(p);
         if (tree_counts) (void)hipHostFree(tree_counts);
         for (auto& e : ev)
@@ -1852,6 +2116,16 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipMemsetAsync(overflow, 0, sizeof(int), st), "clear flag");
     hipLaunchKernelGGL(gather_sorted_kernel, dim3(pblocks), dim3(256), 0, st, n, idx_sorted,
                        s.x, s.y, s.z, s.m, sxs, sys, szs, sms);
+    {  // largest |m|, read with the statistics (bounds the near field's self term)
+        hipcub::TransformInputIterator<double, AbsValue, const double*> am(sms, AbsValue{});
+        double* mx = reinterpret_cast<double*>(stats_dev + kStatMassMax);
+        size_t need = 0;
+        FMM_HIP(hipcub::DeviceReduce::Max(nullptr, need, am, mx, n, st), "size mass reduction");
+        if (need > scan_tmp_bytes)
+            FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
+                    "allocate scan storage");
+        FMM_HIP(hipcub::DeviceReduce::Max(scan_tmp, scan_tmp_bytes, am, mx, n, st), "mass reduction");
+    }
 
     // 4: levels.  Levels 0..lmax hold internal nodes; level lmax + 1 only
     // leaves (it is split too, to mark them).  A level's node count is at
@@ -2128,9 +2402,10 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
     hipLaunchKernelGGL(stats_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves, leaves,
                        nb, ne, nfar, nnear, nsrc, stats_dev);
-    unsigned long long sv[kStatInternal + 1];
+    unsigned long long sv[kStatMassMax + 1];
     FMM_HIP(hipMemcpyAsync(sv, stats_dev, sizeof(sv), hipMemcpyDeviceToHost, st), "read stats");
     FMM_HIP(hipStreamSynchronize(st), "stats");
+    std::memcpy(&mass_abs_max, &sv[kStatMassMax], sizeof(double));
     last.node_count = 1 + 8 * static_cast<size_t>(sv[kStatInternal]);
     last.leaf_count = static_cast<size_t>(nleaves);
     last.far_interactions = sv[0];
@@ -2223,12 +2498,30 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     hipLaunchKernelGGL((eval_far_cells_kernel<O>), grid, block, 0, st, leaves, geom, mcom, kc32, nb, \
                        ne, idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, tp, toff, \
                        tcnt, s.ax, s.ay, s.az)
+#define GC_NEAR32(S)                                                                             \
+    hipLaunchKernelGGL((eval_near_cells_kernel<S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
+                       items, leaves, geom, nb, ne, sxs, sys, szs, srec, near_off, near, eps, tp,  \
+                       toff, tcnt, npart)
     if (mixed) {
         if (order == 0) GC_FAR32(0);
         else if (order == 2) GC_FAR32(2);
         else GC_FAR32(4);
-        if (safe) GC_NEAR(float, true);
-        else GC_NEAR(float, false);
+        // the sources-across-lanes kernel adds the self pair as an exact
+        // zero, which needs its factor G m eps^-3 finite in FP32
+        const double e2 = static_cast<double>(static_cast<float>(eps * eps));
+        const double rs = e2 > 0.0 ? 1.0 / std::sqrt(e2) : HUGE_VAL;
+        const bool self_zero = safe || (rs * rs <= 1e30 &&
+                                        std::fabs(G) * mass_abs_max * std::max(rs, rs * rs * rs) <= 1e30);
+        if (self_zero) {
+            if (static_cast<size_t>(s.n) > srec_cap)
+                FMM_HIP(realloc_dev(&srec, &srec_cap, static_cast<size_t>(s.n)), "allocate source records");
+            hipLaunchKernelGGL(near_records_kernel, grid, block, 0, st, leaves, geom, nb, ne, sxs, sys, szs,
+                               sms, G, srec);
+            if (safe) GC_NEAR32(true);
+            else GC_NEAR32(false);
+        } else {
+            GC_NEAR(float, false);
+        }
     } else {
         if (order == 0) GC_FAR(double, 0, kc64);
         else if (order == 2) GC_FAR(double, 2, kc64);
@@ -2238,6 +2531,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     }
 #undef GC_FAR
 #undef GC_FAR32
+#undef GC_NEAR32
 #undef GC_NEAR
     hipLaunchKernelGGL(near_merge_kernel, grid, block, 0, st, leaves, nb, ne, idx_sorted, icount, ioff,
                        items, npart, b, e, tp, toff, tcnt, s.ax, s.ay, s.az, s.vx, s.vy, s.vz,

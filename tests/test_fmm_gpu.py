@@ -56,6 +56,18 @@ def test_plummer_against_oracle(gpu, theta, leaf, order, eps):
     _check(gpu, pos, m, eps, theta, leaf, order)
 
 
+def test_tiny_softening_checks_the_self_pair(gpu):
+    # eps = 1e-13: the self pair's factor G m eps^-3 (1e39) would overflow
+    # FP32, so the mixed near field drops the self pair by comparing sorted
+    # indices instead of adding it as an exact zero; both must match the oracle
+    pos, vel, m = gpu.generators.plummer(3000, seed=13)
+    for eps in (1e-13, 1e-3):
+        want, _ = O.fmm(pos, m, 1.0, eps, 0.5, 16, 32, 4)
+        got, _ = _fmm(gpu, "mixed", pos, m, eps, 0.5, 16)
+        assert np.isfinite(got).all()
+        assert rel_err(got, want).max() < MIXED_TOL, eps
+
+
 def test_two_disk_and_reference_smoke_threshold(gpu):
     pos, vel, m = gpu.generators.config0()
     _check(gpu, pos, m, 0.05, 0.5, 16)
