@@ -236,45 +236,51 @@ __device__ __forceinline__ int digit_at(const unsigned long long* keys,
 // Children of the nodes of level L, nodes [lvl[L], lvl[L + 1]): bounds[9]
 // in sorted order and the non-empty count.  Launched over `bound` >= the
 // level's node count; entries past it get count 0, so the scan over
-// `bound` entries is the scan over the level.
+// `bound` entries is the scan over the level.  Eight lanes per node: lane d
+// finds the first position whose digit is >= d by its own binary search.
+constexpr int kSplitLanes = 8;
+
 __global__ __launch_bounds__(256) void split_count_kernel(
     int level, int bound, const int* __restrict__ lvl, int cap, int max_depth,
     const unsigned long long* __restrict__ keys, const unsigned long long* __restrict__ keys_lo,
     const int* __restrict__ nb, const int* __restrict__ ne, int* __restrict__ bounds,
     int* __restrict__ count, int* __restrict__ overflow) {
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int j = t / kSplitLanes, d = t % kSplitLanes;  // a group's exits are uniform (one j)
     if (j >= bound) return;
     const int k = lvl[level] + j;
     if (k >= lvl[level + 1]) {
-        count[j] = 0;
+        if (d == 0) count[j] = 0;
         return;
     }
     const int b = nb[k], e = ne[k];
     int* bd = bounds + 9 * j;
     const bool internal = (e - b) > cap && level < max_depth;
     if (!internal) {
-        count[j] = 0;
+        if (d == 0) count[j] = 0;
         return;
     }
     if (level >= kMaxLevels || (level >= kLevels && !keys_lo)) {  // excluded by the host
-        atomicOr(overflow, 1);
-        count[j] = 0;
+        if (d == 0) {
+            atomicOr(overflow, 1);
+            count[j] = 0;
+        }
         return;
     }
-    bd[0] = b;
-    bd[8] = e;
-    for (int d = 1; d < 8; ++d) {  // first position with digit >= d
-        int lo = b, hi = e;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (digit_at(keys, keys_lo, mid, level + 1) < d) lo = mid + 1;
-            else hi = mid;
-        }
-        bd[d] = lo;
+    int lo = b, hi = e;  // first position with digit >= d (b for d = 0)
+    while (d > 0 && lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (digit_at(keys, keys_lo, mid, level + 1) < d) lo = mid + 1;
+        else hi = mid;
     }
-    int c = 0;
-    for (int d = 0; d < 8; ++d) c += bd[d + 1] > bd[d] ? 1 : 0;
-    count[j] = c;
+    bd[d] = lo;
+    if (d == 0) bd[8] = e;
+    const int up = __shfl_down(lo, 1, kSplitLanes);
+    int c = (d == kSplitLanes - 1 ? e : up) > lo ? 1 : 0;  // child d is non-empty
+    c += __shfl_xor(c, 1, kSplitLanes);
+    c += __shfl_xor(c, 2, kSplitLanes);
+    c += __shfl_xor(c, 4, kSplitLanes);
+    if (d == 0) count[j] = c;
 }
 
 // Emits level L + 1 (children in digit order, parents in order) at
@@ -337,10 +343,23 @@ __global__ __launch_bounds__(256) void internal_count_kernel(int nnodes, const i
 #include "upward_generated.inc"
 #undef GC_UP_INLINE
 
-// One thread per node of a level: mass and centre of mass, then the raw
+// Eight lanes per node of a level: mass and centre of mass, then the raw
 // moments about the COM — P2M over the leaf's particles (key order, the
-// sorted copies) or M2M over the children (multipole.cpp:226-261,
-// fmm_solver.cpp:157-208).  Moments stay in registers (generated code).
+// sorted copies; lane g takes particles g, g + 8, ...) or M2M over the
+// children (lane g takes child g) (multipole.cpp:226-261,
+// fmm_solver.cpp:157-208).  The lanes' partial sums are added by a fixed
+// butterfly over the 8 lanes (every lane ends with the same sums).
+// Moments stay in registers (generated code).
+constexpr int kUpLanes = 8;
+
+__device__ __forceinline__ double up_sum(double v) {
+#pragma clang fp contract(off)
+    v += __shfl_xor(v, 1, kUpLanes);
+    v += __shfl_xor(v, 2, kUpLanes);
+    v += __shfl_xor(v, 4, kUpLanes);
+    return v;
+}
+
 __global__ __launch_bounds__(64) void upward_kernel(
     int lb, int le, const int* __restrict__ nb, const int* __restrict__ ne,
     const int* __restrict__ first_child, const int* __restrict__ nchild,
@@ -348,49 +367,60 @@ __global__ __launch_bounds__(64) void upward_kernel(
     const double* __restrict__ sz, const double* __restrict__ sm, double* __restrict__ mcom,
     double* __restrict__ mom) {
 #pragma clang fp contract(off)
-    const int k = lb + blockIdx.x * 64 + threadIdx.x;
-    if (k >= le) return;
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const int g = t & (kUpLanes - 1);
+    // a group past the level's end still runs its shuffles (all lanes of
+    // a group take the same path), then writes nothing
+    const int k = lb + t / kUpLanes;
+    const bool live = k < le;
+    const int kk = live ? k : lb;
     double mass = 0.0, w[3] = {0.0, 0.0, 0.0}, M[kMom];
 #pragma unroll
     for (int a = 0; a < kMom; ++a) M[a] = 0.0;
-    const int nc = nchild[k];
-    const int b = nb[k], e = ne[k];
-    const int fc = first_child[k];
+    const int nc = nchild[kk];
+    const int b = nb[kk], e = ne[kk];
+    const int fc = first_child[kk];
     if (nc == 0) {
-        for (int j = b; j < e; ++j) {
+        for (int j = b + g; j < e; j += kUpLanes) {
             const double mj = sm[j];
             mass += mj;
             w[0] += sx[j] * mj;
             w[1] += sy[j] * mj;
             w[2] += sz[j] * mj;
         }
-    } else {
-        for (int c = fc; c < fc + nc; ++c) {
-            const double cm = mcom[4 * c];
-            mass += cm;
-            for (int d = 0; d < 3; ++d) w[d] += mcom[4 * c + 1 + d] * cm;
-        }
+    } else if (g < nc) {
+        const int c = fc + g;
+        const double cm = mcom[4 * c];
+        mass += cm;
+        for (int d = 0; d < 3; ++d) w[d] += mcom[4 * c + 1 + d] * cm;
     }
+    mass = up_sum(mass);
+    for (int d = 0; d < 3; ++d) w[d] = up_sum(w[d]);
     double com[3];
-    for (int d = 0; d < 3; ++d) com[d] = mass > 0.0 ? w[d] / mass : geom[4 * k + d];
+    for (int d = 0; d < 3; ++d) com[d] = mass > 0.0 ? w[d] / mass : geom[4 * kk + d];
     if (nc == 0) {
-        for (int j = b; j < e; ++j)
+        for (int j = b + g; j < e; j += kUpLanes)
             p2m_add(sx[j] - com[0], sy[j] - com[1], sz[j] - com[2], sm[j], M);
-    } else {
-        for (int c = fc; c < fc + nc; ++c) {
-            double C[kMom];
-            const double* cmom = mom + static_cast<size_t>(kMom) * c;
+    } else if (g < nc) {
+        const int c = fc + g;
+        double C[kMom];
+        const double* cmom = mom + static_cast<size_t>(kMom) * c;
 #pragma unroll
-            for (int a = 4; a < kMom; ++a) C[a] = cmom[a];
-            m2m_add(mcom[4 * c + 1] - com[0], mcom[4 * c + 2] - com[1], mcom[4 * c + 3] - com[2],
-                    mcom[4 * c], C, M);
-        }
+        for (int a = 4; a < kMom; ++a) C[a] = cmom[a];
+        m2m_add(mcom[4 * c + 1] - com[0], mcom[4 * c + 2] - com[1], mcom[4 * c + 3] - com[2],
+                mcom[4 * c], C, M);
     }
-    mcom[4 * k] = mass;
-    for (int d = 0; d < 3; ++d) mcom[4 * k + 1 + d] = com[d];
+#pragma unroll
+    for (int a = 0; a < kMom; ++a) M[a] = up_sum(M[a]);
+    if (!live) return;
+    if (g == 0) {
+        mcom[4 * k] = mass;
+        for (int d = 0; d < 3; ++d) mcom[4 * k + 1 + d] = com[d];
+    }
     double* out = mom + static_cast<size_t>(kMom) * k;
 #pragma unroll
-    for (int a = 0; a < kMom; ++a) out[a] = M[a];
+    for (int a = 0; a < kMom; ++a)
+        if (a % kUpLanes == g) out[a] = M[a];
 }
 
 // Per node, the far-field coefficient table of far_m2p.hpp from the FP64
@@ -2168,7 +2198,8 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     for (int L = 0; L < nlev; ++L) {
         const int bl = static_cast<int>(bound[L]);
         const unsigned blocks = static_cast<unsigned>((bl + 255) / 256);
-        hipLaunchKernelGGL(split_count_kernel, dim3(blocks), dim3(256), 0, st, L, bl, lvl, cap,
+        const unsigned sblocks = static_cast<unsigned>((static_cast<size_t>(bl) * kSplitLanes + 255) / 256);
+        hipLaunchKernelGGL(split_count_kernel, dim3(sblocks), dim3(256), 0, st, L, bl, lvl, cap,
                            max_depth, keys_sorted, klo, nb, ne, bounds, count, overflow);
         FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, count, offset, bl, st),
                 "scan children");
@@ -2198,8 +2229,8 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     for (int level = depth; level >= 0; --level) {
         const int lb = level_begin[level], le = level_begin[level + 1];
         if (le <= lb) continue;
-        hipLaunchKernelGGL(upward_kernel, dim3((le - lb + 63) / 64), dim3(64), 0, st, lb, le, nb, ne,
-                           first_child, nchild, geom, sxs, sys, szs, sms, mcom, mom);
+        hipLaunchKernelGGL(upward_kernel, dim3((kUpLanes * (le - lb) + 63) / 64), dim3(64), 0, st, lb, le,
+                           nb, ne, first_child, nchild, geom, sxs, sys, szs, sms, mcom, mom);
     }
     hipLaunchKernelGGL(far_coef_kernel, dim3((nnodes + 63) / 64), dim3(64), 0, st, nnodes, mcom, mom,
                        kc64, kc32);
