@@ -52,9 +52,10 @@ constexpr int kMom = 35;          // raw Cartesian moments, total degree <= 4
 constexpr int kEvalLanes = 64;
 // device statistics: far, near, M2P, pairs (FMM) or M2P, pairs, leaves
 // (tree), then internal nodes at kStatInternal
-constexpr int kStats = 8;
+constexpr int kStats = 16;
 constexpr int kStatInternal = 4;
 constexpr int kStatMassMax = 5;  // largest |m| of the build (double bits)
+constexpr int kStatTotals = 8;   // build totals read with the statistics (build_totals_kernel)
 
 __device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double gc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -1849,6 +1850,28 @@ struct ToI64 {
 };
 
 // Partial rows a leaf's near items need: items x targets.
+// The totals a build's host side sizes its buffers by, read in one copy
+// with the statistics: list overflow flags, far and near list lengths,
+// near work items and their partial rows.
+__global__ void build_totals_kernel(int nleaves, const int* __restrict__ leaves,
+                                    const int* __restrict__ nb, const int* __restrict__ ne,
+                                    const int* __restrict__ nfar, const int* __restrict__ nnear,
+                                    const long long* __restrict__ far_off,
+                                    const long long* __restrict__ near_off,
+                                    const int* __restrict__ icount, const int* __restrict__ ioff,
+                                    const long long* __restrict__ row_off,
+                                    const int* __restrict__ overflow,
+                                    unsigned long long* __restrict__ out) {
+    const int l = nleaves - 1;
+    const int leaf = leaves[l];
+    out[0] = static_cast<unsigned long long>(*overflow);
+    out[1] = static_cast<unsigned long long>(far_off[l] + nfar[l]);
+    out[2] = static_cast<unsigned long long>(near_off[l] + nnear[l]);
+    out[3] = static_cast<unsigned long long>(ioff[l] + icount[l]);
+    out[4] = static_cast<unsigned long long>(row_off[l] +
+                                             static_cast<long long>(icount[l]) * (ne[leaf] - nb[leaf]));
+}
+
 struct AbsValue {
     __host__ __device__ double operator()(double v) const { return v < 0.0 ? -v : v; }
 };
@@ -2295,7 +2318,30 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         FMM_HIP(realloc_dev(&lpos_near, &c2, static_cast<size_t>(nleaves) * 2), "allocate list stash");
         FMM_HIP(realloc_dev(&lpos_far, &lpos_cap, c2), "allocate list stash");
     }
-    int lovf = 0;
+    if (static_cast<size_t>(nleaves) > icap) {
+        size_t c2 = 0, c3 = 0;
+        icap = 0;  // set by the last allocation only
+        FMM_HIP(realloc_dev(&ioff, &c2, static_cast<size_t>(nleaves) * 2), "allocate near items");
+        FMM_HIP(realloc_dev(&row_off, &c3, c2), "allocate near items");
+        FMM_HIP(realloc_dev(&icount, &icap, c3), "allocate near items");
+    }
+    // 64-bit offsets: widen the counts with a scan over long long
+    auto widen = [&](const int* in, long long* out) -> hipError_t {
+        size_t need = 0;
+        hipcub::TransformInputIterator<long long, ToI64, const int*> it(in, ToI64());
+        hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, out, nleaves, st);
+        if (e != hipSuccess) return e;
+        if (need > scan_tmp_bytes) {
+            e = realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need);
+            if (e != hipSuccess) return e;
+        }
+        return hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, it, out, nleaves, st);
+    };
+    // The walk counts (and stashes) the lists; everything that depends
+    // only on the counts follows on the device (list offsets, near work
+    // items, statistics, totals), and one copy reads the totals.  A frontier
+    // overflow repeats the walk with larger frontiers.
+    unsigned long long sv[kStatTotals + 5];
     for (;;) {
         const size_t need = static_cast<size_t>(lwaves) * 2 * fcap;
         if (need > frontier_ints)
@@ -2311,37 +2357,48 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
                                geom, mcom, first_child, nchild, nb, ne, theta, nfar, nnear, nsrc,
                                far_off, near_off, 0ll, 0ll, lpos_far, lpos_near, far, near,
                                frontier, fcap, overflow);
-        FMM_HIP(hipMemcpyAsync(&lovf, overflow, sizeof(int), hipMemcpyDeviceToHost, st), "read flag");
+        FMM_HIP(widen(nfar, far_off), "scan far lists");
+        FMM_HIP(widen(nnear, near_off), "scan near lists");
+        // near-field work items: leaf li gets max(1, ceil(nnear / kNearItem))
+        // items, each with one partial row per target of the leaf
+        hipLaunchKernelGGL(near_item_count_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st,
+                           nleaves, nnear, icount);
+        {
+            size_t need1 = 0;
+            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need1, icount, ioff, nleaves, st), "size scan");
+            RowsPerLeaf rows_fn{leaves, nb, ne, icount};
+            hipcub::CountingInputIterator<int> ci(0);
+            hipcub::TransformInputIterator<long long, RowsPerLeaf, hipcub::CountingInputIterator<int>> rit(ci, rows_fn);
+            size_t need2 = 0;
+            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, rit, row_off, nleaves, st), "size scan");
+            need1 = std::max(need1, need2);
+            if (need1 > scan_tmp_bytes)
+                FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need1),
+                        "allocate scan storage");
+            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, icount, ioff, nleaves, st),
+                    "scan near items");
+            FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, rit, row_off, nleaves, st),
+                    "scan near rows");
+        }
+        // statistics (FmmStats, fmm_solver.hpp:20-25, plus pair counts)
+        FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
+        hipLaunchKernelGGL(stats_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves, leaves,
+                           nb, ne, nfar, nnear, nsrc, stats_dev);
+        hipLaunchKernelGGL(build_totals_kernel, dim3(1), dim3(1), 0, st, nleaves, leaves, nb, ne, nfar,
+                           nnear, far_off, near_off, icount, ioff, row_off, overflow,
+                           stats_dev + kStatTotals);
+        FMM_HIP(hipMemcpyAsync(sv, stats_dev, sizeof(sv), hipMemcpyDeviceToHost, st), "read totals");
         FMM_HIP(hipStreamSynchronize(st), "list counts");
-        if (!(lovf & 1)) break;
+        if (!(sv[kStatTotals] & 1)) break;
         if (fcap >= (1 << 24)) {
             if (err) *err = "interaction-list frontier beyond 16M nodes";
             return GC_ERR_UNSUPPORTED;
         }
         fcap *= 4;
     }
-    // 64-bit offsets: widen the counts with a scan over long long
-    auto widen = [&](const int* in, long long* out) -> hipError_t {
-        size_t need = 0;
-        hipcub::TransformInputIterator<long long, ToI64, const int*> it(in, ToI64());
-        hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, need, it, out, nleaves, st);
-        if (e != hipSuccess) return e;
-        if (need > scan_tmp_bytes) {
-            e = realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need);
-            if (e != hipSuccess) return e;
-        }
-        return hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, it, out, nleaves, st);
-    };
-    FMM_HIP(widen(nfar, far_off), "scan far lists");
-    FMM_HIP(widen(nnear, near_off), "scan near lists");
-    long long lo[2] = {0, 0};
-    int lc[2] = {0, 0};
-    FMM_HIP(hipMemcpyAsync(&lo[0], far_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "list size");
-    FMM_HIP(hipMemcpyAsync(&lo[1], near_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "list size");
-    FMM_HIP(hipMemcpyAsync(&lc[0], nfar + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
-    FMM_HIP(hipMemcpyAsync(&lc[1], nnear + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "list size");
-    FMM_HIP(hipStreamSynchronize(st), "list sizes");
-    const long long total_far = lo[0] + lc[0], total_near = lo[1] + lc[1];
+    const int lovf = static_cast<int>(sv[kStatTotals]);
+    const long long total_far = static_cast<long long>(sv[kStatTotals + 1]);
+    const long long total_near = static_cast<long long>(sv[kStatTotals + 2]);
     if (static_cast<size_t>(total_far) + 1 > farcap)
         FMM_HIP(realloc_dev(&far, &farcap,
                             static_cast<size_t>(total_far) + 1 + static_cast<size_t>(total_far) / 4),
@@ -2379,47 +2436,8 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
             wcap_far = wcap_near = 0;
     }
 
-    // near-field work items: leaf li gets max(1, ceil(nnear / kNearItem))
-    // items, each with one partial row per target of the leaf
-    if (static_cast<size_t>(nleaves) > icap) {
-        size_t c2 = 0, c3 = 0;
-        icap = 0;  // set by the last allocation only
-        FMM_HIP(realloc_dev(&ioff, &c2, static_cast<size_t>(nleaves) * 2), "allocate near items");
-        FMM_HIP(realloc_dev(&row_off, &c3, c2), "allocate near items");
-        FMM_HIP(realloc_dev(&icount, &icap, c3), "allocate near items");
-    }
-    hipLaunchKernelGGL(near_item_count_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st,
-                       nleaves, nnear, icount);
-    {
-        size_t need = 0;
-        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, icount, ioff, nleaves, st), "size scan");
-        RowsPerLeaf rows_fn{leaves, nb, ne, icount};
-        hipcub::CountingInputIterator<int> ci(0);
-        hipcub::TransformInputIterator<long long, RowsPerLeaf, hipcub::CountingInputIterator<int>> rit(ci, rows_fn);
-        size_t need2 = 0;
-        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need2, rit, row_off, nleaves, st), "size scan");
-        need = std::max(need, need2);
-        if (need > scan_tmp_bytes)
-            FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
-                    "allocate scan storage");
-        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, icount, ioff, nleaves, st),
-                "scan near items");
-        FMM_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_tmp_bytes, rit, row_off, nleaves, st),
-                "scan near rows");
-    }
-    int ilast[2] = {0, 0};
-    long long rlast = 0;
-    int leaf_last = 0, nb_last = 0, ne_last = 0;
-    FMM_HIP(hipMemcpyAsync(&ilast[0], ioff + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "item count");
-    FMM_HIP(hipMemcpyAsync(&ilast[1], icount + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "item count");
-    FMM_HIP(hipMemcpyAsync(&rlast, row_off + nleaves - 1, sizeof(long long), hipMemcpyDeviceToHost, st), "row count");
-    FMM_HIP(hipMemcpyAsync(&leaf_last, leaves + nleaves - 1, sizeof(int), hipMemcpyDeviceToHost, st), "row count");
-    FMM_HIP(hipStreamSynchronize(st), "item count");
-    FMM_HIP(hipMemcpyAsync(&nb_last, nb + leaf_last, sizeof(int), hipMemcpyDeviceToHost, st), "row count");
-    FMM_HIP(hipMemcpyAsync(&ne_last, ne + leaf_last, sizeof(int), hipMemcpyDeviceToHost, st), "row count");
-    FMM_HIP(hipStreamSynchronize(st), "row count");
-    nitems = ilast[0] + ilast[1];
-    const size_t rows = static_cast<size_t>(rlast) + static_cast<size_t>(ilast[1]) * (ne_last - nb_last);
+    nitems = static_cast<int>(sv[kStatTotals + 3]);
+    const size_t rows = static_cast<size_t>(sv[kStatTotals + 4]);
     if (static_cast<size_t>(nitems) > itemcap)
         FMM_HIP(realloc_dev(&items, &itemcap,
                             static_cast<size_t>(nitems) + static_cast<size_t>(nitems) / 4),
@@ -2428,14 +2446,6 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
         FMM_HIP(realloc_dev(&npart, &partcap, 3 * rows + 3 * rows / 4), "allocate near partials");
     hipLaunchKernelGGL(near_item_fill_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves,
                        leaves, nb, ne, nnear, icount, ioff, row_off, items);
-
-    // statistics (FmmStats, fmm_solver.hpp:20-25, plus pair counts)
-    FMM_HIP(hipMemsetAsync(stats_dev, 0, sizeof(unsigned long long) * 4, st), "clear stats");
-    hipLaunchKernelGGL(stats_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves, leaves,
-                       nb, ne, nfar, nnear, nsrc, stats_dev);
-    unsigned long long sv[kStatMassMax + 1];
-    FMM_HIP(hipMemcpyAsync(sv, stats_dev, sizeof(sv), hipMemcpyDeviceToHost, st), "read stats");
-    FMM_HIP(hipStreamSynchronize(st), "stats");
     std::memcpy(&mass_abs_max, &sv[kStatMassMax], sizeof(double));
     last.node_count = 1 + 8 * static_cast<size_t>(sv[kStatInternal]);
     last.leaf_count = static_cast<size_t>(nleaves);
