@@ -56,6 +56,15 @@ def test_plummer_against_oracle(gpu, theta, leaf, order, eps):
     _check(gpu, pos, m, eps, theta, leaf, order)
 
 
+def test_large_leaves(gpu):
+    # leaves of up to 300 particles: target chunks of the near kernel (24)
+    # and far kernel (64) repeat within a leaf, and pieces of near leaves
+    # can exceed 64 x 64 sources (the near kernel then finds a source's
+    # leaf by binary search instead of the bitmap)
+    pos, vel, m = gpu.generators.plummer(6000, seed=17)
+    _check(gpu, pos, m, 0.02, 0.5, 300)
+
+
 def test_tiny_softening_checks_the_self_pair(gpu):
     # eps = 1e-13: the self pair's factor G m eps^-3 (1e39) would overflow
     # FP32, so the mixed near field drops the self pair by comparing sorted
