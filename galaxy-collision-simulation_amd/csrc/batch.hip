@@ -23,7 +23,11 @@
 #include "mixed_tiles.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
 #include <map>
 #include <mutex>
 #include <new>
@@ -173,6 +177,62 @@ struct Tuned {
 };
 std::mutex g_tune_mutex;
 std::map<std::tuple<int, int, long long, int>, Tuned> g_tune_cache;
+
+// The autotune results also persist in a text file, one line per result:
+//   gcsim-tune 1 <arch> <CUs> <runs> <tiles> <items at chunk 1> <requested
+//   form> <form> <chunk> <blocks> <ipt> <symmetric blocks> <item length>
+// so a new process on the same device model reuses them.  The file is
+// $GCSIM_TUNE_CACHE, else $HOME/.cache/gcsim/batch_tune.txt; an empty
+// GCSIM_TUNE_CACHE turns the file off.  The last line of a key wins; lines
+// that do not parse, or hold values out of range, are ignored.
+std::string tune_file_path() {
+    if (const char* e = std::getenv("GCSIM_TUNE_CACHE")) return e;
+    const char* h = std::getenv("HOME");
+    if (!h || !*h) return std::string();
+    return std::string(h) + "/.cache/gcsim/batch_tune.txt";
+}
+
+bool tune_file_lookup(const std::string& arch, int cus,
+                      const std::tuple<int, int, long long, int>& key, Tuned* out) {
+    const std::string path = tune_file_path();
+    if (path.empty()) return false;
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line;
+    bool found = false;
+    while (std::getline(f, line)) {
+        char tag[32] = {0}, a[128] = {0};
+        int ver = 0, c = 0, runs = 0, tiles = 0, req = 0;
+        long long items = 0;
+        Tuned t{};
+        const int got = std::sscanf(line.c_str(), "%31s %d %127s %d %d %d %lld %d %d %d %d %d %d %d", tag,
+                                    &ver, a, &c, &runs, &tiles, &items, &req, &t.form, &t.chunk,
+                                    &t.blocks, &t.ipt, &t.sym_blocks, &t.sym_len);
+        if (got != 14 || std::strcmp(tag, "gcsim-tune") != 0 || ver != 1) continue;
+        if (arch != a || c != cus || std::make_tuple(runs, tiles, items, req) != key) continue;
+        const bool ok = (t.form == GC_PAIRS_TARGETS || t.form == GC_PAIRS_SYMMETRIC) && t.chunk >= 1 &&
+                        t.blocks >= 0 && (t.ipt == 0 || t.ipt == 1 || t.ipt == 2 || t.ipt == 4) &&
+                        t.sym_blocks >= 0 && t.sym_len >= 0;
+        if (!ok) continue;
+        *out = t;
+        found = true;
+    }
+    return found;
+}
+
+void tune_file_store(const std::string& arch, int cus,
+                     const std::tuple<int, int, long long, int>& key, const Tuned& t) {
+    const std::string path = tune_file_path();
+    if (path.empty()) return;
+    std::error_code ec;
+    const std::filesystem::path dir = std::filesystem::path(path).parent_path();
+    if (!dir.empty()) std::filesystem::create_directories(dir, ec);
+    std::ofstream f(path, std::ios::app);
+    if (!f) return;  // a cache that cannot be written is only a cache
+    f << "gcsim-tune 1 " << arch << ' ' << cus << ' ' << std::get<0>(key) << ' ' << std::get<1>(key)
+      << ' ' << std::get<2>(key) << ' ' << std::get<3>(key) << ' ' << t.form << ' ' << t.chunk << ' '
+      << t.blocks << ' ' << t.ipt << ' ' << t.sym_blocks << ' ' << t.sym_len << '\n';
+}
 
 }  // namespace
 }  // namespace gc
@@ -750,10 +810,17 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
         }
     }
     const int requested = b->pairs;
+    hipDeviceProp_t prop;
+    GB_HIP(b, hipGetDeviceProperties(&prop, b->device), "device properties");
+    const int cus = std::max(1, prop.multiProcessorCount);
+    std::string arch(prop.gcnArchName);
+    if (arch.empty() || arch.find(' ') != std::string::npos) arch = "unknown";
+    if (!cached && tune_file_lookup(arch, cus, key, &best_t)) {
+        cached = true;
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[key] = best_t;
+    }
     if (!cached) {
-        hipDeviceProp_t prop;
-        GB_HIP(b, hipGetDeviceProperties(&prop, b->device), "device properties");
-        const int cus = std::max(1, prop.multiProcessorCount);
         double best = 1e30;
         auto time_it = [&](double* ms) -> gc_status {
             gc_status st = gc_batch_time_force_kernel(b, 1, ms, nullptr);  // warm
@@ -811,6 +878,7 @@ gc_status gc_batch_autotune(gc_batch* b, int* chunk_out, int* blocks_out) {
         }
         std::lock_guard<std::mutex> lock(g_tune_mutex);
         g_tune_cache[key] = best_t;
+        tune_file_store(arch, cus, key, best_t);
     }
     b->tuned_form = best_t.form;
     gc_status st = build_schedule(b, best_t.chunk);

@@ -245,8 +245,15 @@ gc_status gc_batch_synchronize(gc_batch* b);
 gc_status gc_batch_set_schedule(gc_batch* b, int chunk_tiles, int blocks, int graph_steps);
 gc_status gc_batch_get_schedule(gc_batch* b, int* chunk_tiles, int* blocks, int* graph_steps);
 /* Times schedule candidates on this batch's force kernel and keeps the
- * fastest; results are cached per process by batch shape and requested pair
- * form (gc_batch_set_pairs). */
+ * fastest; results are cached by device model (architecture name and CU
+ * count), batch shape and requested pair form (gc_batch_set_pairs), per
+ * process and in a text file that later processes read:
+ * $GCSIM_TUNE_CACHE, else $HOME/.cache/gcsim/batch_tune.txt (an empty
+ * GCSIM_TUNE_CACHE turns the file off).  One line per result:
+ *   gcsim-tune 1 <arch> <CUs> <runs> <tiles> <items at chunk 1>
+ *   <requested form> <form> <chunk> <blocks> <ipt> <symmetric blocks>
+ *   <item length>
+ * the last line of a key wins. */
 gc_status gc_batch_autotune(gc_batch* b, int* chunk_tiles, int* blocks);
 gc_status gc_batch_time_force_kernel(gc_batch* b, int reps, double* ms_per_launch,
                                      double* interactions_per_launch);
