@@ -204,6 +204,10 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
             __syncthreads();  // LDS reuse by the next item
             continue;
         }
+        // the next source tile of the item is loaded into registers while
+        // this one is evaluated
+        constexpr int kPer = kTile / kThreads;
+        float4 pre[kPer];
         for (int J = item.jb; J < item.je; ++J) {
             float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
             tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
@@ -211,12 +215,19 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
             for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
             const float4* src = jp + static_cast<size_t>(J) * kTile;
             {
-                for (int e = t; e < kTile; e += kThreads) {
-                    const float4 v = src[e];
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) {
+                    const int e = t + u * kThreads;
+                    const float4 v = J > item.jb ? pre[u] : src[e];
                     src2[e / kChunk][e % kChunk] = v;
                     src2[e / kChunk][kChunk + e % kChunk] = v;
                 }
                 __syncthreads();
+                if (J + 1 < item.je) {
+                    const float4* nsrc = src + kTile;
+#pragma unroll
+                    for (int u = 0; u < kPer; ++u) pre[u] = nsrc[t + u * kThreads];
+                }
                 if (pending >= 0) {
                     store_row(pending);
                     pending = -1;
