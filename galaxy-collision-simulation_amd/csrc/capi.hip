@@ -16,6 +16,34 @@
 
 using namespace gc;
 
+namespace {
+
+// Everything a captured chunk of gc_step bakes into its kernel arguments:
+// the graph is replayed only while all of it is unchanged.
+struct StepKey {
+    hipStream_t stream = nullptr;
+    double dt = 0.0, G = 0.0, eps = 0.0;
+    int solver = -1, precision = -1;
+    int ipt = 0, jsplit = 0, sources = 0, pairs = 0, items = 0, steps = 0;
+    size_t n = 0, cap = 0, ntiles = 0;
+    const void* ptr[9] = {};
+    int nitems = 0;
+    const void* row_start = nullptr;
+
+    bool operator==(const StepKey& o) const {
+        if (stream != o.stream || dt != o.dt || G != o.G || eps != o.eps || solver != o.solver ||
+            precision != o.precision || ipt != o.ipt || jsplit != o.jsplit ||
+            sources != o.sources || pairs != o.pairs || items != o.items || steps != o.steps ||
+            n != o.n || cap != o.cap || ntiles != o.ntiles)
+            return false;
+        for (int k = 0; k < 9; ++k)
+            if (ptr[k] != o.ptr[k]) return false;
+        return nitems == o.nitems && row_start == o.row_start;
+    }
+};
+
+}  // namespace
+
 struct gc_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;
@@ -39,6 +67,11 @@ struct gc_ctx {
     Src64* jp64 = nullptr;       // FP64 throughput form: packed sources
     double* part64 = nullptr;    // ... and its per-split partial sums
     size_t jp64_n = 0, part64_n = 0;
+    // gc_step: chunks of graph_steps steps replayed from a captured hipGraph
+    // (graph_steps <= 1: every kernel launched eagerly)
+    int graph_steps = 10;
+    hipGraphExec_t step_graph = nullptr;
+    StepKey step_key{};
     std::string err;
 };
 
@@ -302,6 +335,124 @@ gc_status force(gc_ctx* c, size_t begin, size_t end, bool close_kick, double hal
     return launch_status(c, "launch mixed direct kernels");
 }
 
+// Buffers of the full-range direct force, allocated before a capture:
+// force() then finds them in place and only launches kernels.
+gc_status prepare_direct(gc_ctx* c) {
+    const gc_params& p = c->params;
+    if (p.precision == GC_PRECISION_FP64) return GC_OK;
+    if (p.precision == GC_PRECISION_FP64_FAST) {
+        int ipt = 2, js = 1, per = 8;
+        fp64_fast_shape(c->s.n, c->s.n, c->shape.ipt, c->shape.jsplit, &ipt, &js, &per);
+        return ensure_fp64_fast(c, js);
+    }
+    if (symmetric_pairs(c, 0, c->s.n)) return ensure_sym(c);
+    int ipt = 1, js = 1;
+    mixed_shape(c, c->s.n, &ipt, &js);
+    return ensure_partials(c, js);
+}
+
+// k KDK steps, each its own opening kick + drift and force + closing kick.
+gc_status launch_plain_steps(gc_ctx* c, double dt, int k) {
+    for (int i = 0; i < k; ++i) {
+        launch_open_step(c->s, 0.5 * dt, dt, c->stream);
+        gc_status st = force(c, 0, c->s.n, true, 0.5 * dt);
+        if (st != GC_OK) return st;
+    }
+    return GC_OK;
+}
+
+// k middle steps of the fused mixed symmetric sequence: the force, then the
+// merge that closes the step, opens the next one and packs its sources.
+void launch_sym_middle(gc_ctx* c, double dt, int k) {
+    const int ipt = sym_ipt(c);
+    for (int i = 0; i < k; ++i) {
+        launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, ipt, c->stream);
+        launch_merge_sym_open(c->s, c->sym, c->mixed, c->params.gravitational_constant, 0.5 * dt,
+                              dt, c->stream);
+    }
+}
+
+StepKey make_step_key(const gc_ctx* c, double dt) {
+    StepKey k;
+    k.stream = c->stream;
+    k.dt = dt;
+    k.G = c->params.gravitational_constant;
+    k.eps = c->params.softening;
+    k.solver = c->params.solver;
+    k.precision = c->params.precision;
+    k.ipt = c->shape.ipt;
+    k.jsplit = c->shape.jsplit;
+    k.sources = c->shape.sources;
+    k.pairs = c->shape.pairs;
+    k.items = c->shape.items;
+    k.steps = c->graph_steps;
+    k.n = c->s.n;
+    k.cap = c->s.cap;
+    k.ntiles = c->mixed.ntiles;
+    const void* ptr[] = {c->s.base,    c->mixed.jp, c->mixed.centre, c->mixed.part,
+                         c->sym.items, c->sym.ipart, c->sym.jpart, c->jp64, c->part64};
+    static_assert(sizeof(ptr) == sizeof(k.ptr), "step key pointers");
+    std::memcpy(k.ptr, ptr, sizeof(ptr));
+    k.nitems = c->sym.nitems;
+    k.row_start = c->sym.row_start;
+    return k;
+}
+
+void drop_step_graph(gc_ctx* c) {
+    if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+    c->step_graph = nullptr;
+    c->step_key = StepKey{};
+}
+
+// Chunks are captured for the direct solver on a non-null stream that is
+// not itself being captured (a caller's own capture records the eager
+// launches instead).
+bool use_step_graph(gc_ctx* c) {
+    if (c->graph_steps <= 1 || c->params.solver != GC_SOLVER_DIRECT || !c->stream) return false;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs == hipStreamCaptureStatusNone;
+}
+
+// Launches `chunks` replays of one chunk of c->graph_steps steps, capturing
+// the chunk first (body() launches it: kernels only, no allocation, copy or
+// memset) unless the graph of an identical chunk is already there.
+template <class Body>
+gc_status replay_chunks(gc_ctx* c, double dt, int chunks, Body body) {
+    const StepKey key = make_step_key(c, dt);
+    if (!c->step_graph || !(c->step_key == key)) {
+        drop_step_graph(c);
+        hipGraph_t g = nullptr;
+        GC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+               "begin step capture");
+        gc_status st = body();
+        const hipError_t le = hipGetLastError();
+        const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+        if (st == GC_OK && le != hipSuccess)
+            st = fail(c, GC_ERR_HIP, std::string("capture step kernels: ") + hipGetErrorString(le));
+        if (st == GC_OK && ee != hipSuccess)
+            st = fail(c, GC_ERR_HIP, std::string("end step capture: ") + hipGetErrorString(ee));
+        if (st != GC_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        const hipError_t ie = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            c->step_graph = nullptr;
+            return fail(c, ie == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
+                        std::string("instantiate step graph: ") + hipGetErrorString(ie));
+        }
+        c->step_key = key;
+    }
+    for (int i = 0; i < chunks; ++i)
+        GC_HIP(c, hipGraphLaunch(c->step_graph, c->stream), "launch step graph");
+    return GC_OK;
+}
+
 gc_status need_particles(gc_ctx* c) {
     if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
     GC_HIP(c, hipSetDevice(c->device), "select device");
@@ -463,6 +614,7 @@ void gc_destroy(gc_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_step_graph(c);
     delete c->fmm;
     free_state(c);
     (void)hipFree(c->vjp_lpack);
@@ -616,34 +768,55 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
     if (st != GC_OK) return st;
     if (!(dt > 0.0)) return fail(c, GC_ERR_INVALID_ARGUMENT, "dt must be positive");
     if (nsteps < 0) return fail(c, GC_ERR_INVALID_ARGUMENT, "steps must be non-negative");
-    if (c->s.n == 0) return GC_OK;
+    if (c->s.n == 0 || nsteps == 0) return GC_OK;
     const double half = 0.5 * dt;
+    if (c->params.solver == GC_SOLVER_DIRECT && (st = prepare_direct(c)) != GC_OK) return st;
+    const int chunk = c->graph_steps;
+    const bool graph = use_step_graph(c);
     if (c->params.solver == GC_SOLVER_DIRECT && c->params.precision == GC_PRECISION_MIXED &&
         symmetric_pairs(c, 0, c->s.n) && nsteps > 1) {
-        // each step's merge also opens the next step and packs its sources
-        st = ensure_sym(c);
-        if (st != GC_OK) return st;
-        const double G = c->params.gravitational_constant;
-        const int ipt = sym_ipt(c);
+        // each step's merge also opens the next step and packs its sources;
+        // the middle steps (force + that merge) are what a chunk repeats
         launch_open_step(c->s, half, dt, c->stream);
-        launch_pack_mixed(c->s, c->mixed, G, c->stream);
-        for (int k = 0; k < nsteps; ++k) {
-            launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, ipt, c->stream);
-            if (k + 1 < nsteps)
-                launch_merge_sym_open(c->s, c->sym, c->mixed, G, half, dt, c->stream);
-            else
-                launch_merge_sym(c->s, c->sym, nullptr, true, half, c->stream);
+        launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
+        int middle = nsteps - 1;
+        if (graph && middle >= chunk) {
+            st = replay_chunks(c, dt, middle / chunk, [&] {
+                launch_sym_middle(c, dt, chunk);
+                return GC_OK;
+            });
+            if (st != GC_OK) return st;
+            middle %= chunk;
         }
+        launch_sym_middle(c, dt, middle);
+        launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, sym_ipt(c), c->stream);
+        launch_merge_sym(c->s, c->sym, nullptr, true, half, c->stream);
         return launch_status(c, "launch step kernels");
     }
-    for (int k = 0; k < nsteps; ++k) {
-        launch_open_step(c->s, half, dt, c->stream);
-        st = force(c, 0, c->s.n, true, half);
+    int left = nsteps;
+    if (graph && left >= chunk) {
+        st = replay_chunks(c, dt, left / chunk, [&] { return launch_plain_steps(c, dt, chunk); });
         if (st != GC_OK) return st;
+        left %= chunk;
     }
+    if ((st = launch_plain_steps(c, dt, left)) != GC_OK) return st;
     return launch_status(c, "launch step kernels");
     GC_ABI_END("gc_step")
 }
+
+gc_status gc_set_graph_steps(gc_ctx* c, int steps) {
+    GC_ABI_BEGIN
+    if (!c) return fail(nullptr, GC_ERR_INVALID_ARGUMENT, "null context");
+    if (steps < 0 || steps > 1000)
+        return fail(c, GC_ERR_INVALID_ARGUMENT, "graph steps must be in [0, 1000]");
+    GC_HIP(c, hipSetDevice(c->device), "select device");
+    if (steps != c->graph_steps) drop_step_graph(c);
+    c->graph_steps = steps;
+    return GC_OK;
+    GC_ABI_END("gc_set_graph_steps")
+}
+
+int gc_get_graph_steps(const gc_ctx* c) { return c ? c->graph_steps : -1; }
 
 gc_status gc_generate_galaxies_device(gc_ctx* c, const gc_galaxy_config* galaxies, int count,
                                       uint64_t seed) {

@@ -143,6 +143,8 @@ SIGNATURES = {
     "gc_set_pairs": (C.c_int, [C.c_void_p, C.c_int]),
     "gc_get_pairs": (C.c_int, [C.c_void_p]),
     "gc_set_pair_items": (C.c_int, [C.c_void_p, C.c_int]),
+    "gc_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int]),
+    "gc_get_graph_steps": (C.c_int, [C.c_void_p]),
     "gc_batch_set_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "gc_batch_get_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "gc_writer_open": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p]),

@@ -153,6 +153,13 @@ class Engine:
         """Source tiles per work item of the symmetric form (0 = automatic)."""
         self._check(self._lib.gc_set_pair_items(self._ctx, int(tiles)), "set pair items")
 
+    def set_graph_steps(self, steps=10):
+        """Steps per hipGraph-captured chunk of step() (0 or 1: eager)."""
+        self._check(self._lib.gc_set_graph_steps(self._ctx, int(steps)), "set graph steps")
+
+    def graph_steps(self) -> int:
+        return int(self._lib.gc_get_graph_steps(self._ctx))
+
     def pairs(self) -> str:
         return {L.GC_PAIRS_TARGETS: "targets",
                 L.GC_PAIRS_SYMMETRIC: "symmetric"}[self._lib.gc_get_pairs(self._ctx)]

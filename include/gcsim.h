@@ -150,8 +150,20 @@ gc_status gc_device_state(gc_ctx* ctx, gc_device_view* out);
 gc_status gc_compute_accelerations(gc_ctx* ctx);
 /* nsteps KDK steps (integrator.cpp:33-42): kick dt/2, drift dt, a(x),
  * kick dt/2.  Requires a(x) of the current positions (gc_compute_accelerations
- * or a previous step).  Asynchronous on the context's stream. */
+ * or a previous step).  Asynchronous on the context's stream.  With the
+ * direct solver, whole chunks of gc_get_graph_steps() steps are replayed
+ * from a hipGraph captured on the context's stream (kernels only; recaptured
+ * when dt, the parameters, the launch shape, the stream or the particle
+ * buffers change); the remainder, a NULL stream and a stream the caller is
+ * capturing get eager launches of the same kernels, so the results are the
+ * same either way. */
 gc_status gc_step(gc_ctx* ctx, double dt, int nsteps);
+/* Steps per captured gc_step chunk, 0..1000 (default 10; 0 or 1 launches
+ * every kernel eagerly).  No reference counterpart: cuda_direct_leapfrog_step
+ * (cpp/cuda/cuda_solver.cu:212-246) launches its kernels one by one.
+ * gc_get_graph_steps returns -1 for a null context. */
+gc_status gc_set_graph_steps(gc_ctx* ctx, int steps);
+int gc_get_graph_steps(const gc_ctx* ctx);
 gc_status gc_synchronize(gc_ctx* ctx);
 
 /* FP64 diagnostics of the resident state on the device (wavefront

@@ -391,6 +391,49 @@ def test_multi_step_fusion_is_bit_identical(gpu, n):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision,pairs", [("mixed", "symmetric"), ("mixed", "targets"),
+                                             ("fp64", None), ("fp64-fast", None)])
+def test_graph_chunks_are_bit_identical(gpu, precision, pairs):
+    # gc_step replays whole chunks of graph_steps steps from a captured
+    # hipGraph.  Every replay, with eager work (downloads, diagnostics, a
+    # one-step call) and synchronisations between replays, and a recapture
+    # after a change of dt, must repeat the eagerly launched trajectory bit
+    # for bit.
+    pos, vel, m = gpu.generators.config1(n=65536)
+    pos, vel, m = pos[:3000], vel[:3000], m[:3000]
+    plan = [(5e-4, 25), (5e-4, 10), (5e-4, 1), (5e-4, 11), (2.5e-4, 20), (5e-4, 30)]
+    runs = []
+    for chunk in (0, 10, 7):
+        with _engine(gpu, precision, 0.02) as eng:
+            if pairs:
+                eng.set_pairs(pairs)
+            eng.set_graph_steps(chunk)
+            assert eng.graph_steps() == chunk
+            eng.upload(pos, vel, m)
+            eng.compute_accelerations()
+            states = []
+            for dt, k in plan:
+                eng.step(dt, k)
+                states.append(eng.download())
+                eng.diagnostics()
+            runs.append(states)
+    for got in runs[1:]:
+        for a_state, b_state in zip(runs[0], got):
+            for a, b in zip(a_state, b_state):
+                assert np.array_equal(a, b)
+
+
+def test_graph_steps_limits(gpu):
+    with _engine(gpu, "mixed", 0.02) as eng:
+        assert eng.graph_steps() == 10
+        with pytest.raises(gpu.GcError):
+            eng.set_graph_steps(-1)
+        with pytest.raises(gpu.GcError):
+            eng.set_graph_steps(1001)
+        eng.set_graph_steps(1000)
+        assert eng.graph_steps() == 1000
+
+
 # ---- FP64 throughput form (precision "fp64-fast") ----------------------------
 FAST64_TOL = 1e-13  # max relative force difference from the oracle's targets form
 
