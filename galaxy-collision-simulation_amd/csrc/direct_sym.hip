@@ -19,11 +19,12 @@
 // serves two interactions without any cross-lane reduction; after 64 steps
 // lane l holds source l's sum.  The four waves' sums are added in wave
 // order through LDS, and each (I, J) pair leaves one FP32 partial per
-// target of tile J.  The diagonal tile J == I runs the
-// one-sided form (tile_pairs of mixed_tiles.hpp).  The i-side sum over each
-// J tile is an FP32 row partial of the pair (I, J).  The merge adds, per
-// target, its row's partials in J order and its column's pair partials in I
-// order, in FP64: the result does not depend on scheduling.
+// target of tile J.  The diagonal tile J == I is an item of its own, each
+// of its unordered pairs once at 4 targets per lane (the one-sided form,
+// tile_pairs of mixed_tiles.hpp, at 1 or 2).  The i-side sums over all of an
+// item's J tiles are the item's FP32 row partial.  The merge adds, per
+// target, its row's item partials in J order and its column's pair partials
+// in I order, in FP64: the result does not depend on scheduling.
 #include "gc_internal.hpp"
 #include "mixed_tiles.hpp"
 
@@ -45,10 +46,6 @@ __host__ __device__ inline size_t pair_slot(int I, int J) {  // I < J
     return static_cast<size_t>(J) * static_cast<size_t>(J - 1) / 2 + static_cast<size_t>(I);
 }
 
-// Row-partial slot of tile pair (I, J), I <= J, within a run.
-__host__ __device__ inline size_t row_slot(int I, int J) {
-    return static_cast<size_t>(J) * static_cast<size_t>(J + 1) / 2 + static_cast<size_t>(I);
-}
 
 #ifndef GC_SYM_LDS_PAD
 #define GC_SYM_LDS_PAD 1024
@@ -161,14 +158,13 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
             zi[k] = active ? z[i] : 0.0;
             mi[k] = jp[i].w;  // G m_i; zero past n
         }
-        // Each (I, J) row partial is stored after the next tile's sources
-        // are staged: vmcnt counts stores too, so stores issued before the
-        // staging loads would be waited for with them.
+        // the item's row partial: its targets' FP32 sums over all of its
+        // source tiles, at the item's slot
         float qx[IPT], qy[IPT], qz[IPT];
-        int pending = -1;
-        auto store_row = [&](int Jp) {
-            float* base = ipart + (item.pbase + item.t0 + row_slot(I - item.t0, Jp - item.t0)) *
-                                      3 * kTile;
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) qx[k] = qy[k] = qz[k] = 0.f;
+        auto store_row = [&]() {
+            float* base = ipart + static_cast<size_t>(item.islot) * 3 * kTile;
 #pragma unroll
             for (int k = 0; k < IPT; ++k) {
                 const int e = k * kThreads + t;
@@ -181,8 +177,6 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
             // the diagonal tile, always an item of its own
             float tx[IPT], ty[IPT], tz[IPT];
             tile_offsets<IPT>(centre, ntiles, I, xi, yi, zi, tx, ty, tz);
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) qx[k] = qy[k] = qz[k] = 0.f;
             const float4* src = jp + static_cast<size_t>(I) * kTile;
             if constexpr (IPT == 4) {
                 // each unordered pair once
@@ -200,7 +194,7 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
                 // one-sided: every target over the tile's sources
                 tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, qx, qy, qz);
             }
-            store_row(I);
+            store_row();
             __syncthreads();  // LDS reuse by the next item
             continue;
         }
@@ -209,10 +203,11 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
         constexpr int kPer = kTile / kThreads;
         float4 pre[kPer];
         for (int J = item.jb; J < item.je; ++J) {
-            float tx[IPT], ty[IPT], tz[IPT], px[IPT], py[IPT], pz[IPT];
+            float tx[IPT], ty[IPT], tz[IPT];
+            float* px = qx;
+            float* py = qy;
+            float* pz = qz;
             tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) px[k] = py[k] = pz[k] = 0.f;
             const float4* src = jp + static_cast<size_t>(J) * kTile;
             {
 #pragma unroll
@@ -228,10 +223,7 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
 #pragma unroll
                     for (int u = 0; u < kPer; ++u) pre[u] = nsrc[t + u * kThreads];
                 }
-                if (pending >= 0) {
-                    store_row(pending);
-                    pending = -1;
-                }
+#pragma unroll
                 for (int c = 0; c < kChunks; ++c) {
                     // lane l holds the running sum of source c*64 + (l + s) % 64
                     // at step s; wave_rol1 after each step keeps that true.
@@ -291,33 +283,25 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
                     }
                 }
             }
-#pragma unroll
-            for (int k = 0; k < IPT; ++k) {
-                qx[k] = px[k];
-                qy[k] = py[k];
-                qz[k] = pz[k];
-            }
-            pending = J;
         }
-        if (pending >= 0) store_row(pending);
+        store_row();
         __syncthreads();  // LDS reuse by the next item
     }
 }
 
-// a_i = sum over J >= T of the (T, J) row partials (J order) + sum over
+// a_i = sum over row T's items of their row partials (J order) + sum over
 // I < T of the (I, T) column partials (I order), in FP64; optional closing
 // half kick.  A block of 16 groups of 64 lanes merges tile T, 4 targets per
 // lane (float4 loads); group g adds the g-th of 16 consecutive ranges of
-// that sequence of nt partials (nt: the tile's run length), and the 16 group
-// sums are added in group order, so the result is deterministic.  Every
-// tile reads nt partials of 3 KB.  Blocks run the longest columns (largest
-// T) first.
+// that sequence of partials, and the 16 group sums are added in group
+// order, so the result is deterministic.  Every partial is 3 KB.  Blocks
+// run the longest columns (largest T) first.
 constexpr int kMergeGroups = 16;
 constexpr int kMergeLanes = kTile / 4;  // 64
 
 // Row and column partials of target tile T summed per group into part.
 __device__ __forceinline__ void merge_sym_groups(
-    int T, const int* __restrict__ tile_t0, const int* __restrict__ tile_nt,
+    int T, const int* __restrict__ tile_t0, const int* __restrict__ row_start,
     const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
     const float* __restrict__ jpart, double (*part)[3][kTile]) {
 #pragma clang fp contract(off)
@@ -326,14 +310,16 @@ __device__ __forceinline__ void merge_sym_groups(
     double a[3][4];
 #pragma unroll
     for (int c = 0; c < 3; ++c) a[c][0] = a[c][1] = a[c][2] = a[c][3] = 0.0;
-    const int t0 = tile_t0[T], nt = tile_nt[T], r = T - t0;
+    const int r = T - tile_t0[T];
     const long long pbase = tile_pbase[T];
-    const int nrow = nt - r;  // row partials (T, J), J = T .. t0 + nt - 1
-    const int q0 = g * nt / kMergeGroups, q1 = (g + 1) * nt / kMergeGroups;
+    const int s0 = row_start[T];
+    const int nrow = row_start[T + 1] - s0;  // row T's items, in J order
+    const int total = nrow + r;
+    const int q0 = g * total / kMergeGroups, q1 = (g + 1) * total / kMergeGroups;
     for (int q = q0; q < q1; ++q) {
         const float* b;
         if (q < nrow) {
-            b = ipart + (pbase + t0 + row_slot(r, r + q)) * 3 * kTile;
+            b = ipart + static_cast<size_t>(s0 + q) * 3 * kTile;
         } else {
             b = jpart + (pbase + pair_slot(q - nrow, r)) * 3 * kTile;
         }
@@ -354,7 +340,7 @@ __device__ __forceinline__ void merge_sym_groups(
 }
 
 __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
-    int n, int ntiles, const int* __restrict__ tile_t0, const int* __restrict__ tile_nt,
+    int n, int ntiles, const int* __restrict__ tile_t0, const int* __restrict__ row_start,
     const long long* __restrict__ tile_pbase, const int* __restrict__ tile_valid,
     const float* __restrict__ ipart, const float* __restrict__ jpart, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
@@ -362,7 +348,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
 #pragma clang fp contract(off)
     __shared__ double part[kMergeGroups][3][kTile];
     const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
-    merge_sym_groups(T, tile_t0, tile_nt, tile_pbase, ipart, jpart, part);
+    merge_sym_groups(T, tile_t0, row_start, tile_pbase, ipart, jpart, part);
     __syncthreads();
     // thread (c, e) of the first 3 * kTile adds target e's component c
     const int t = threadIdx.x;
@@ -388,7 +374,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_kernel(
 // unchanged; the state and the tiles are touched once instead of three
 // times.
 __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_kernel(
-    int n, int ntiles, const int* __restrict__ tile_t0, const int* __restrict__ tile_nt,
+    int n, int ntiles, const int* __restrict__ tile_t0, const int* __restrict__ row_start,
     const long long* __restrict__ tile_pbase, const float* __restrict__ ipart,
     const float* __restrict__ jpart, double* __restrict__ x, double* __restrict__ y,
     double* __restrict__ z, double* __restrict__ vx, double* __restrict__ vy,
@@ -400,7 +386,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_ker
     __shared__ double box[2][3][4];   // per component: wave minima, maxima
     __shared__ float offs[3][kTile];
     const int T = ntiles - 1 - static_cast<int>(blockIdx.x);
-    merge_sym_groups(T, tile_t0, tile_nt, tile_pbase, ipart, jpart, part);
+    merge_sym_groups(T, tile_t0, row_start, tile_pbase, ipart, jpart, part);
     __syncthreads();
     const int t = threadIdx.x;
     const bool active = t < 3 * kTile;  // thread (c, e): component c of target e
@@ -707,12 +693,11 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
     SymHostPlan h;
     if (!build_sym_plan(run_t0, run_nt, nruns, len, kSymMaxTiles, h, err)) return 1;
     plan.nitems = static_cast<int>(h.items.size());
-    plan.ipart_floats = (static_cast<size_t>(std::max<long long>(0, h.pairs)) + h.ntiles) * 3 * kTile;
+    plan.ipart_floats = std::max<size_t>(1, h.items.size()) * 3 * kTile;
     plan.jpart_floats = std::max<size_t>(1, static_cast<size_t>(h.pairs)) * 3 * kTile;
     hipError_t e = hipMalloc(&plan.items, sizeof(SymItem) * h.items.size());
     if (e == hipSuccess) e = hipMalloc(&plan.row_start, sizeof(int) * h.row_start.size());
     if (e == hipSuccess) e = hipMalloc(&plan.tile_t0, sizeof(int) * h.tile_t0.size());
-    if (e == hipSuccess) e = hipMalloc(&plan.tile_nt, sizeof(int) * h.tile_nt.size());
     if (e == hipSuccess) e = hipMalloc(&plan.tile_pbase, sizeof(long long) * h.tile_pbase.size());
     if (e == hipSuccess) e = hipMalloc(&plan.ipart, sizeof(float) * plan.ipart_floats);
     if (e == hipSuccess) e = hipMalloc(&plan.jpart, sizeof(float) * plan.jpart_floats);
@@ -724,9 +709,6 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(plan.tile_t0, h.tile_t0.data(), sizeof(int) * h.tile_t0.size(),
-                           hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(plan.tile_nt, h.tile_nt.data(), sizeof(int) * h.tile_nt.size(),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
         e = hipMemcpyAsync(plan.tile_pbase, h.tile_pbase.data(),
@@ -757,7 +739,6 @@ void sym_release(SymPlan& plan) {
     (void)hipFree(plan.items);
     (void)hipFree(plan.row_start);
     (void)hipFree(plan.tile_t0);
-    (void)hipFree(plan.tile_nt);
     (void)hipFree(plan.tile_pbase);
     (void)hipFree(plan.ipart);
     (void)hipFree(plan.jpart);
@@ -803,7 +784,7 @@ void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
                       bool close_kick, double half_dt, hipStream_t st) {
     hipLaunchKernelGGL(merge_sym_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups), 0, st,
-                       static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.tile_nt,
+                       static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.row_start,
                        plan.tile_pbase, tile_valid, plan.ipart, plan.jpart, s.ax, s.ay, s.az, s.vx,
                        s.vy, s.vz, close_kick ? 1 : 0, half_dt);
 }
@@ -839,7 +820,7 @@ void launch_vjp_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& p
 void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,
                            double half_dt, double dt, hipStream_t st) {
     hipLaunchKernelGGL(merge_sym_open_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups),
-                       0, st, static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.tile_nt,
+                       0, st, static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.row_start,
                        plan.tile_pbase, plan.ipart, plan.jpart, s.x, s.y, s.z, s.vx, s.vy, s.vz,
                        s.ax, s.ay, s.az, s.m, half_dt, dt, G, L.jp, L.centre);
 }

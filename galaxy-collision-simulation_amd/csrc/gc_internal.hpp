@@ -56,9 +56,8 @@ struct SymPlan {
     SymItem* items = nullptr;   // longest first
     int* row_start = nullptr;   // ntiles + 1 slot offsets; a row's slots in J order
     int* tile_t0 = nullptr;     // first tile of each tile's run
-    int* tile_nt = nullptr;     // tile count of each tile's run
     long long* tile_pbase = nullptr;  // pair-partial base of each tile's run
-    float* ipart = nullptr;     // (pairs + ntiles) * 3 * kTile row partials (x | y | z)
+    float* ipart = nullptr;     // nitems * 3 * kTile row partials (x | y | z), item slots
     float* jpart = nullptr;     // per run nt (nt - 1) / 2 pairs * 3 * kTile
     size_t ipart_floats = 0, jpart_floats = 0;
 };

@@ -11,12 +11,11 @@
 // [row_start[T], row_start[T + 1]) in J order.  The launch order is longest
 // item first (ties in slot order, diagonal items after single tiles), one
 // block per item, so the hardware dispatcher balances the tail with the
-// shortest items.  The force kernel
-// writes one row partial per tile pair (I, J), J >= I: slot
-// pbase + t0 + (J - t0)(J - t0 + 1) / 2 + (I - t0) of (pairs + ntiles) slots
-// (runs' slot ranges do not overlap since each run's pbase + t0 is at least
-// the previous run's end); the item slots serve the adjoint's (hi, lo) row
-// partials.
+// shortest items.  The force kernel writes one FP32 row partial per item
+// (the sums of the item's targets over all of its source tiles) at islot,
+// and one column partial per tile pair (I, J), I < J, of a run at
+// pbase + J (J - 1) / 2 + I (tiles counted from the run's first); the
+// adjoint keeps its (hi, lo) row partials at the item slots too.
 #pragma once
 
 #include <algorithm>
