@@ -1,6 +1,11 @@
 """The benchmark workloads at their configured shapes (configs/*.toml), on
 the GPU, against the FP64 oracle path.
 
+C0: two-disk N = 1000, eps 0.05, dt 1e-3, 300 KDK steps: the FP64 device
+    path repeats the oracle's FP64 leapfrog (targets form) bit for bit at
+    every 50th step and at the end, its diagnostics within 1e-12 of the
+    oracle's; the mixed run's forces stay within 1e-5 of FP64 forces of its
+    own state and its energy drift within 2x of the FP64 run's.
 C1: two-disk N = 65,536, eps 0.02, dt 5e-4, 1000 KDK steps, FP64 and mixed.
     The mixed run's relative energy drift stays within 2x of the FP64 run's
     (the FP64 device path is bit-identical to the oracle's targets form),
@@ -30,6 +35,46 @@ from helpers import rel_err
 pytestmark = pytest.mark.gpu
 
 MIXED_TOL = 1e-5  # max relative force error, mixed vs FP64
+
+
+def test_c0_configured_shape(gpu):
+    n, eps, dt, steps, every = 1000, 0.05, 1e-3, 300, 50
+    pos, vel, mass = gpu.generators.config0()
+    assert len(mass) == n
+    solver = O.make_solver(O.SOLVER_DIRECT_TARGETS, 1.0, eps)
+    rp, rv = pos.copy(), vel.copy()
+    ra = O.compute_accelerations(solver, rp, mass)
+    engines = {p: gpu.Engine("hip-direct", p, softening=eps) for p in ("fp64", "mixed")}
+    probe = gpu.Engine("hip-direct", "fp64", softening=eps)
+    energy = {p: [] for p in engines}
+    try:
+        for e in engines.values():
+            e.upload(pos, vel, mass)
+            e.compute_accelerations()
+        for k in range(0, steps + 1, every):
+            if k:
+                for e in engines.values():
+                    e.step(dt, every)
+                rp, rv, ra = O.leapfrog(solver, rp, rv, ra, mass, dt, every)
+            gp, gv, ga = engines["fp64"].download()
+            assert np.array_equal(gp, rp) and np.array_equal(gv, rv) and np.array_equal(ga, ra), k
+            d = engines["fp64"].diagnostics()
+            want = O.diagnostics(rp, rv, mass, 1.0, eps)
+            assert abs(d["total_energy"] - want["total_energy"]) <= 1e-12 * abs(want["total_energy"])
+            for p, e in engines.items():
+                energy[p].append(e.diagnostics()["total_energy"])
+            pm, vm, am = engines["mixed"].download()
+            probe.upload(pm, vm, mass)
+            probe.compute_accelerations()
+            err = rel_err(am, probe.download()[2]).max()
+            assert err <= MIXED_TOL, (k, err)
+    finally:
+        for e in list(engines.values()) + [probe]:
+            e.close()
+    drift = {p: (np.abs(np.array(s) - s[0]) / abs(s[0])).max() for p, s in energy.items()}
+    print(f"C0 drift fp64 {drift['fp64']:.3e} mixed {drift['mixed']:.3e}", flush=True)
+    assert drift["fp64"] > 0
+    assert drift["mixed"] <= 2.0 * drift["fp64"]
 
 
 def test_c1_energy_drift_and_force_error_over_1000_steps(gpu):
