@@ -11,6 +11,12 @@ C1: two-disk N = 65,536, eps 0.02, dt 5e-4, 1000 KDK steps, FP64 and mixed.
     (the FP64 device path is bit-identical to the oracle's targets form),
     and every 50 steps the mixed forces of the mixed trajectory's state are
     within 1e-5 (max relative) of FP64 forces of the same state.
+C2: Plummer N = 2^20, leaves <= 64, p = 4, theta 0.5, eps 0.01, dt 1e-3,
+    10 KDK steps of hip-fmm in FP32 (FP64 state) beside the same 10 steps
+    in FP64: the trajectories agree to 1e-9 in position; the final FP32
+    forces are within 1e-5 of FP64 FMM forces of the same state and of the
+    oracle FMM on 4,096 targets, and within the FMM's truncation of FP64
+    direct sums there.
 C3: two-disk N = 16,384, eps 0.05, dt 1e-3, 200 KDK steps, checkpoints
     every 20, loss = mean |x(T) - x_target|^2 with targets from a run whose
     initial velocities carry N(0, 0.005): gradients in (v0, m) by the FP64
@@ -112,6 +118,35 @@ def test_c1_energy_drift_and_force_error_over_1000_steps(gpu):
           f"max force err {worst:.2e}", flush=True)
     assert drift["fp64"] > 0
     assert drift["mixed"] <= 2.0 * drift["fp64"]
+
+
+def test_c2_configured_shape(gpu):
+    n, eps, dt, steps = 1 << 20, 0.01, 1e-3, 10
+    pos, vel, m = gpu.generators.plummer(n, seed=1)
+    opts = dict(softening=eps, theta=0.5, leaf_capacity=64, expansion_order=4)
+    runs = {}
+    for prec in ("mixed", "fp64"):
+        with gpu.Engine("hip-fmm", prec, **opts) as eng:
+            eng.upload(pos, vel, m)
+            eng.compute_accelerations()
+            eng.step(dt, steps)
+            runs[prec] = eng.download()
+    (pm, vm, am), (pf, vf, af) = runs["mixed"], runs["fp64"]
+    assert np.abs(pm - pf).max() < 1e-9
+    assert np.abs(vm - vf).max() < 1e-7
+    with gpu.Engine("hip-fmm", "fp64", **opts) as eng:
+        eng.upload(pm, vm, m)
+        eng.compute_accelerations()
+        a64 = eng.download()[2]
+    err = rel_err(am, a64)
+    print(f"C2 FP32 vs FP64 forces of the final state: max {err.max():.2e}, "
+          f"median {np.median(err):.2e}", flush=True)
+    assert err.max() <= MIXED_TOL
+    want, _ = O.fmm(pm, m, 1.0, eps, 0.5, 64, 32, 4, begin=0, end=4096)
+    assert rel_err(am[:4096], want[:4096]).max() <= MIXED_TOL
+    direct = O.direct_targets(pm, m, 1.0, eps, 0, 4096, threads=0)
+    err = rel_err(am[:4096], direct[:4096])
+    assert np.median(err) < 1e-4 and err.max() < 1e-2
 
 
 def _c3_inputs(gpu, n):
