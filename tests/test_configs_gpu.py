@@ -13,7 +13,8 @@ C1: two-disk N = 65,536, eps 0.02, dt 5e-4, 1000 KDK steps, FP64 and mixed.
     within 1e-5 (max relative) of FP64 forces of the same state.
 C2: Plummer N = 2^20, leaves <= 64, p = 4, theta 0.5, eps 0.01, dt 1e-3,
     10 KDK steps of hip-fmm in FP32 (FP64 state) beside the same 10 steps
-    in FP64: the trajectories agree to 1e-9 in position; the final FP32
+    in FP64: the trajectories agree to 1e-9 in position and the FP32 run's
+    relative energy drift is within 2x of the FP64 run's; the final FP32
     forces are within 1e-5 of FP64 FMM forces of the same state and of the
     oracle FMM on 4,096 targets, and within the FMM's truncation of FP64
     direct sums there.
@@ -124,13 +125,18 @@ def test_c2_configured_shape(gpu):
     n, eps, dt, steps = 1 << 20, 0.01, 1e-3, 10
     pos, vel, m = gpu.generators.plummer(n, seed=1)
     opts = dict(softening=eps, theta=0.5, leaf_capacity=64, expansion_order=4)
-    runs = {}
+    runs, drift = {}, {}
     for prec in ("mixed", "fp64"):
         with gpu.Engine("hip-fmm", prec, **opts) as eng:
             eng.upload(pos, vel, m)
             eng.compute_accelerations()
+            e0 = eng.diagnostics()["total_energy"]
             eng.step(dt, steps)
+            drift[prec] = abs(eng.diagnostics()["total_energy"] - e0) / abs(e0)
             runs[prec] = eng.download()
+    print(f"C2 energy drift over {steps} steps: FP32 {drift['mixed']:.3e}, "
+          f"FP64 {drift['fp64']:.3e}", flush=True)
+    assert drift["mixed"] <= 2.0 * drift["fp64"]
     (pm, vm, am), (pf, vf, af) = runs["mixed"], runs["fp64"]
     assert np.abs(pm - pf).max() < 1e-9
     assert np.abs(vm - vf).max() < 1e-7
