@@ -1999,8 +1999,18 @@ struct FmmEngine::Impl {
     int nnodes = 0, nleaves = 0;
     gc_fmm_stats last{};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // the near field runs on a second stream (default priority) beside the
+    // far field: fork after the build, join before the merge
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 
     ~Impl() {
+        if (side) {
+            (void)hipStreamSynchronize(side);
+            (void)hipStreamDestroy(side);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
         void* ptrs[] = {keys, keys_sorted, keys_lo, keys_lo_sorted, idx, idx_sorted, sxs, sys, szs, sms, sort_tmp, bbox_part, geom, mcom, mom,
                         kc64, kc32, nb, ne, first_child, nchild, bounds, count, offset,
                         scan_tmp, overflow, lvl, deepest, geom_root, flag, flag_pos, leaves, nfar, nnear, nsrc, far_off,
@@ -2541,7 +2551,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
                        idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, tp, toff,     \
                        tcnt, s.ax, s.ay, s.az)
 #define GC_NEAR(F, S)                                                                            \
-    hipLaunchKernelGGL((eval_near_kernel<F, S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
+    hipLaunchKernelGGL((eval_near_kernel<F, S>), dim3(static_cast<unsigned>(nitems)), block, 0, nst, \
                        items, leaves, geom, nb, ne, sxs, sys, szs, sms, near_off, near, G, eps,   \
                        tp, toff, tcnt, npart)
 #define GC_FAR32(O)                                                                              \
@@ -2549,13 +2559,21 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
                        ne, idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, tp, toff, \
                        tcnt, s.ax, s.ay, s.az)
 #define GC_NEAR32(S)                                                                             \
-    hipLaunchKernelGGL((eval_near_cells_kernel<S>), dim3(static_cast<unsigned>(nitems)), block, 0, st, \
+    hipLaunchKernelGGL((eval_near_cells_kernel<S>), dim3(static_cast<unsigned>(nitems)), block, 0, nst, \
                        items, leaves, geom, nb, ne, sxs, sys, szs, srec, near_off, near, eps, tp,  \
                        toff, tcnt, npart)
+    // far field on st, near field on the side stream: independent until the
+    // merge (the far kernels write the accelerations, the near kernels their
+    // own partial rows)
+    if (mixed && static_cast<size_t>(s.n) > srec_cap)
+        FMM_HIP(realloc_dev(&srec, &srec_cap, static_cast<size_t>(s.n)), "allocate source records");
+    if (!side) FMM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "create FMM side stream");
+    if (!fork) FMM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "create FMM events");
+    if (!join) FMM_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming), "create FMM events");
+    FMM_HIP(hipEventRecord(fork, st), "fork near field");
+    FMM_HIP(hipStreamWaitEvent(side, fork, 0), "fork near field");
+    const hipStream_t nst = side;
     if (mixed) {
-        if (order == 0) GC_FAR32(0);
-        else if (order == 2) GC_FAR32(2);
-        else GC_FAR32(4);
         // the sources-across-lanes kernel adds the self pair as an exact
         // zero, which needs its factor G m eps^-3 finite in FP32
         const double e2 = static_cast<double>(static_cast<float>(eps * eps));
@@ -2563,22 +2581,26 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
         const bool self_zero = safe || (rs * rs <= 1e30 &&
                                         std::fabs(G) * mass_abs_max * std::max(rs, rs * rs * rs) <= 1e30);
         if (self_zero) {
-            if (static_cast<size_t>(s.n) > srec_cap)
-                FMM_HIP(realloc_dev(&srec, &srec_cap, static_cast<size_t>(s.n)), "allocate source records");
-            hipLaunchKernelGGL(near_records_kernel, grid, block, 0, st, leaves, geom, nb, ne, sxs, sys, szs,
-                               sms, G, srec);
+            hipLaunchKernelGGL(near_records_kernel, grid, block, 0, nst, leaves, geom, nb, ne, sxs, sys,
+                               szs, sms, G, srec);
             if (safe) GC_NEAR32(true);
             else GC_NEAR32(false);
         } else {
             GC_NEAR(float, false);
         }
+        if (order == 0) GC_FAR32(0);
+        else if (order == 2) GC_FAR32(2);
+        else GC_FAR32(4);
     } else {
+        if (safe) GC_NEAR(double, true);
+        else GC_NEAR(double, false);
         if (order == 0) GC_FAR(double, 0, kc64);
         else if (order == 2) GC_FAR(double, 2, kc64);
         else GC_FAR(double, 4, kc64);
-        if (safe) GC_NEAR(double, true);
-        else GC_NEAR(double, false);
     }
+    FMM_HIP(hipGetLastError(), "launch FMM far and near fields");
+    FMM_HIP(hipEventRecord(join, nst), "join near field");
+    FMM_HIP(hipStreamWaitEvent(st, join, 0), "join near field");
 #undef GC_FAR
 #undef GC_FAR32
 #undef GC_NEAR32
