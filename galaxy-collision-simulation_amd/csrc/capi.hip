@@ -21,7 +21,6 @@ namespace {
 // Everything a captured chunk of gc_step bakes into its kernel arguments:
 // the graph is replayed only while all of it is unchanged.
 struct StepKey {
-    hipStream_t stream = nullptr;
     double dt = 0.0, G = 0.0, eps = 0.0;
     int solver = -1, precision = -1;
     int ipt = 0, jsplit = 0, sources = 0, pairs = 0, items = 0, steps = 0;
@@ -31,7 +30,7 @@ struct StepKey {
     const void* row_start = nullptr;
 
     bool operator==(const StepKey& o) const {
-        if (stream != o.stream || dt != o.dt || G != o.G || eps != o.eps || solver != o.solver ||
+        if (dt != o.dt || G != o.G || eps != o.eps || solver != o.solver ||
             precision != o.precision || ipt != o.ipt || jsplit != o.jsplit ||
             sources != o.sources || pairs != o.pairs || items != o.items || steps != o.steps ||
             n != o.n || cap != o.cap || ntiles != o.ntiles)
@@ -68,9 +67,12 @@ struct gc_ctx {
     double* part64 = nullptr;    // ... and its per-split partial sums
     size_t jp64_n = 0, part64_n = 0;
     // gc_step: chunks of graph_steps steps replayed from a captured hipGraph
-    // (graph_steps <= 1: every kernel launched eagerly)
+    // (graph_steps <= 1: every kernel launched eagerly).  Chunks are captured
+    // on a stream of their own, which holds no other work, so a capture
+    // never meets work or events that other threads wait on in `stream`.
     int graph_steps = 10;
     hipGraphExec_t step_graph = nullptr;
+    hipStream_t capture_stream = nullptr;
     StepKey step_key{};
     std::string err;
 };
@@ -374,7 +376,6 @@ void launch_sym_middle(gc_ctx* c, double dt, int k) {
 
 StepKey make_step_key(const gc_ctx* c, double dt) {
     StepKey k;
-    k.stream = c->stream;
     k.dt = dt;
     k.G = c->params.gravitational_constant;
     k.eps = c->params.softening;
@@ -404,11 +405,11 @@ void drop_step_graph(gc_ctx* c) {
     c->step_key = StepKey{};
 }
 
-// Chunks are captured for the direct solver on a non-null stream that is
-// not itself being captured (a caller's own capture records the eager
-// launches instead).
+// Chunks are replayed for the direct solver on a stream that is not itself
+// being captured (a caller's own capture records the eager launches
+// instead).
 bool use_step_graph(gc_ctx* c) {
-    if (c->graph_steps <= 1 || c->params.solver != GC_SOLVER_DIRECT || !c->stream) return false;
+    if (c->graph_steps <= 1 || c->params.solver != GC_SOLVER_DIRECT) return false;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(c->stream, &cs) != hipSuccess) {
         (void)hipGetLastError();
@@ -417,20 +418,28 @@ bool use_step_graph(gc_ctx* c) {
     return cs == hipStreamCaptureStatusNone;
 }
 
-// Launches `chunks` replays of one chunk of c->graph_steps steps, capturing
-// the chunk first (body() launches it: kernels only, no allocation, copy or
-// memset) unless the graph of an identical chunk is already there.
+// Launches `chunks` replays of one chunk of c->graph_steps steps on the
+// context's stream, capturing the chunk first (body() launches it on
+// c->stream, which points at the capture stream meanwhile: kernels only, no
+// allocation, copy or memset) unless the graph of an identical chunk is
+// already there.
 template <class Body>
 gc_status replay_chunks(gc_ctx* c, double dt, int chunks, Body body) {
     const StepKey key = make_step_key(c, dt);
     if (!c->step_graph || !(c->step_key == key)) {
         drop_step_graph(c);
+        if (!c->capture_stream)
+            GC_HIP(c, hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking),
+                   "create step capture stream");
         hipGraph_t g = nullptr;
-        GC_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+        GC_HIP(c, hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal),
                "begin step capture");
+        hipStream_t const user = c->stream;
+        c->stream = c->capture_stream;
         gc_status st = body();
+        c->stream = user;
         const hipError_t le = hipGetLastError();
-        const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+        const hipError_t ee = hipStreamEndCapture(c->capture_stream, &g);
         if (st == GC_OK && le != hipSuccess)
             st = fail(c, GC_ERR_HIP, std::string("capture step kernels: ") + hipGetErrorString(le));
         if (st == GC_OK && ee != hipSuccess)
@@ -615,6 +624,7 @@ void gc_destroy(gc_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_step_graph(c);
+    if (c->capture_stream) (void)hipStreamDestroy(c->capture_stream);
     delete c->fmm;
     free_state(c);
     (void)hipFree(c->vjp_lpack);

@@ -395,16 +395,18 @@ def test_multi_step_fusion_is_bit_identical(gpu, n):
                                              ("fp64", None), ("fp64-fast", None)])
 def test_graph_chunks_are_bit_identical(gpu, precision, pairs):
     # gc_step replays whole chunks of graph_steps steps from a captured
-    # hipGraph.  Every replay, with eager work (downloads, diagnostics, a
-    # one-step call) and synchronisations between replays, and a recapture
-    # after a change of dt, must repeat the eagerly launched trajectory bit
-    # for bit.
+    # hipGraph (on the context's own stream and on the null stream).  Every
+    # replay, with eager work (downloads, diagnostics, a one-step call) and
+    # synchronisations between replays, and a recapture after a change of
+    # dt, must repeat the eagerly launched trajectory bit for bit.
     pos, vel, m = gpu.generators.config1(n=65536)
     pos, vel, m = pos[:3000], vel[:3000], m[:3000]
     plan = [(5e-4, 25), (5e-4, 10), (5e-4, 1), (5e-4, 11), (2.5e-4, 20), (5e-4, 30)]
     runs = []
-    for chunk in (0, 10, 7):
+    for chunk, null_stream in ((0, False), (10, False), (7, False), (10, True)):
         with _engine(gpu, precision, 0.02) as eng:
+            if null_stream:
+                eng.set_stream(0)
             if pairs:
                 eng.set_pairs(pairs)
             eng.set_graph_steps(chunk)
