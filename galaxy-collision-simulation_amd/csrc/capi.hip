@@ -68,8 +68,10 @@ struct gc_ctx {
     size_t jp64_n = 0, part64_n = 0;
     // gc_step: chunks of graph_steps steps replayed from a captured hipGraph
     // (graph_steps <= 1: every kernel launched eagerly).  Chunks are captured
-    // on a stream of their own, which holds no other work, so a capture
-    // never meets work or events that other threads wait on in `stream`.
+    // on a stream of their own (created with the context, which keeps its
+    // creation out of the first stepping call), which holds no other work,
+    // so a capture never meets work or events that other threads wait on in
+    // `stream`.
     int graph_steps = 10;
     hipGraphExec_t step_graph = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -605,6 +607,7 @@ gc_status gc_create(gc_ctx** out, int device, const gc_params* p) {
     c->params = *p;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     if (e == hipSuccess) e = hipMalloc(&c->diag_out, sizeof(double) * 16);
