@@ -533,8 +533,8 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not launched.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for {args.gpus}): the engine runs "
+              "on a single device and launches no multi-GPU sharding.", file=sys.stderr)
         return 2
 
     import gcsim
