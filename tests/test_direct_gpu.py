@@ -425,6 +425,28 @@ def test_graph_chunks_are_bit_identical(gpu, precision, pairs):
                 assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n,form", [(262144, "symmetric"), (262145, "targets")])
+def test_mixed_at_the_symmetric_size_limit(gpu, n, form):
+    # The symmetric form covers up to 1,024 tiles (262,144 particles); one
+    # particle more runs the per-target form.  Both against the FP64
+    # throughput path (<= 1e-13 of the oracle) on two-disk states.
+    pos, vel, m = gpu.generators.two_disk((n + 1) // 2, softening=0.02, seed=5)
+    pos, vel, m = pos[:n], vel[:n], m[:n]
+    with _engine(gpu, "mixed", 0.02) as eng:
+        eng.upload(pos, vel, m)
+        assert eng.pairs() == form
+        eng.compute_accelerations()
+        got = eng.download()[2]
+    with gpu.Engine("hip-direct", "fp64-fast", softening=0.02) as e64:
+        e64.upload(pos, vel, m)
+        e64.compute_accelerations()
+        want = e64.download()[2]
+    err = rel_err(got, want)
+    print(f"n={n} {form}: max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= MIXED_TOL
+    assert np.median(err) < 1e-6
+
+
 def test_graph_steps_limits(gpu):
     with _engine(gpu, "mixed", 0.02) as eng:
         assert eng.graph_steps() == 10
