@@ -254,16 +254,19 @@ def workload_adjoint(args):
         eng = gcsim.Engine("hip-direct", prec, softening=0.05)
         with torch.no_grad():  # targets from the perturbed-velocity run
             target, _ = adjoint.trajectory(x0, vt, m, dt, steps, engine=eng)
-        va, ma = v0.clone().requires_grad_(), m.clone().requires_grad_()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        xT, _ = adjoint.trajectory(x0, va, ma, dt, steps, engine=eng, checkpoint_every=20)
-        loss = ((xT - target) ** 2).sum(1).mean()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        loss.backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
+        # one untimed forward + backward first (the engine's VJP buffers and
+        # step graphs are set up there), then the timed one
+        for _ in range(2):
+            va, ma = v0.clone().requires_grad_(), m.clone().requires_grad_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xT, _ = adjoint.trajectory(x0, va, ma, dt, steps, engine=eng, checkpoint_every=20)
+            loss = ((xT - target) ** 2).sum(1).mean()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
         out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1, "loss": loss.item(),
                      "grad_v_norm": va.grad.norm().item(), "grad_m_norm": ma.grad.norm().item()}
         out[prec + "_grad"] = (va.grad.detach().clone(), ma.grad.detach().clone())
