@@ -447,6 +447,50 @@ def test_mixed_at_the_symmetric_size_limit(gpu, n, form):
     assert np.median(err) < 1e-6
 
 
+def test_contexts_on_threads_step_concurrently(gpu):
+    # One context per host thread: several threads stepping their own
+    # contexts at once (each capturing and replaying its step graphs, one of
+    # them on the FMM solver with its second stream) give the results of the
+    # same runs stepped one after another.
+    import threading
+
+    pos, vel, m = gpu.generators.config1(n=65536)
+    pos, vel, m = pos[:3000], vel[:3000], m[:3000]
+    cases = [("hip-direct", "mixed"), ("hip-direct", "fp64-fast"), ("hip-direct", "mixed"),
+             ("hip-fmm", "mixed")]
+
+    def run(solver, precision, out, k):
+        with gpu.Engine(solver, precision, softening=0.02) as eng:
+            eng.upload(pos, vel, m)
+            eng.compute_accelerations()
+            for _ in range(3):
+                eng.step(5e-4, 25)
+                eng.diagnostics()
+            out[k] = eng.download()
+
+    alone = {}
+    for k, (solver, precision) in enumerate(cases):
+        run(solver, precision, alone, k)
+    together, errors = {}, []
+
+    def guarded(*a):
+        try:
+            run(*a)
+        except Exception as e:  # reported below, in the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=guarded, args=(sv, pr, together, k))
+               for k, (sv, pr) in enumerate(cases)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for k in range(len(cases)):
+        for a, b in zip(alone[k], together[k]):
+            assert np.array_equal(a, b), cases[k]
+
+
 def test_graph_steps_limits(gpu):
     with _engine(gpu, "mixed", 0.02) as eng:
         assert eng.graph_steps() == 10
