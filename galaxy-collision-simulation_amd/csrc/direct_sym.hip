@@ -186,10 +186,16 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
                     src2[e / kChunk][kChunk + e % kChunk] = v;
                 }
                 __syncthreads();
+                // a chunk of padding (no particle at or past its first slot)
+                // adds nothing: its sources have zero mass and its source
+                // sums belong to no particle
                 diag_chunk<0, SAFE>(&src2[0][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
-                diag_chunk<1, SAFE>(&src2[1][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
-                diag_chunk<2, SAFE>(&src2[2][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
-                diag_chunk<3, SAFE>(&src2[3][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+                if (valid > kChunk)
+                    diag_chunk<1, SAFE>(&src2[1][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+                if (valid > 2 * kChunk)
+                    diag_chunk<2, SAFE>(&src2[2][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
+                if (valid > 3 * kChunk)
+                    diag_chunk<3, SAFE>(&src2[3][lane], lane, eps2, tx, ty, tz, mi, qx, qy, qz);
             } else {
                 // one-sided: every target over the tile's sources
                 tile_pairs<IPT, SAFE>(src, eps2, tx, ty, tz, qx, qy, qz);
@@ -209,6 +215,10 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
             float* pz = qz;
             tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
             const float4* src = jp + static_cast<size_t>(J) * kTile;
+            // source chunks holding particles of tile J (the rest is padding:
+            // zero-mass sources whose sums belong to no particle, which the
+            // merge never reads)
+            const int jvalid = tile_valid ? tile_valid[J] : n - J * kTile;
             {
 #pragma unroll
                 for (int u = 0; u < kPer; ++u) {
@@ -225,6 +235,7 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
                 }
 #pragma unroll
                 for (int c = 0; c < kChunks; ++c) {
+                    if (c * kChunk >= jvalid) break;
                     // lane l holds the running sum of source c*64 + (l + s) % 64
                     // at step s; wave_rol1 after each step keeps that true.
                     float ax = 0.f, ay = 0.f, az = 0.f;
