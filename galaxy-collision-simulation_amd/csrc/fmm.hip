@@ -1273,14 +1273,16 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
 // slots, each with one FP32 sum per component in every lane's registers
 // (the spare slots hold a padding target whose sums are never written).
 // The item's sources, 64 near leaves at a time laid end to end, pass 64 per
-// step: lane k takes source k of the step, whose leaf it finds from a bitmap
-// of the leaves' first sources (one 64-bit word and its prefix count), and
-// adds its pair term to every slot, the target read from LDS (the same
+// step: lane k takes source k of the step, whose leaf it finds with two
+// mbcnt instructions from a bitmap of the leaves' first sources (NearPiece),
+// and adds its pair term to every slot, the target read from LDS (the same
 // address for all lanes).  A source is its FP32 record (offset from its own
 // leaf's centre, G m; near_records_kernel) plus the FP32 offset of its
 // leaf's centre from the target leaf's; the next step's record is loaded
-// while this step is evaluated.  After each piece of near leaves the slot
-// sums are added over the wave by the far field's fixed network
+// while this step is evaluated.  Sources past a piece's end belong to a
+// sentinel leaf whose records are the zero records after the last
+// particle's, placed at the padding point.  After the item's last piece the
+// slot sums are added over the wave by the far field's fixed network
 // (row_quad_sum, then the four row sums in row order) into FP64 sums held
 // by lane t for target t.
 // The self pair (fmm_solver.cpp:295-310) adds exactly zero here: its offset
@@ -1291,12 +1293,18 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
 constexpr int kNearTargets = 24;
 
 // Source records of the listed leaves' particles (sorted positions): offset
-// from the leaf's centre and G m, in FP32.  One block per leaf.
+// from the leaf's centre and G m, in FP32.  One block per leaf; block 0 also
+// writes the 2 kEvalLanes zero records at [n, n + 2 kEvalLanes) that stand
+// for the sources past the end of a piece.
 __global__ __launch_bounds__(kEvalLanes) void near_records_kernel(
     const int* __restrict__ leaves, const double* __restrict__ geom, const int* __restrict__ nb,
     const int* __restrict__ ne, const double* __restrict__ sx, const double* __restrict__ sy,
-    const double* __restrict__ sz, const double* __restrict__ sm, double G,
+    const double* __restrict__ sz, const double* __restrict__ sm, double G, int n,
     float4* __restrict__ rec) {
+    if (blockIdx.x == 0) {
+        rec[n + threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec[n + kEvalLanes + threadIdx.x] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     const int leaf = leaves[blockIdx.x];
     const double cx = geom[4 * leaf], cy = geom[4 * leaf + 1], cz = geom[4 * leaf + 2];
     for (int e = nb[leaf] + static_cast<int>(threadIdx.x); e < ne[leaf]; e += kEvalLanes)
@@ -1304,14 +1312,20 @@ __global__ __launch_bounds__(kEvalLanes) void near_records_kernel(
                              static_cast<float>(sz[e] - cz), static_cast<float>(G * sm[e]));
 }
 
-// A piece of near leaves in LDS: per leaf (first sorted index - first piece
-// index, centre offset from the target leaf's centre as FP32 bits), the
-// bitmap of first sources and each word's count of earlier first sources;
-// pieces of more than 64 x 64 sources use a binary search over qpre.
+// A piece of up to kNearPiece near leaves in LDS, then a sentinel leaf.
+// info: per leaf (first sorted index - first piece index, centre offset
+// from the target leaf's centre as FP32 bits); the sentinel's first index
+// maps piece source `total` to zero record n.  qpre: each leaf's first piece
+// index (total for the sentinel).  Pieces of at most 64 x 64 sources also
+// mark each leaf's first source, and the sentinel's, in a bitmap of 65
+// words; word w holds the bitmap word shifted down by one bit (low half in
+// .x, high half in .y) and, in .z, the marks before the word minus one plus
+// the word's bit 0, so that source 64 w + l belongs to leaf
+// .z + (marks of .x/.y below lane l).  Longer pieces search qpre.
 struct NearPiece {
-    int4 info[kNearPiece];
-    unsigned long long starts[kNearPiece];
-    int wbase[kNearPiece];
+    int4 info[kNearPiece + 1];
+    int4 word[kNearPiece + 1];
+    unsigned long long starts[kNearPiece + 1];
     int qpre[kNearPiece + 1];
 };
 
@@ -1319,13 +1333,16 @@ template <bool SAFE, int NT>
 __device__ __forceinline__ void near_cells_chunk(
     const NearItem& item, double cx, double cy, double cz, float pad, float eps2,
     const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
-    const float4* __restrict__ srec, long long no, const int* __restrict__ near,
+    const float4* __restrict__ srec, int nrec0, long long no, const int* __restrict__ near,
     const float4* tgt, NearPiece& pc, float (&red)[4][kRowQuad][4], double (&a64)[3]) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x, q = lane & 3, row = lane >> 4;
+    float acc[NT][3];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = acc[t][2] = 0.0f;
     for (int q0 = item.q0; q0 < item.q1; q0 += kNearPiece) {
         const int kq = min(kNearPiece, item.q1 - q0);
-        __syncthreads();
+        __syncthreads();  // the previous piece's readers are done
         int len = 0, first = 0;
         float ox = 0.0f, oy = 0.0f, oz = 0.0f;
         if (lane < kq) {
@@ -1342,33 +1359,51 @@ __device__ __forceinline__ void near_cells_chunk(
             if (lane >= o) incl += v;
         }
         const int excl = incl - len;
-        const int total = __shfl(incl, 63, 64);
-        const bool bits = total <= kNearPiece * 64;  // uniform
-        pc.info[lane] = make_int4(first - excl, __float_as_int(ox), __float_as_int(oy), __float_as_int(oz));
-        pc.qpre[lane + 1] = incl;
-        if (lane == 0) pc.qpre[0] = 0;
+        // uniform, and in scalar registers: the step loop below has no
+        // divergent branch
+        const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+        const bool bits = total <= kNearPiece * 64;
+        if (lane < kq)
+            pc.info[lane] = make_int4(first - excl, __float_as_int(ox), __float_as_int(oy), __float_as_int(oz));
+        if (lane == 0) {
+            pc.info[kq] = make_int4(nrec0 - total, __float_as_int(pad), __float_as_int(pad), __float_as_int(pad));
+            pc.qpre[0] = 0;
+            pc.starts[kNearPiece] = 0ull;
+        }
+        if (lane < kq) pc.qpre[lane + 1] = incl;
         pc.starts[lane] = 0ull;
         __syncthreads();
-        if (bits && lane < kq) atomicOr(&pc.starts[excl >> 6], 1ull << (excl & 63));
+        if (bits) {
+            if (lane < kq) atomicOr(&pc.starts[excl >> 6], 1ull << (excl & 63));
+            if (lane == 0) atomicOr(&pc.starts[total >> 6], 1ull << (total & 63));
+        }
         __syncthreads();
         if (bits) {
-            const int cnt = __popcll(pc.starts[lane]);
+            const unsigned long long sw = pc.starts[lane];
+            const int cnt = __popcll(sw);
             int wi = cnt;
             for (int o = 1; o < 64; o <<= 1) {
                 const int v = __shfl_up(wi, o, 64);
                 if (lane >= o) wi += v;
             }
-            pc.wbase[lane] = wi - cnt;
+            pc.word[lane] = make_int4(static_cast<int>(sw >> 1), static_cast<int>(sw >> 33),
+                                      wi - cnt - 1 + static_cast<int>(sw & 1ull), 0);
+            if (lane == 63) {
+                const unsigned long long s64 = pc.starts[kNearPiece];
+                pc.word[kNearPiece] = make_int4(static_cast<int>(s64 >> 1), static_cast<int>(s64 >> 33),
+                                                wi - 1 + static_cast<int>(s64 & 1ull), 0);
+            }
         }
         __syncthreads();
-        // leaf slot of piece source j (-1 past the end)
-        auto locate = [&](int j) -> int {
-            if (j >= total) return -1;
+        // leaf slot of piece source j = 64 w + lane (the sentinel's past the end)
+        auto locate = [&](int w, int j) -> int {
             if (bits) {
-                const int w = j >> 6;
-                return pc.wbase[w] + __popcll(pc.starts[w] << (63 - (j & 63))) - 1;
+                const int4 wd = pc.word[w];
+                return static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                    static_cast<unsigned>(wd.y),
+                    __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(wd.x), static_cast<unsigned>(wd.z))));
             }
-            int lo = 0, hi = kq - 1;  // last leaf with qpre <= j
+            int lo = 0, hi = kq;  // last leaf with qpre <= j
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
                 if (pc.qpre[mid] <= j) lo = mid;
@@ -1376,30 +1411,18 @@ __device__ __forceinline__ void near_cells_chunk(
             }
             return lo;
         };
-        float acc[NT][3];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = acc[t][2] = 0.0f;
-        int nlo = locate(lane);
-        int4 ninf = make_int4(0, 0, 0, 0);
-        float4 nrec = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (nlo >= 0) {
-            ninf = pc.info[nlo];
-            nrec = srec[lane + ninf.x];
-        }
+        int4 ninf = pc.info[locate(0, lane)];
+        float4 nrec = srec[lane + ninf.x];
         for (int cb = 0; cb < total; cb += kEvalLanes) {
-            float x = pad, y = pad, z = pad, gm = 0.0f;
-            if (nlo >= 0) {
-                x = __int_as_float(ninf.y) + nrec.x;
-                y = __int_as_float(ninf.z) + nrec.y;
-                z = __int_as_float(ninf.w) + nrec.z;
-                gm = nrec.w;
-            }
+            const float x = __int_as_float(ninf.y) + nrec.x;
+            const float y = __int_as_float(ninf.z) + nrec.y;
+            const float z = __int_as_float(ninf.w) + nrec.z;
+            const float gm = nrec.w;
+            // the next step's source, also after the last step (the
+            // sentinel's: j - total < 2 kEvalLanes, and w <= kNearPiece)
             const int jn = cb + kEvalLanes + lane;
-            nlo = cb + kEvalLanes < total ? locate(jn) : -1;
-            if (nlo >= 0) {
-                ninf = pc.info[nlo];
-                nrec = srec[jn + ninf.x];
-            }
+            ninf = pc.info[locate((cb >> 6) + 1, jn)];
+            nrec = srec[jn + ninf.x];
             // the targets stay in LDS (read each step, not held in registers),
             // each read two targets ahead of its use (3,947 us against 4,065
             // us reading each at its use, Plummer N = 2^20)
@@ -1424,22 +1447,22 @@ __device__ __forceinline__ void near_cells_chunk(
                 acc[t][2] = gc_fma(dz, kk, acc[t][2]);
             }
         }
+    }
 #pragma unroll
-        for (int t0 = 0; t0 < NT; t0 += kRowQuad) {
-            float r[3];
+    for (int t0 = 0; t0 < NT; t0 += kRowQuad) {
+        float r[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            r[d] = row_quad_sum(acc[t0][d], acc[t0 + 1][d], acc[t0 + 2][d], acc[t0 + 3][d], lane);
+        __syncthreads();
+        if ((lane & 15) < kRowQuad)
+            for (int d = 0; d < 3; ++d) red[row][q][d] = r[d];
+        __syncthreads();
+        if ((lane >> 2) == (t0 >> 2)) {
 #pragma unroll
             for (int d = 0; d < 3; ++d)
-                r[d] = row_quad_sum(acc[t0][d], acc[t0 + 1][d], acc[t0 + 2][d], acc[t0 + 3][d], lane);
-            __syncthreads();
-            if ((lane & 15) < kRowQuad)
-                for (int d = 0; d < 3; ++d) red[row][q][d] = r[d];
-            __syncthreads();
-            if ((lane >> 2) == (t0 >> 2)) {
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    a64[d] += static_cast<double>(((red[0][q][d] + red[1][q][d]) + red[2][q][d]) +
-                                                  red[3][q][d]);
-            }
+                a64[d] += static_cast<double>(((red[0][q][d] + red[1][q][d]) + red[2][q][d]) +
+                                              red[3][q][d]);
         }
     }
 }
@@ -1449,7 +1472,7 @@ __global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4)))
     const NearItem* __restrict__ items, const int* __restrict__ leaves,
     const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
-    const float4* __restrict__ srec, const long long* __restrict__ near_off,
+    const float4* __restrict__ srec, int nrec0, const long long* __restrict__ near_off,
     const int* __restrict__ near, double eps, const int* __restrict__ tpos,
     const int* __restrict__ toff, const int* __restrict__ tcnt, double* __restrict__ part) {
     __shared__ float4 tgt[kNearTargets];
@@ -1486,8 +1509,8 @@ __global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4)))
         }
         double a64[3] = {0.0, 0.0, 0.0};  // lane t: target t
 #define GC_NC(NT)                                                                                  \
-    near_cells_chunk<SAFE, NT>(item, cx, cy, cz, pad, eps2, geom, nb, ne, srec, no, near, tgt, pc,    \
-                               red, a64)
+    near_cells_chunk<SAFE, NT>(item, cx, cy, cz, pad, eps2, geom, nb, ne, srec, nrec0, no, near, tgt, \
+                               pc, red, a64)
         switch ((c + 3) / 4) {  // uniform per block
         case 1: GC_NC(4); break;
         case 2: GC_NC(8); break;
@@ -2560,13 +2583,15 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
                        tcnt, s.ax, s.ay, s.az)
 #define GC_NEAR32(S)                                                                             \
     hipLaunchKernelGGL((eval_near_cells_kernel<S>), dim3(static_cast<unsigned>(nitems)), block, 0, nst, \
-                       items, leaves, geom, nb, ne, sxs, sys, szs, srec, near_off, near, eps, tp,  \
-                       toff, tcnt, npart)
+                       items, leaves, geom, nb, ne, sxs, sys, szs, srec, static_cast<int>(s.n),    \
+                       near_off, near, eps, tp, toff, tcnt, npart)
     // far field on st, near field on the side stream: independent until the
     // merge (the far kernels write the accelerations, the near kernels their
     // own partial rows)
-    if (mixed && static_cast<size_t>(s.n) > srec_cap)
-        FMM_HIP(realloc_dev(&srec, &srec_cap, static_cast<size_t>(s.n)), "allocate source records");
+    // the particles' records, then 2 kEvalLanes zero records
+    if (mixed && static_cast<size_t>(s.n) + 2 * kEvalLanes > srec_cap)
+        FMM_HIP(realloc_dev(&srec, &srec_cap, static_cast<size_t>(s.n) + 2 * kEvalLanes),
+                "allocate source records");
     if (!side) FMM_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "create FMM side stream");
     if (!fork) FMM_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "create FMM events");
     if (!join) FMM_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming), "create FMM events");
@@ -2582,7 +2607,7 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
                                         std::fabs(G) * mass_abs_max * std::max(rs, rs * rs * rs) <= 1e30);
         if (self_zero) {
             hipLaunchKernelGGL(near_records_kernel, grid, block, 0, nst, leaves, geom, nb, ne, sxs, sys,
-                               szs, sms, G, srec);
+                               szs, sms, G, static_cast<int>(s.n), srec);
             if (safe) GC_NEAR32(true);
             else GC_NEAR32(false);
         } else {
