@@ -420,48 +420,74 @@ bool use_step_graph(gc_ctx* c) {
     return cs == hipStreamCaptureStatusNone;
 }
 
-// Launches `chunks` replays of one chunk of c->graph_steps steps on the
-// context's stream, capturing the chunk first (body() launches it on
-// c->stream, which points at the capture stream meanwhile: kernels only, no
-// allocation, copy or memset) unless the graph of an identical chunk is
-// already there.
+// Makes c->step_graph the graph of one chunk of c->graph_steps steps,
+// capturing it unless the graph of an identical chunk is already there:
+// body() launches the chunk on c->stream, which points at the capture
+// stream meanwhile (kernels only: no allocation, copy or memset).  Nothing
+// runs on the device here.
 template <class Body>
-gc_status replay_chunks(gc_ctx* c, double dt, int chunks, Body body) {
+gc_status ensure_step_graph(gc_ctx* c, double dt, Body body) {
     const StepKey key = make_step_key(c, dt);
-    if (!c->step_graph || !(c->step_key == key)) {
-        drop_step_graph(c);
-        if (!c->capture_stream)
-            GC_HIP(c, hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking),
-                   "create step capture stream");
-        hipGraph_t g = nullptr;
-        GC_HIP(c, hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal),
-               "begin step capture");
-        hipStream_t const user = c->stream;
-        c->stream = c->capture_stream;
-        gc_status st = body();
-        c->stream = user;
-        const hipError_t le = hipGetLastError();
-        const hipError_t ee = hipStreamEndCapture(c->capture_stream, &g);
-        if (st == GC_OK && le != hipSuccess)
-            st = fail(c, GC_ERR_HIP, std::string("capture step kernels: ") + hipGetErrorString(le));
-        if (st == GC_OK && ee != hipSuccess)
-            st = fail(c, GC_ERR_HIP, std::string("end step capture: ") + hipGetErrorString(ee));
-        if (st != GC_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return st;
-        }
-        const hipError_t ie = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) {
-            c->step_graph = nullptr;
-            return fail(c, ie == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
-                        std::string("instantiate step graph: ") + hipGetErrorString(ie));
-        }
-        c->step_key = key;
+    if (c->step_graph && c->step_key == key) return GC_OK;
+    drop_step_graph(c);
+    if (!c->capture_stream)
+        GC_HIP(c, hipStreamCreateWithFlags(&c->capture_stream, hipStreamNonBlocking),
+               "create step capture stream");
+    hipGraph_t g = nullptr;
+    GC_HIP(c, hipStreamBeginCapture(c->capture_stream, hipStreamCaptureModeThreadLocal),
+           "begin step capture");
+    hipStream_t const user = c->stream;
+    c->stream = c->capture_stream;
+    gc_status st = body();
+    c->stream = user;
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(c->capture_stream, &g);
+    if (st == GC_OK && le != hipSuccess)
+        st = fail(c, GC_ERR_HIP, std::string("capture step kernels: ") + hipGetErrorString(le));
+    if (st == GC_OK && ee != hipSuccess)
+        st = fail(c, GC_ERR_HIP, std::string("end step capture: ") + hipGetErrorString(ee));
+    if (st != GC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
     }
-    for (int i = 0; i < chunks; ++i)
-        GC_HIP(c, hipGraphLaunch(c->step_graph, c->stream), "launch step graph");
+    const hipError_t ie = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        c->step_graph = nullptr;
+        return fail(c, ie == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
+                    std::string("instantiate step graph: ") + hipGetErrorString(ie));
+    }
+    c->step_key = key;
     return GC_OK;
+}
+
+// Launches `chunks` replays of the chunk graph on the context's stream and
+// returns true, or returns false when the chunk cannot be captured or
+// instantiated: then nothing has been launched, the context's error text is
+// left as it was, and the caller launches those steps eagerly (a step that
+// has begun is never left half done for the sake of a graph).  A failed
+// launch of the graph is an error of the step.
+template <class Body>
+bool replay_chunks(gc_ctx* c, double dt, int chunks, Body body, gc_status* st) {
+    const std::string saved = c->err, saved_thread = g_thread_err;
+    if (ensure_step_graph(c, dt, body) != GC_OK) {
+        (void)hipGetLastError();
+        drop_step_graph(c);
+        c->err = saved;
+        g_thread_err = saved_thread;
+        *st = GC_OK;
+        return false;
+    }
+    *st = GC_OK;
+    for (int i = 0; i < chunks; ++i) {
+        const hipError_t e = hipGraphLaunch(c->step_graph, c->stream);
+        if (e != hipSuccess) {
+            *st = fail(c, e == hipErrorOutOfMemory ? GC_ERR_OUT_OF_MEMORY : GC_ERR_HIP,
+                       std::string("launch step graph: ") + hipGetErrorString(e));
+            break;
+        }
+    }
+    return true;
 }
 
 gc_status need_particles(gc_ctx* c) {
@@ -793,11 +819,11 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
         launch_open_step(c->s, half, dt, c->stream);
         launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
         int middle = nsteps - 1;
-        if (graph && middle >= chunk) {
-            st = replay_chunks(c, dt, middle / chunk, [&] {
+        if (graph && middle >= chunk &&
+            replay_chunks(c, dt, middle / chunk, [&] {
                 launch_sym_middle(c, dt, chunk);
                 return GC_OK;
-            });
+            }, &st)) {
             if (st != GC_OK) return st;
             middle %= chunk;
         }
@@ -807,8 +833,8 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
         return launch_status(c, "launch step kernels");
     }
     int left = nsteps;
-    if (graph && left >= chunk) {
-        st = replay_chunks(c, dt, left / chunk, [&] { return launch_plain_steps(c, dt, chunk); });
+    if (graph && left >= chunk &&
+        replay_chunks(c, dt, left / chunk, [&] { return launch_plain_steps(c, dt, chunk); }, &st)) {
         if (st != GC_OK) return st;
         left %= chunk;
     }

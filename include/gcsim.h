@@ -154,9 +154,10 @@ gc_status gc_compute_accelerations(gc_ctx* ctx);
  * direct solver, whole chunks of gc_get_graph_steps() steps are replayed on
  * the context's stream from a hipGraph (kernels only, captured once on a
  * private stream of the context; recaptured when dt, the parameters, the
- * launch shape or the particle buffers change); the remainder and a stream
- * the caller is capturing get eager launches of the same kernels, so the
- * results are the same either way. */
+ * launch shape or the particle buffers change); the remainder, a stream the
+ * caller is capturing and a chunk that cannot be captured or instantiated
+ * get eager launches of the same kernels, so the results are the same
+ * either way. */
 gc_status gc_step(gc_ctx* ctx, double dt, int nsteps);
 /* Steps per captured gc_step chunk, 0..1000 (default 10; 0 or 1 launches
  * every kernel eagerly).  No reference counterpart: cuda_direct_leapfrog_step
