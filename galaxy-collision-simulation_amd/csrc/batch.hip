@@ -677,18 +677,31 @@ gc_status gc_batch_step(gc_batch* b, double dt, int nsteps) {
     gc_status st = schedule_ready(b);
     if (st != GC_OK) return st;
     int left = nsteps;
-    if (b->graph_steps > 1 && left >= b->graph_steps) {
-        if (!b->graph || b->graph_dt != dt) {
-            drop_graph(b);
-            hipGraph_t g = nullptr;
-            GB_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (b->graph_steps > 1 && left >= b->graph_steps && (!b->graph || b->graph_dt != dt)) {
+        // capture one chunk on the batch's own stream (nothing runs); a chunk
+        // that cannot be captured or instantiated is launched eagerly below
+        drop_graph(b);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
             for (int k = 0; k < b->graph_steps; ++k) launch_step(b, dt);
-            GB_HIP(b, hipStreamEndCapture(b->stream, &g), "end capture");
-            hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            GB_HIP(b, e, "instantiate graph");
-            b->graph_dt = dt;
+            const hipError_t le = hipGetLastError();
+            e = hipStreamEndCapture(b->stream, &g);
+            if (e == hipSuccess) e = le;
         }
+        if (e == hipSuccess && g) {
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) b->graph = nullptr;
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (e == hipSuccess && b->graph) {
+            b->graph_dt = dt;
+        } else {
+            (void)hipGetLastError();
+            drop_graph(b);
+        }
+    }
+    if (b->graph_steps > 1 && b->graph && b->graph_dt == dt) {
         while (left >= b->graph_steps) {
             GB_HIP(b, hipGraphLaunch(b->graph, b->stream), "launch graph");
             left -= b->graph_steps;
