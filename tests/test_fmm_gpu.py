@@ -65,6 +65,21 @@ def test_large_leaves(gpu):
     _check(gpu, pos, m, 0.02, 0.5, 300)
 
 
+def test_pieces_of_sixty_four_full_leaves(gpu):
+    # a cell-centred 32^3 lattice with leaves of up to 64 particles: 512
+    # leaves of exactly 64 and near lists of about 137 leaves, so the near
+    # kernel's pieces of 64 near leaves hold exactly 64 x 64 sources, the
+    # largest piece its bitmap covers (the sentinel leaf then starts in the
+    # bitmap's last word)
+    g = (np.arange(32) + 0.5) / 32
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    m = np.random.default_rng(5).uniform(0.5, 1.5, len(pos))
+    want, a64, a32, st = _check(gpu, pos, m, 0.01, 0.5, 64)
+    assert st["leaf_count"] == 512
+    assert st["near_interactions"] >= 2 * 64 * 512
+
+
 def test_tiny_softening_checks_the_self_pair(gpu):
     # eps = 1e-13: the self pair's factor G m eps^-3 (1e39) would overflow
     # FP32, so the mixed near field drops the self pair by comparing sorted
