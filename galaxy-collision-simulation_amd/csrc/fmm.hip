@@ -1269,9 +1269,10 @@ __global__ __launch_bounds__(kEvalLanes) void eval_near_kernel(
 }
 
 // Near field in FP32 with the sources across the lanes.  A chunk of c <= 24
-// targets is evaluated for NT = c rounded up to a multiple of 4 target
+// targets is evaluated for NT = c rounded up to an even number of target
 // slots, each with one FP32 sum per component in every lane's registers
-// (the spare slots hold a padding target whose sums are never written).
+// (a spare slot holds a padding target whose sums are never written; the
+// network below adds slots in fours, the slots past NT holding zeros).
 // The item's sources, 64 near leaves at a time laid end to end, pass 64 per
 // step: lane k takes source k of the step, whose leaf it finds with two
 // mbcnt instructions from a bitmap of the leaves' first sources (NearPiece),
@@ -1337,9 +1338,10 @@ __device__ __forceinline__ void near_cells_chunk(
     const float4* tgt, NearPiece& pc, float (&red)[4][kRowQuad][4], double (&a64)[3]) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x, q = lane & 3, row = lane >> 4;
-    float acc[NT][3];
+    constexpr int NTR = (NT + kRowQuad - 1) / kRowQuad * kRowQuad;  // network slots
+    float acc[NTR][3];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t][0] = acc[t][1] = acc[t][2] = 0.0f;
+    for (int t = 0; t < NTR; ++t) acc[t][0] = acc[t][1] = acc[t][2] = 0.0f;
     for (int q0 = item.q0; q0 < item.q1; q0 += kNearPiece) {
         const int kq = min(kNearPiece, item.q1 - q0);
         __syncthreads();  // the previous piece's readers are done
@@ -1449,7 +1451,7 @@ __device__ __forceinline__ void near_cells_chunk(
         }
     }
 #pragma unroll
-    for (int t0 = 0; t0 < NT; t0 += kRowQuad) {
+    for (int t0 = 0; t0 < NTR; t0 += kRowQuad) {
         float r[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d)
@@ -1511,12 +1513,18 @@ __global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4)))
 #define GC_NC(NT)                                                                                  \
     near_cells_chunk<SAFE, NT>(item, cx, cy, cz, pad, eps2, geom, nb, ne, srec, nrec0, no, near, tgt, \
                                pc, red, a64)
-        switch ((c + 3) / 4) {  // uniform per block
-        case 1: GC_NC(4); break;
-        case 2: GC_NC(8); break;
-        case 3: GC_NC(12); break;
-        case 4: GC_NC(16); break;
-        case 5: GC_NC(20); break;
+        switch ((c + 1) / 2) {  // uniform per block
+        case 1: GC_NC(2); break;
+        case 2: GC_NC(4); break;
+        case 3: GC_NC(6); break;
+        case 4: GC_NC(8); break;
+        case 5: GC_NC(10); break;
+        case 6: GC_NC(12); break;
+        case 7: GC_NC(14); break;
+        case 8: GC_NC(16); break;
+        case 9: GC_NC(18); break;
+        case 10: GC_NC(20); break;
+        case 11: GC_NC(22); break;
         default: GC_NC(24); break;
         }
 #undef GC_NC
