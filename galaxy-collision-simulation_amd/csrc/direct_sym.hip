@@ -128,12 +128,12 @@ __device__ __forceinline__ void diag_chunk(const float4* row, int lane, float ep
 // source sums are shared by a lane's IPT targets, so the three DPP moves of
 // a step serve 2 * IPT interactions.
 template <int IPT, bool SAFE>
-__global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 4))) void direct_sym_kernel(
+__global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(4, 4))) void direct_sym_kernel(
     const SymItem* __restrict__ items, int nitems, int n, const int* __restrict__ tile_valid,
     int ntiles, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const float4* __restrict__ jp,
-    const double* __restrict__ centre, float eps2, float* __restrict__ ipart,
-    float* __restrict__ jpart) {
+    const double* __restrict__ centre, float eps2, const float* __restrict__ wt,
+    float* __restrict__ ipart, float* __restrict__ jpart) {
     constexpr int kThreads = kTile / IPT;
     constexpr int kWaves = kThreads / 64;
     // Source tile J in LDS, each 64-source chunk stored twice in a row so
@@ -146,6 +146,8 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const SymItem item = items[it];
         const int I = item.I;
+        // G m of every particle of tile I when they are all equal (0: not)
+        const float wI = (!SAFE && IPT == 4 && wt != nullptr) ? wt[I] : 0.f;
         double xi[IPT], yi[IPT], zi[IPT];
         float mi[IPT];
         const int valid = tile_valid ? tile_valid[I] : n - I * kTile;
@@ -213,7 +215,30 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
             float* px = qx;
             float* py = qy;
             float* pz = qz;
-            tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
+            // Equal-mass tiles I and J (w = G m): offsets scaled by
+            // w_J^-1/2 and eps^2 by 1 / w_J give d' r'^3 = w_J d r^-3, the
+            // target's term, with no mass product; the source sums
+            // sum d' r'^3 are scaled by -w_I / w_J once per chunk.
+            // eps'^2 rides in the fourth slot of each staged source.
+            const float wJ = wI != 0.f ? wt[J] : 0.f;
+            const bool uni = wJ != 0.f;
+            double sd = 1.0;
+            float sJ = 1.f, eps2u = eps2, fJ = 0.f;
+            if (uni) {
+                sd = 1.0 / sqrt(static_cast<double>(wJ));
+                sJ = static_cast<float>(sd);
+                eps2u = static_cast<float>(static_cast<double>(eps2) / static_cast<double>(wJ));
+                fJ = static_cast<float>(-static_cast<double>(wI) / static_cast<double>(wJ));
+                const double cx = centre[J], cy = centre[ntiles + J], cz = centre[2 * ntiles + J];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    tx[k] = static_cast<float>((xi[k] - cx) * sd);
+                    ty[k] = static_cast<float>((yi[k] - cy) * sd);
+                    tz[k] = static_cast<float>((zi[k] - cz) * sd);
+                }
+            } else {
+                tile_offsets<IPT>(centre, ntiles, J, xi, yi, zi, tx, ty, tz);
+            }
             const float4* src = jp + static_cast<size_t>(J) * kTile;
             // source chunks holding particles of tile J (the rest is padding:
             // zero-mass sources whose sums belong to no particle, which the
@@ -223,7 +248,8 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
 #pragma unroll
                 for (int u = 0; u < kPer; ++u) {
                     const int e = t + u * kThreads;
-                    const float4 v = J > item.jb ? pre[u] : src[e];
+                    float4 v = J > item.jb ? pre[u] : src[e];
+                    if (uni) v = make_float4(v.x * sJ, v.y * sJ, v.z * sJ, eps2u);
                     src2[e / kChunk][e % kChunk] = v;
                     src2[e / kChunk][kChunk + e % kChunk] = v;
                 }
@@ -240,6 +266,32 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
                     // at step s; wave_rol1 after each step keeps that true.
                     float ax = 0.f, ay = 0.f, az = 0.f;
                     const float4* row = &src2[c][lane];
+                    if (uni) {
+#pragma unroll 8
+                        for (int s = 0; s < kChunk; ++s) {
+                            const float4 q = row[s];
+#pragma unroll
+                            for (int k = 0; k < IPT; ++k) {
+                                const float dx = q.x - tx[k], dy = q.y - ty[k], dz = q.z - tz[k];
+                                const float s2 = __builtin_fmaf(
+                                    dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, q.w)));
+                                const float r = __builtin_amdgcn_rsqf(s2);
+                                const float r3 = (r * r) * r;
+                                px[k] = __builtin_fmaf(dx, r3, px[k]);
+                                py[k] = __builtin_fmaf(dy, r3, py[k]);
+                                pz[k] = __builtin_fmaf(dz, r3, pz[k]);
+                                ax = __builtin_fmaf(dx, r3, ax);
+                                ay = __builtin_fmaf(dy, r3, ay);
+                                az = __builtin_fmaf(dz, r3, az);
+                            }
+                            ax = wave_rol1(ax);
+                            ay = wave_rol1(ay);
+                            az = wave_rol1(az);
+                        }
+                        ax *= fJ;
+                        ay *= fJ;
+                        az *= fJ;
+                    } else {
 #pragma unroll 8
                     for (int s = 0; s < kChunk; ++s) {
                         const float4 q = row[s];
@@ -266,6 +318,7 @@ __global__ __launch_bounds__(kTile / IPT) __attribute__((amdgpu_waves_per_eu(1, 
                         ax = wave_rol1(ax);
                         ay = wave_rol1(ay);
                         az = wave_rol1(az);
+                    }
                     }
                     // after 64 steps lane l holds source c*64 + l; wave 0 adds
                     // the other waves' sums in wave order and stores the chunk
@@ -695,6 +748,29 @@ __global__ __launch_bounds__(kMergeLanes * kVjpMergeGroups) void vjp_sym_merge_k
     }
 }
 
+// wt[T] = the G m of tile T's particles when the tile is full and they all
+// share it, within [2^-40, 2^40] and with eps^2 / (G m) in [1e-20, 1e30]
+// (the scaled form of the symmetric kernel stays finite), else 0.  One wave
+// per tile.
+__global__ __launch_bounds__(256) void sym_tile_weights_kernel(const float4* __restrict__ jp,
+                                                               const int* __restrict__ tile_valid,
+                                                               int n, int ntiles, float eps2,
+                                                               float* __restrict__ wt) {
+    const int T = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (T >= ntiles) return;
+    const int valid = tile_valid ? tile_valid[T] : n - T * kTile;
+    const float4* src = jp + static_cast<size_t>(T) * kTile;
+    const float w0 = src[0].w;
+    bool same = valid >= kTile;
+#pragma unroll
+    for (int k = 0; k < kTile / 64; ++k) same = same && src[k * 64 + lane].w == w0;
+    const bool all_same = __all(same);
+    const bool in_range = w0 >= 0x1p-40f && w0 <= 0x1p40f && eps2 >= 1e-20f * w0 &&
+                          eps2 <= 1e30f * w0;
+    if (lane == 0) wt[T] = all_same && in_range ? w0 : 0.f;
+}
+
 }  // namespace
 
 int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nruns, int len,
@@ -712,6 +788,7 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
     if (e == hipSuccess) e = hipMalloc(&plan.tile_pbase, sizeof(long long) * h.tile_pbase.size());
     if (e == hipSuccess) e = hipMalloc(&plan.ipart, sizeof(float) * plan.ipart_floats);
     if (e == hipSuccess) e = hipMalloc(&plan.jpart, sizeof(float) * plan.jpart_floats);
+    if (e == hipSuccess) e = hipMalloc(&plan.wt, sizeof(float) * std::max(1, h.ntiles));
     if (e == hipSuccess)
         e = hipMemcpyAsync(plan.items, h.items.data(), sizeof(SymItem) * h.items.size(),
                            hipMemcpyHostToDevice, st);
@@ -753,6 +830,7 @@ void sym_release(SymPlan& plan) {
     (void)hipFree(plan.tile_pbase);
     (void)hipFree(plan.ipart);
     (void)hipFree(plan.jpart);
+    (void)hipFree(plan.wt);
     plan = SymPlan{};
 }
 
@@ -766,11 +844,17 @@ void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const dou
     if (safe) {
         hipLaunchKernelGGL((direct_sym_kernel<IPT, true>), grid, dim3(kTile / IPT), lds, st,
                            plan.items, plan.nitems, n, tile_valid, ntiles, s.x, s.y, s.z, jp, centre,
-                           eps2, plan.ipart, plan.jpart);
+                           eps2, nullptr, plan.ipart, plan.jpart);
     } else {
+        const float* wt = nullptr;
+        if (IPT == 4 && plan.wt != nullptr && ntiles <= plan.ntiles) {
+            hipLaunchKernelGGL(sym_tile_weights_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, st,
+                               jp, tile_valid, n, ntiles, eps2, plan.wt);
+            wt = plan.wt;
+        }
         hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), grid, dim3(kTile / IPT), lds, st,
                            plan.items, plan.nitems, n, tile_valid, ntiles, s.x, s.y, s.z, jp, centre,
-                           eps2, plan.ipart, plan.jpart);
+                           eps2, wt, plan.ipart, plan.jpart);
     }
 }
 

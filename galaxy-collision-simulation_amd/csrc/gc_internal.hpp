@@ -59,6 +59,8 @@ struct SymPlan {
     long long* tile_pbase = nullptr;  // pair-partial base of each tile's run
     float* ipart = nullptr;     // nitems * 3 * kTile row partials (x | y | z), item slots
     float* jpart = nullptr;     // per run nt (nt - 1) / 2 pairs * 3 * kTile
+    float* wt = nullptr;        // ntiles: the G m that every particle of a full tile
+                                // shares, or 0 (written before each evaluation)
     size_t ipart_floats = 0, jpart_floats = 0;
 };
 

@@ -226,6 +226,57 @@ def test_mixed_symmetric_edge_cases(gpu):
     assert rel_err(got, O.direct_targets(pos[:333], m[:333], 1.0, 0.02)).max() <= MIXED_TOL
 
 
+def _equal_mass_cloud(n, seed):
+    """A random cloud whose 256-particle tiles mostly hold equal masses:
+    tiles 0-1 one mass, tile 2 another, tile 3 equal masses but one particle
+    1 ulp heavier, the rest a third mass; a partial last tile when n is not
+    a multiple of 256."""
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(n, 3))
+    m = np.full(n, 0.37)
+    m[512:768] = 2.9e-3
+    m[768:1024] = 0.11
+    m[800] = np.nextafter(np.float32(0.11), np.float32(1.0))
+    m[1024:] = 1.3e-5
+    return pos, m
+
+
+@pytest.mark.parametrize("n", [2048, 3001])
+def test_mixed_symmetric_equal_mass_tiles(gpu, n):
+    # tile pairs whose particles share one mass per tile are evaluated in
+    # coordinates scaled by (G m_J)^-1/2 with no per-pair mass product; tiles
+    # of unequal masses, partial tiles and the diagonal tiles take the
+    # general pair step.  Both are held to the FP32 rounding bound of the
+    # random-cloud test against the FP64 oracle.
+    pos, m = _equal_mass_cloud(n, 31 + n)
+    for G, eps in ((1.0, 0.02), (0.7, 0.3)):
+        ref = O.direct_targets(pos, m, G, eps)
+        got = _accel(gpu, "mixed", pos, None, m, eps, G=G, pairs="symmetric")
+        err = np.linalg.norm(got - ref, axis=1)
+        assert (err <= 64 * 2.0**-24 * pair_magnitude_sums(pos, m, G, eps)).all(), (G, eps)
+        assert np.median(rel_err(got, ref)) < 1e-6
+        scale = np.abs(m[:, None] * ref).sum(axis=0).max()
+        assert np.abs((m[:, None] * got).sum(axis=0)).max() <= 1e-6 * scale
+        # the per-target form (no tile-mass factoring) agrees as closely
+        tgt = _accel(gpu, "mixed", pos, None, m, eps, G=G, pairs="targets")
+        assert (np.linalg.norm(got - tgt, axis=1)
+                <= 128 * 2.0**-24 * pair_magnitude_sums(pos, m, G, eps)).all()
+
+
+@pytest.mark.parametrize("gm, eps", [(1e-13, 0.02), (2e12, 0.02), (1.0, 1e-11), (4e-6, 1e-13)])
+def test_mixed_symmetric_equal_mass_outside_scaled_range(gpu, gm, eps):
+    # G m outside [2^-40, 2^40], or eps^2 / (G m) below 1e-20: the scaled
+    # form is not used and the general step gives the oracle's forces
+    rng = np.random.default_rng(5)
+    n = 1024
+    pos, m = rng.normal(size=(n, 3)), np.full(n, gm)
+    ref = O.direct_targets(pos, m, 1.0, eps)
+    got = _accel(gpu, "mixed", pos, None, m, eps, pairs="symmetric")
+    err = np.linalg.norm(got - ref, axis=1)
+    assert np.isfinite(got).all()
+    assert (err <= 64 * 2.0**-24 * pair_magnitude_sums(pos, m, 1.0, eps)).all()
+
+
 def test_mixed_symmetric_kdk_tracks_fp64(gpu):
     pos, vel, m = gpu.generators.config0()
     with _engine(gpu, "mixed", 0.05) as mx, _engine(gpu, "fp64", 0.05) as fp:
