@@ -55,6 +55,7 @@ constexpr int kEvalLanes = 64;
 constexpr int kStats = 16;
 constexpr int kStatInternal = 4;
 constexpr int kStatMassMax = 5;  // largest |m| of the build (double bits)
+constexpr int kStatMassMin = 6;  // smallest m of the build (double bits)
 constexpr int kStatTotals = 8;   // build totals read with the statistics (build_totals_kernel)
 
 __device__ __forceinline__ float gc_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -1330,9 +1331,9 @@ struct NearPiece {
     int qpre[kNearPiece + 1];
 };
 
-template <bool SAFE, int NT>
+template <bool SAFE, bool UNI, int NT>
 __device__ __forceinline__ void near_cells_chunk(
-    const NearItem& item, double cx, double cy, double cz, float pad, float eps2,
+    const NearItem& item, double cx, double cy, double cz, float pad,
     const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
     const float4* __restrict__ srec, int nrec0, long long no, const int* __restrict__ near,
     const float4* tgt, NearPiece& pc, float (&red)[4][kRowQuad][4], double (&a64)[3]) {
@@ -1368,7 +1369,10 @@ __device__ __forceinline__ void near_cells_chunk(
         if (lane < kq)
             pc.info[lane] = make_int4(first - excl, __float_as_int(ox), __float_as_int(oy), __float_as_int(oz));
         if (lane == 0) {
-            pc.info[kq] = make_int4(nrec0 - total, __float_as_int(pad), __float_as_int(pad), __float_as_int(pad));
+            // with one mass for all (UNI) the sentinel's zero records carry
+            // no G m = 0: they sit 1e16 away, where r^3 underflows to 0
+            const float sp = UNI ? 1e16f : pad;
+            pc.info[kq] = make_int4(nrec0 - total, __float_as_int(sp), __float_as_int(sp), __float_as_int(sp));
             pc.qpre[0] = 0;
             pc.starts[kNearPiece] = 0ull;
         }
@@ -1436,14 +1440,15 @@ __device__ __forceinline__ void near_cells_chunk(
                 Tn = Tn2;
                 if (t + 2 < NT) Tn2 = tgt[t + 2];
                 const float dx = x - T.x, dy = y - T.y, dz = z - T.z;
-                const float s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, eps2)));
+                // T.w: eps^2 (a VGPR operand, not an SGPR one)
+                const float s2 = gc_fma(dz, dz, gc_fma(dy, dy, gc_fma(dx, dx, T.w)));
                 float r;
                 if (SAFE) {
                     r = s2 > 1e-30f ? gc_rsq(s2) : 0.0f;
                 } else {
                     r = gc_rsq(s2);
                 }
-                const float kk = (gm * r) * (r * r);
+                const float kk = UNI ? (r * r) * r : (gm * r) * (r * r);
                 acc[t][0] = gc_fma(dx, kk, acc[t][0]);
                 acc[t][1] = gc_fma(dy, kk, acc[t][1]);
                 acc[t][2] = gc_fma(dz, kk, acc[t][2]);
@@ -1469,13 +1474,15 @@ __device__ __forceinline__ void near_cells_chunk(
     }
 }
 
-template <bool SAFE>
+// UNI: every particle has the G m `gmu`; the pair terms leave it out and
+// the target sums are multiplied by it in FP64 before they are written.
+template <bool SAFE, bool UNI>
 __global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4))) void eval_near_cells_kernel(
     const NearItem* __restrict__ items, const int* __restrict__ leaves,
     const double* __restrict__ geom, const int* __restrict__ nb, const int* __restrict__ ne,
     const double* __restrict__ sx, const double* __restrict__ sy, const double* __restrict__ sz,
     const float4* __restrict__ srec, int nrec0, const long long* __restrict__ near_off,
-    const int* __restrict__ near, double eps, const int* __restrict__ tpos,
+    const int* __restrict__ near, double eps, float gmu, const int* __restrict__ tpos,
     const int* __restrict__ toff, const int* __restrict__ tcnt, double* __restrict__ part) {
     __shared__ float4 tgt[kNearTargets];
     __shared__ int tgp[kNearTargets];  // sorted positions of the chunk's targets
@@ -1502,16 +1509,16 @@ __global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4)))
             if (lane < c) {
                 const int pos = target_pos(tpos, base, lane);
                 tgt[lane] = make_float4(static_cast<float>(sx[pos] - cx), static_cast<float>(sy[pos] - cy),
-                                        static_cast<float>(sz[pos] - cz), 0.0f);
+                                        static_cast<float>(sz[pos] - cz), eps2);
                 tgp[lane] = pos;
             } else {
-                tgt[lane] = make_float4(-pad, -pad, -pad, 0.0f);
+                tgt[lane] = make_float4(-pad, -pad, -pad, eps2);
                 tgp[lane] = -1;
             }
         }
         double a64[3] = {0.0, 0.0, 0.0};  // lane t: target t
 #define GC_NC(NT)                                                                                  \
-    near_cells_chunk<SAFE, NT>(item, cx, cy, cz, pad, eps2, geom, nb, ne, srec, nrec0, no, near, tgt, \
+    near_cells_chunk<SAFE, UNI, NT>(item, cx, cy, cz, pad, geom, nb, ne, srec, nrec0, no, near, tgt, \
                                pc, red, a64)
         switch ((c + 1) / 2) {  // uniform per block
         case 1: GC_NC(2); break;
@@ -1531,9 +1538,10 @@ __global__ __launch_bounds__(kEvalLanes) __attribute__((amdgpu_waves_per_eu(4)))
         __syncthreads();
         if (lane < c) {
             double* out = part + 3 * (static_cast<size_t>(item.slot) + (tgp[lane] - lb));
-            out[0] = a64[0];
-            out[1] = a64[1];
-            out[2] = a64[2];
+            const double g = UNI ? static_cast<double>(gmu) : 1.0;
+            out[0] = a64[0] * g;
+            out[1] = a64[1] * g;
+            out[2] = a64[2] * g;
         }
     }
 }
@@ -2019,6 +2027,7 @@ struct FmmEngine::Impl {
     int nitems = 0;
     unsigned long long* stats_dev = nullptr;
     double mass_abs_max = 0.0;  // largest |m| of the last build
+    double mass_min = 0.0;      // smallest m of the last build
     float4* srec = nullptr;     // FP32 source records of the near field
     size_t srec_cap = 0;
     // Barnes-Hut work counts (M2P, pairs, leaves) of the last evaluation, in
@@ -2228,6 +2237,14 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
             FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
                     "allocate scan storage");
         FMM_HIP(hipcub::DeviceReduce::Max(scan_tmp, scan_tmp_bytes, am, mx, n, st), "mass reduction");
+        // smallest m: equal to the largest |m| when every particle has one
+        // mass, which lets the near field leave out its per-pair G m
+        double* mn = reinterpret_cast<double*>(stats_dev + kStatMassMin);
+        FMM_HIP(hipcub::DeviceReduce::Min(nullptr, need, sms, mn, n, st), "size mass reduction");
+        if (need > scan_tmp_bytes)
+            FMM_HIP(realloc_dev(reinterpret_cast<unsigned char**>(&scan_tmp), &scan_tmp_bytes, need),
+                    "allocate scan storage");
+        FMM_HIP(hipcub::DeviceReduce::Min(scan_tmp, scan_tmp_bytes, sms, mn, n, st), "mass reduction");
     }
 
     // 4: levels.  Levels 0..lmax hold internal nodes; level lmax + 1 only
@@ -2497,6 +2514,7 @@ int FmmEngine::Impl::build(const DeviceState& s, const gc_params& p, hipStream_t
     hipLaunchKernelGGL(near_item_fill_kernel, dim3((nleaves + 255) / 256), dim3(256), 0, st, nleaves,
                        leaves, nb, ne, nnear, icount, ioff, row_off, items);
     std::memcpy(&mass_abs_max, &sv[kStatMassMax], sizeof(double));
+    std::memcpy(&mass_min, &sv[kStatMassMin], sizeof(double));
     last.node_count = 1 + 8 * static_cast<size_t>(sv[kStatInternal]);
     last.leaf_count = static_cast<size_t>(nleaves);
     last.far_interactions = sv[0];
@@ -2589,10 +2607,10 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
     hipLaunchKernelGGL((eval_far_cells_kernel<O>), grid, block, 0, st, leaves, geom, mcom, kc32, nb, \
                        ne, idx_sorted, sxs, sys, szs, far_off, nfar, far, G, eps, b, e, tp, toff, \
                        tcnt, s.ax, s.ay, s.az)
-#define GC_NEAR32(S)                                                                             \
-    hipLaunchKernelGGL((eval_near_cells_kernel<S>), dim3(static_cast<unsigned>(nitems)), block, 0, nst, \
+#define GC_NEAR32(S, U)                                                                          \
+    hipLaunchKernelGGL((eval_near_cells_kernel<S, U>), dim3(static_cast<unsigned>(nitems)), block, 0, nst, \
                        items, leaves, geom, nb, ne, sxs, sys, szs, srec, static_cast<int>(s.n),    \
-                       near_off, near, eps, tp, toff, tcnt, npart)
+                       near_off, near, eps, gmu, tp, toff, tcnt, npart)
     // far field on st, near field on the side stream: independent until the
     // merge (the far kernels write the accelerations, the near kernels their
     // own partial rows)
@@ -2613,11 +2631,18 @@ int FmmEngine::Impl::launch_eval(const DeviceState& s, const gc_params& p, size_
         const double rs = e2 > 0.0 ? 1.0 / std::sqrt(e2) : HUGE_VAL;
         const bool self_zero = safe || (rs * rs <= 1e30 &&
                                         std::fabs(G) * mass_abs_max * std::max(rs, rs * rs * rs) <= 1e30);
+        // one mass for every particle: the pair terms leave out G m (the
+        // self pair's factor is then eps^-3, finite by rs^3 <= 1e30)
+        const bool uni = !safe && mass_min == mass_abs_max && mass_min > 0.0 &&
+                         rs * rs * rs <= 1e30 && std::isfinite(G * mass_min) && G * mass_min > 0.0 &&
+                         static_cast<float>(G * mass_min) > 0.0f;
+        const float gmu = uni ? static_cast<float>(G * mass_min) : 0.0f;
         if (self_zero) {
             hipLaunchKernelGGL(near_records_kernel, grid, block, 0, nst, leaves, geom, nb, ne, sxs, sys,
                                szs, sms, G, static_cast<int>(s.n), srec);
-            if (safe) GC_NEAR32(true);
-            else GC_NEAR32(false);
+            if (safe) GC_NEAR32(true, false);
+            else if (uni) GC_NEAR32(false, true);
+            else GC_NEAR32(false, false);
         } else {
             GC_NEAR(float, false);
         }

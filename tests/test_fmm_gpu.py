@@ -65,6 +65,26 @@ def test_large_leaves(gpu):
     _check(gpu, pos, m, 0.02, 0.5, 300)
 
 
+@pytest.mark.parametrize("G", [1.0, 3e-4])
+def test_equal_masses_leave_out_the_pair_mass(gpu, G):
+    # every particle of one mass: the FP32 near field sums d r^-3 and
+    # multiplies by G m once per target (its padding sources sit where r^3
+    # underflows to zero); one particle 1 ulp heavier takes the per-pair
+    # G m form.  Both match the oracle's FMM to FP32 rounding.
+    pos, vel, _ = gpu.generators.plummer(4000, seed=23)
+    m = np.full(len(pos), 2.5e-4)
+    m2 = m.copy()
+    m2[1234] = np.nextafter(m2[1234], 1.0)
+    for mass in (m, m2):
+        want, st = O.fmm(pos, mass, G, 0.01, 0.5, 64, 32, 4)
+        with gpu.Engine("hip-fmm", "mixed", softening=0.01, theta=0.5, leaf_capacity=64,
+                        expansion_order=4, G=G) as eng:
+            eng.upload(pos, vel, mass)
+            eng.compute_accelerations()
+            got = eng.download()[2]
+        assert rel_err(got, want).max() < MIXED_TOL
+
+
 def test_pieces_of_sixty_four_full_leaves(gpu):
     # a cell-centred 32^3 lattice with leaves of up to 64 particles: 512
     # leaves of exactly 64 and near lists of about 137 leaves, so the near
