@@ -370,9 +370,11 @@ gc_status launch_plain_steps(gc_ctx* c, double dt, int k) {
 void launch_sym_middle(gc_ctx* c, double dt, int k) {
     const int ipt = sym_ipt(c);
     for (int i = 0; i < k; ++i) {
-        launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, ipt, c->stream);
-        launch_merge_sym_open(c->s, c->sym, c->mixed, c->params.gravitational_constant, 0.5 * dt,
-                              dt, c->stream);
+        // the tile weights come from the pack before the first middle step
+        // and from each merge after it
+        launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, ipt, c->stream, true);
+        launch_merge_sym_open(c->s, c->sym, c->mixed, c->params.gravitational_constant,
+                              c->params.softening, 0.5 * dt, dt, c->stream);
     }
 }
 
@@ -818,6 +820,7 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
         // the middle steps (force + that merge) are what a chunk repeats
         launch_open_step(c->s, half, dt, c->stream);
         launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
+        launch_sym_weights(c->s, c->mixed, c->sym, c->params.softening, c->stream);
         int middle = nsteps - 1;
         if (graph && middle >= chunk &&
             replay_chunks(c, dt, middle / chunk, [&] {
@@ -828,7 +831,7 @@ gc_status gc_step(gc_ctx* c, double dt, int nsteps) {
             middle %= chunk;
         }
         launch_sym_middle(c, dt, middle);
-        launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, sym_ipt(c), c->stream);
+        launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, sym_ipt(c), c->stream, true);
         launch_merge_sym(c->s, c->sym, nullptr, true, half, c->stream);
         return launch_status(c, "launch step kernels");
     }
@@ -998,12 +1001,15 @@ gc_status gc_time_force_kernel(gc_ctx* c, int reps, double* ms, double* inter) {
         }
         if (st != GC_OK) return st;
         launch_pack_mixed(c->s, c->mixed, c->params.gravitational_constant, c->stream);
+        // the tile weights once, as the stepping merge keeps them: the
+        // timed launches are the force kernel's alone
+        if (sym) launch_sym_weights(c->s, c->mixed, c->sym, c->params.softening, c->stream);
     }
     GC_HIP(c, hipEventRecord(c->ev0, c->stream), "record start event");
     for (int r = 0; r < reps; ++r) {
         if (sym) {
             launch_direct_sym(c->s, c->mixed, c->sym, c->params.softening, sym_ipt(c),
-                              c->stream);
+                              c->stream, true);
         } else if (fast64) {
             launch_direct_fp64_fast(c->s, c->jp64, c->part64, c->params.softening, ipt, js, per, 0,
                                     n, c->stream);

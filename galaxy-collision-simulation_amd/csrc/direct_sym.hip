@@ -47,6 +47,14 @@ __host__ __device__ inline size_t pair_slot(int I, int J) {  // I < J
 }
 
 
+// A tile's shared G m is used by the scaled equal-mass pair step only within
+// [2^-40, 2^40] and with eps^2 / (G m) in [1e-20, 1e30], where the scaled
+// squares stay finite wherever the unscaled terms are above FP32's normal
+// range.
+__device__ __forceinline__ bool sym_weight_in_range(float w0, float eps2) {
+    return w0 >= 0x1p-40f && w0 <= 0x1p40f && eps2 >= 1e-20f * w0 && eps2 <= 1e30f * w0;
+}
+
 #ifndef GC_SYM_LDS_PAD
 #define GC_SYM_LDS_PAD 1024
 #endif
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_ker
     double* __restrict__ z, double* __restrict__ vx, double* __restrict__ vy,
     double* __restrict__ vz, double* __restrict__ ax, double* __restrict__ ay,
     double* __restrict__ az, const double* __restrict__ m, double half_dt, double dt, double G,
-    float4* __restrict__ jp, double* __restrict__ centre) {
+    float eps2, float4* __restrict__ jp, double* __restrict__ centre, float* __restrict__ wt) {
 #pragma clang fp contract(off)
     __shared__ double part[kMergeGroups][3][kTile];
     __shared__ double box[2][3][4];   // per component: wave minima, maxima
@@ -495,12 +503,18 @@ __global__ __launch_bounds__(kMergeLanes * kMergeGroups) void merge_sym_open_ker
         if (e == 0) centre[c * ntiles + T] = mid;
     }
     __syncthreads();
+    // the tile's weight for the next evaluation (sym_tile_weights_kernel's)
+    const float w0 = static_cast<float>(G * m[T * kTile]);
+    bool same = true;
     if (t < kTile) {
         float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
         if (t < valid) o = make_float4(offs[0][t], offs[1][t], offs[2][t],
                                        static_cast<float>(G * m[i]));
         jp[i] = o;
+        same = t < valid && o.w == w0;
     }
+    const bool all_same = __syncthreads_and(same) != 0;
+    if (wt != nullptr && t == 0) wt[T] = all_same && sym_weight_in_range(w0, eps2) ? w0 : 0.f;
 }
 
 // ---- force VJP, each unordered pair once ---------------------------------
@@ -749,9 +763,7 @@ __global__ __launch_bounds__(kMergeLanes * kVjpMergeGroups) void vjp_sym_merge_k
 }
 
 // wt[T] = the G m of tile T's particles when the tile is full and they all
-// share it, within [2^-40, 2^40] and with eps^2 / (G m) in [1e-20, 1e30]
-// (the scaled form of the symmetric kernel stays finite), else 0.  One wave
-// per tile.
+// share it (sym_weight_in_range), else 0.  One wave per tile.
 __global__ __launch_bounds__(256) void sym_tile_weights_kernel(const float4* __restrict__ jp,
                                                                const int* __restrict__ tile_valid,
                                                                int n, int ntiles, float eps2,
@@ -766,9 +778,7 @@ __global__ __launch_bounds__(256) void sym_tile_weights_kernel(const float4* __r
 #pragma unroll
     for (int k = 0; k < kTile / 64; ++k) same = same && src[k * 64 + lane].w == w0;
     const bool all_same = __all(same);
-    const bool in_range = w0 >= 0x1p-40f && w0 <= 0x1p40f && eps2 >= 1e-20f * w0 &&
-                          eps2 <= 1e30f * w0;
-    if (lane == 0) wt[T] = all_same && in_range ? w0 : 0.f;
+    if (lane == 0) wt[T] = all_same && sym_weight_in_range(w0, eps2) ? w0 : 0.f;
 }
 
 }  // namespace
@@ -837,7 +847,7 @@ void sym_release(SymPlan& plan) {
 template <int IPT>
 void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const double* centre,
                     const int* tile_valid, int ntiles, const SymPlan& plan, float eps2, int blocks,
-                    hipStream_t st) {
+                    bool weights_ready, hipStream_t st) {
     const int n = static_cast<int>(s.n);
     const dim3 grid(static_cast<unsigned>(blocks > 0 ? std::min(blocks, plan.nitems) : plan.nitems));
     const unsigned lds = IPT == 4 ? kSymLdsPad : 0;
@@ -848,8 +858,9 @@ void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const dou
     } else {
         const float* wt = nullptr;
         if (IPT == 4 && plan.wt != nullptr && ntiles <= plan.ntiles) {
-            hipLaunchKernelGGL(sym_tile_weights_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, st,
-                               jp, tile_valid, n, ntiles, eps2, plan.wt);
+            if (!weights_ready)
+                hipLaunchKernelGGL(sym_tile_weights_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, st,
+                                   jp, tile_valid, n, ntiles, eps2, plan.wt);
             wt = plan.wt;
         }
         hipLaunchKernelGGL((direct_sym_kernel<IPT, false>), grid, dim3(kTile / IPT), lds, st,
@@ -860,20 +871,30 @@ void launch_sym_ipt(bool safe, const DeviceState& s, const float4* jp, const dou
 
 void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const double* centre,
                              const int* tile_valid, int ntiles, const SymPlan& plan, double eps,
-                             int ipt, int blocks, hipStream_t st) {
+                             int ipt, int blocks, hipStream_t st, bool weights_ready) {
     const float eps2 = static_cast<float>(eps * eps);
     const bool safe = !(eps2 >= kTinySeparation2);
+    const bool wr = weights_ready;
     switch (ipt) {
-    case 4: launch_sym_ipt<4>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, st); break;
-    case 2: launch_sym_ipt<2>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, st); break;
-    default: launch_sym_ipt<1>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, st); break;
+    case 4: launch_sym_ipt<4>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, wr, st); break;
+    case 2: launch_sym_ipt<2>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, wr, st); break;
+    default: launch_sym_ipt<1>(safe, s, jp, centre, tile_valid, ntiles, plan, eps2, blocks, wr, st); break;
     }
 }
 
 void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
-                       int ipt, hipStream_t st) {
+                       int ipt, hipStream_t st, bool weights_ready) {
     launch_direct_sym_tiles(s, L.jp, L.centre, nullptr, static_cast<int>(L.ntiles), plan, eps, ipt,
-                            0, st);
+                            0, st, weights_ready);
+}
+
+void launch_sym_weights(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
+                        hipStream_t st) {
+    const int ntiles = static_cast<int>(L.ntiles);
+    if (plan.wt == nullptr || ntiles > plan.ntiles || ntiles < 1) return;
+    hipLaunchKernelGGL(sym_tile_weights_kernel, dim3((ntiles + 3) / 4), dim3(256), 0, st, L.jp,
+                       nullptr, static_cast<int>(s.n), ntiles, static_cast<float>(eps * eps),
+                       plan.wt);
 }
 
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
@@ -913,11 +934,12 @@ void launch_vjp_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& p
 }
 
 void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,
-                           double half_dt, double dt, hipStream_t st) {
+                           double eps, double half_dt, double dt, hipStream_t st) {
     hipLaunchKernelGGL(merge_sym_open_kernel, dim3(plan.ntiles), dim3(kMergeLanes * kMergeGroups),
                        0, st, static_cast<int>(s.n), plan.ntiles, plan.tile_t0, plan.row_start,
                        plan.tile_pbase, plan.ipart, plan.jpart, s.x, s.y, s.z, s.vx, s.vy, s.vz,
-                       s.ax, s.ay, s.az, s.m, half_dt, dt, G, L.jp, L.centre);
+                       s.ax, s.ay, s.az, s.m, half_dt, dt, G, static_cast<float>(eps * eps), L.jp,
+                       L.centre, plan.wt);
 }
 
 }  // namespace gc

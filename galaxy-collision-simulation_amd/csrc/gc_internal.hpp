@@ -119,20 +119,27 @@ int sym_prepare_runs(SymPlan& plan, const int* run_t0, const int* run_nt, int nr
                      hipStream_t st, std::string* err);
 void sym_release(SymPlan& plan);
 // ipt: targets per lane (1, 2, 4)
+// weights_ready: plan.wt already holds the tiles' shared G m for these
+// sources (launch_sym_weights, or the merge that packed them); otherwise a
+// weights kernel runs first
 void launch_direct_sym(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
-                       int ipt, hipStream_t st);
+                       int ipt, hipStream_t st, bool weights_ready = false);
 // tile_valid: valid targets per tile, or null for the first s.n slots;
 // blocks > 0: persistent grid of that many blocks over the items
 void launch_direct_sym_tiles(const DeviceState& s, const float4* jp, const double* centre,
                              const int* tile_valid, int ntiles, const SymPlan& plan, double eps,
-                             int ipt, int blocks, hipStream_t st);
+                             int ipt, int blocks, hipStream_t st, bool weights_ready = false);
+// plan.wt for the packed sources of L (first s.n slots)
+void launch_sym_weights(const DeviceState& s, const MixedLayout& L, const SymPlan& plan, double eps,
+                        hipStream_t st);
 // tile_valid: valid targets per tile, or null for the first s.n slots
 void launch_merge_sym(const DeviceState& s, const SymPlan& plan, const int* tile_valid,
                       bool close_kick, double half_dt, hipStream_t st);
 // merge with the closing kick, then the next step's opening kick and drift
 // and the source tiles of the next force evaluation (first s.n slots)
+// (and plan.wt for those tiles)
 void launch_merge_sym_open(const DeviceState& s, const SymPlan& plan, MixedLayout& L, double G,
-                           double half_dt, double dt, hipStream_t st);
+                           double eps, double half_dt, double dt, hipStream_t st);
 // integrate.hip
 void launch_open_step(const DeviceState& s, double half_dt, double dt, hipStream_t st);
 void launch_kick(const DeviceState& s, double dt, hipStream_t st);
