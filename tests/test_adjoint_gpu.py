@@ -177,6 +177,26 @@ def test_pair_once_vjp_matches_per_target_vjp(gpu, n):
         assert (im - ref_m).abs().max().item() < 1e-5 * sm
 
 
+@pytest.mark.parametrize("ipt", [0, 1, 4])
+def test_pair_once_vjp_equal_mass_tiles(gpu, ipt):
+    """Tiles of one mass each (two masses over the 8 tiles): the pair-once
+    VJP against torch autograd in FP64, at the mixed path's 1e-5 bound,
+    and deterministic."""
+    n = 2048
+    g = torch.Generator().manual_seed(77)
+    x = torch.randn(n, 3, generator=g, dtype=torch.float64).to(_dev())
+    m = torch.full((n,), 3e-4, dtype=torch.float64)
+    m[1024:] = 7e-4  # tiles 4-7 of another mass
+    m = m.to(_dev())
+    lam = torch.randn(n, 3, generator=g, dtype=torch.float64).to(_dev())
+    xr, mr = x.clone().requires_grad_(), m.clone().requires_grad_()
+    (_torch_accel(xr, mr, 0.05) * lam).sum().backward()
+    (gx, gm), (gx2, gm2) = _vjp_mixed_form(gpu, x, m, lam, 0.05, "symmetric", ipt)
+    assert torch.equal(gx, gx2) and torch.equal(gm, gm2)
+    assert (gx - xr.grad).abs().max().item() < 1e-5 * xr.grad.abs().max().item()
+    assert (gm - mr.grad).abs().max().item() < 1e-5 * mr.grad.abs().max().item()
+
+
 def test_mixed_vjp_forms_without_softening(gpu):
     """eps = 0 runs the guarded kernels (a pair closer than 1e-15 adds
     nothing); both mixed forms against the FP64 VJP, and a coincident pair
